@@ -1,0 +1,149 @@
+/*
+ * mtt.h - C ABI of the MI355X-native CFR+/DCFR solver core (libmtt.so).
+ *
+ * The reference (Azerus96/MTT-GPU-Solver) ships CUDA __global__ kernels with
+ * raw device pointers and standalone programs; it has no library API.  Each
+ * entry point below names the reference kernel or program it replaces
+ * (file:line, relative to the reference repository root) and keeps that
+ * kernel's argument meaning, order and memory layouts, adding an explicit
+ * hipStream_t.  Caller-owned device pointers; every function returns an
+ * int status (MTT_OK = 0) instead of exiting the process, and
+ * mtt_last_error() gives the message of the last failure on this thread.
+ *
+ * Layouts kept from the reference:
+ *   card        c = rank*4 + suit, rank 0='2'..12='A'
+ *   combos      lexicographic (c1<c2) combos avoiding the flop: 1176 alive,
+ *               padded to 1216 (mtt_ev_dumper.cu:20-21,149-152)
+ *   per-node    state[(node*P + p)*1216 + combo], P = 2 (HU) or 4 (4-way)
+ *   per-action  state[(infoset*MTT_MAX_ACTIONS + a)*1216 + combo]
+ *   EV dump     float32[flop][4 scenario][2 player][169 class], no header
+ */
+#ifndef MTT_H
+#define MTT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef MTT_NO_HIP_TYPES
+#include <hip/hip_runtime_api.h>
+typedef hipStream_t mtt_stream_t;
+#else
+typedef void* mtt_stream_t;
+#endif
+
+/* ------------------------------------------------------------ status -- */
+enum {
+    MTT_OK = 0,
+    MTT_ERR_ARG = 1,     /* bad argument or size */
+    MTT_ERR_OOM = 2,     /* device or host allocation failed */
+    MTT_ERR_HIP = 3,     /* HIP runtime error */
+    MTT_ERR_TREE = 4,    /* invalid tree (offsets, action counts, types) */
+    MTT_ERR_STATE = 5,   /* call not valid in the handle's current state */
+    MTT_ERR_IO = 6       /* file could not be written */
+};
+const char* mtt_last_error(void);
+const char* mtt_version(void);
+
+/* ------------------------------------------------------------ layouts -- */
+#define MTT_N_ALIVE_COMBOS 1176
+#define MTT_N_PADDED_COMBOS 1216
+#define MTT_N_GLOBAL_COMBOS 1326
+#define MTT_N_CANONICAL_FLOPS 1755
+#define MTT_N_CLASSES 169
+/* Action stride of per-action state.  The reference uses 3
+ * (honest_multistreet_solver.cu:12); the bet menus here need 4. */
+#define MTT_MAX_ACTIONS 4
+
+/* honest_multistreet_solver.cu:14 / multiway_4way_engine.cu:18 */
+enum { MTT_NODE_DECISION = 0, MTT_NODE_FOLD = 1, MTT_NODE_SHOWDOWN = 2, MTT_NODE_CHANCE = 3 };
+
+/* HU tree node, honest_multistreet_solver.cu:16-21 (32 bytes, same offsets).
+ * river_idx sits in the reference's padding at offset 14: for a chance node
+ * turn_idx < 0 deals the turn (children = the 49 flop-deck cards in order)
+ * and turn_idx >= 0 deals the river (48 children, deck order without the
+ * turn); a showdown with river_idx < 0 is a turn all-in (turn matrix
+ * turn_idx), otherwise the river board (turn_idx, river_idx), both deck
+ * indices; a single-board river game uses turn_idx = river_idx = -1. */
+typedef struct mtt_flat_node {
+    uint8_t type, player, num_actions, active_mask;
+    int32_t children_offset, infoset_offset;
+    int16_t turn_idx, river_idx;
+    float pot_size, committed[2];
+} __attribute__((aligned(16))) mtt_flat_node;
+
+/* 4-way tree node, multiway_4way_engine.cu:20-24 (32 bytes, same offsets). */
+typedef struct mtt_flat_node4 {
+    uint8_t type, player, num_actions, active_mask;
+    int32_t children_offset, infoset_offset;
+    float pot_size, committed[4];
+} __attribute__((aligned(16))) mtt_flat_node4;
+
+/* mtt_ev_dumper.cu:24 (16 bytes). */
+typedef struct mtt_alive_combo {
+    uint8_t c1, c2;
+    uint64_t mask;
+} __attribute__((aligned(8))) mtt_alive_combo;
+
+/* ---------------------------------------------- cards and evaluator -- */
+/* fast_7card_score, mtt_ev_dumper.cu:53-83 (host side). */
+uint32_t mtt_eval7(const uint8_t cards[7]);
+/* Host batch of the same evaluator: n hands of 7 cards. */
+int mtt_eval7_host_batch(const uint8_t* cards, int64_t n, uint32_t* out);
+/* Device batch: d_cards n*7 bytes, d_out n scores. */
+int mtt_eval7_batch(const uint8_t* d_cards, int64_t n, uint32_t* d_out, mtt_stream_t stream);
+/* get_canonical_class, mtt_ev_dumper.cu:26-32. */
+int mtt_canonical_class(int c1, int c2);
+/* generate_all_1755_flops, mtt_ev_dumper.cu:34-51: out 1755*4 bytes
+ * {c1,c2,c3,orbit_weight}; returns the flop count. */
+int mtt_canonical_flops(uint8_t* out);
+/* Lexicographic index of combo {a,b} among 1326. */
+int mtt_combo_index(int a, int b);
+
+/* ------------------------------------------------- equity builders -- */
+enum { MTT_LAYOUT_GLOBAL1326 = 0, MTT_LAYOUT_ALIVE1216 = 1 };
+/* Exhaustive win/tie counts for flops (new, config-2 builder; evaluator
+ * mtt_ev_dumper.cu:53-83).  For each flop f and non-conflicting pair (i,j):
+ * W[i][j] = runouts (of the C(45,2)=990 avoiding both hands) where i beats
+ * j, T[i][j] = ties; blocked pairs are 0.  d_W, d_T: int32 n_flops*S*S with
+ * S = 1326 (global combo index) or 1216 (flop-alive index).  flops: host
+ * array of n_flops*3 cards.  Bit-exact against the reference evaluator. */
+int mtt_flop_counts(const uint8_t* flops, int n_flops, int32_t* d_W, int32_t* d_T,
+                    int layout, mtt_stream_t stream);
+
+/* kernel_evaluate_flop_ev, mtt_ev_dumper.cu:85-116: subsampled per-combo EV
+ * of one flop against reach_p1/reach_p0 (1216 floats).  d_combos 1216
+ * entries, d_deck 49 cards, d_flop 3 cards, outputs 1216 floats (1176 used). */
+int mtt_evaluate_flop_ev(const mtt_alive_combo* d_combos, const uint8_t* d_deck,
+                         const uint8_t* d_flop, const float* d_reach_p0,
+                         const float* d_reach_p1, float* d_out_ev_p0, float* d_out_ev_p1,
+                         mtt_stream_t stream);
+/* EV-dump tensor for canonical flops [first, first+count) into host memory
+ * out (count*1352 floats), computed on the GPU with the uniform 1/1176
+ * reaches and host-identical class/scenario accumulation
+ * (mtt_ev_dumper.cu:141-168). */
+int mtt_ev_dump_tensor(int first, int count, float* out);
+/* bin/ev_dumper [--limit N] (mtt_ev_dumper.cu:118-176): writes the tensor of
+ * the first `limit` flops to `path` (the reference always writes
+ * "mtt_ev_dump_1755.bin" in the working directory; NULL means that). */
+int mtt_ev_dump(int limit, const char* path);
+
+/* kernel_gen_49_turn_matrices, honest_multistreet_solver.cu:56-86:
+ * d_out[49][1216][1216] FP32, E[t][j][h] = (W + T/2)/n over the n rivers
+ * avoiding both hands and the turn; blocked and padded entries 0. */
+int mtt_gen_turn_matrices(const mtt_alive_combo* d_combos, const uint8_t* d_deck,
+                          const uint8_t* d_flop, float* d_out, mtt_stream_t stream);
+/* River equity matrices of the same layout for boards (turn t, river r):
+ * d_out[n_boards][1216][1216], E in {0, 1/2, 1}; boards given as host pairs
+ * of deck indices. */
+int mtt_gen_river_matrices(const mtt_alive_combo* d_combos, const uint8_t* d_deck,
+                           const uint8_t* d_flop, const int16_t* boards, int n_boards,
+                           float* d_out, mtt_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MTT_H */

@@ -1,0 +1,48 @@
+/*
+ * CPU oracle for the MI355X CFR+/DCFR solver core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  liboracle.so restates the reference's algorithms
+ * on the CPU so that the HIP kernels can be checked against it.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product never links, calls or falls back to it.
+ *
+ * Parity pins (what checks each restated function; details in the tests):
+ *   or_eval7            C(52,7) category counts (closed form), distinct-value
+ *                       counts, hand-picked quirk cases, and the committed EV
+ *                       dump (mtt_ev_dumper.cu output) within 2e-6.
+ *   or_canonical_*      orbit weights sum to C(52,3); class map covers 1326.
+ *   or_ev_dump          the reference's own data/mtt_ev_dump_1755.bin slice.
+ *   or_flop_counts      W+W^T+T = 990 on unblocked pairs, T symmetric, brute
+ *                       force on sampled pairs, suit-permutation invariance.
+ *   or_turn_matrix      equals the mean of river sign matrices (brute force).
+ *   HU CFR (or_cfr.c)   river clairvoyance toy game (closed form), brute-force
+ *                       pair enumeration of showdown/fold values, zero-sum
+ *                       accounting, dense vs sorted showdown.
+ *   best response       exploitability of toy games (closed form), >= 0.
+ *   multiway showdown   O(N^4) brute force vs prefix scan.
+ *   DCFR weights        telescoping closed form.
+ *   ICM                 2-player closed form, symmetry, sum = prize pool,
+ *                       brute-force permutation enumeration.
+ */
+#ifndef MTT_ORACLE_H
+#define MTT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+uint32_t or_eval7(const uint8_t c[7]);
+void or_eval7_batch(const uint8_t* cards, int64_t n, uint32_t* out);
+void or_category_counts(uint64_t out[9]);
+int or_canonical_class(int c1, int c2);
+int or_canonical_flops(uint8_t* out);
+void or_ev_dump(int first, int count, float* out);
+int or_combo_index(int a, int b);
+void or_flop_counts(const uint8_t flop[3], int32_t* W, int32_t* T);
+void or_turn_matrix(const uint8_t flop[3], int turn_card, float* E);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,133 @@
+"""GPU parity of the evaluator and equity builders (through the C ABI)
+against the CPU oracle and the reference's committed EV dump."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _alive(flop):
+    fl = set(int(c) for c in flop)
+    return [(a, b) for a, b in itertools.combinations(range(52), 2) if a not in fl and b not in fl]
+
+
+def _combo_struct(mtt, flop):
+    arr = (mtt.AliveCombo * mtt.N_PAD)()
+    for i, (a, b) in enumerate(_alive(flop)):
+        arr[i].c1, arr[i].c2, arr[i].mask = a, b, (1 << a) | (1 << b)
+    return np.frombuffer(bytes(arr), np.uint8)
+
+
+def _deck(flop):
+    return np.array([c for c in range(52) if c not in set(int(x) for x in flop)], np.uint8)
+
+
+def test_gpu_eval_batch(mtt, oracle, torch_dev):
+    import torch
+    rng = np.random.default_rng(11)
+    cards = np.argsort(rng.random((1 << 20, 52)), axis=1)[:, :7].astype(np.uint8)
+    want = oracle.eval7_batch(cards)
+    d_cards = torch.from_numpy(cards).to(torch_dev)
+    d_out = torch.zeros(cards.shape[0], dtype=torch.int32, device=torch_dev)
+    mtt.call("mtt_eval7_batch", d_cards.data_ptr(), cards.shape[0], d_out.data_ptr(), None)
+    torch.cuda.synchronize()
+    got = d_out.cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("flop_idx", [0, 40, 1000])
+def test_gpu_flop_counts_bitexact(mtt, oracle, torch_dev, flop_idx):
+    import torch
+    flops = oracle.canonical_flops()
+    flop = flops[flop_idx, :3].copy()
+    print("flop", flop_idx, flop.tolist(), flush=True)
+    W0, T0 = oracle.flop_counts(flop)
+    dW = torch.full((1326 * 1326,), -7, dtype=torch.int32, device=torch_dev)
+    dT = torch.full((1326 * 1326,), -7, dtype=torch.int32, device=torch_dev)
+    mtt.call("mtt_flop_counts", flop.ctypes.data, 1, dW.data_ptr(), dT.data_ptr(), mtt.LAYOUT_GLOBAL1326, None)
+    torch.cuda.synchronize()
+    W = dW.cpu().numpy().reshape(1326, 1326)
+    T = dT.cpu().numpy().reshape(1326, 1326)
+    assert np.array_equal(W, W0)
+    assert np.array_equal(T, T0)
+
+
+def test_gpu_flop_counts_batch_and_alive_layout(mtt, oracle, torch_dev):
+    import torch
+    flops = oracle.canonical_flops()
+    sel = [3, 500, 1754]
+    fl = np.ascontiguousarray(flops[sel, :3])
+    S = mtt.N_PAD
+    dW = torch.zeros(len(sel) * S * S, dtype=torch.int32, device=torch_dev)
+    dT = torch.zeros(len(sel) * S * S, dtype=torch.int32, device=torch_dev)
+    mtt.call("mtt_flop_counts", fl.ctypes.data, len(sel), dW.data_ptr(), dT.data_ptr(), mtt.LAYOUT_ALIVE1216, None)
+    torch.cuda.synchronize()
+    W = dW.cpu().numpy().reshape(len(sel), S, S)
+    T = dT.cpu().numpy().reshape(len(sel), S, S)
+    for k, fi in enumerate(sel):
+        W0, T0 = oracle.flop_counts(fl[k])
+        g = [oracle.lib().or_combo_index(a, b) for a, b in _alive(fl[k])]
+        A = np.ix_(g, g)
+        assert np.array_equal(W[k, :1176, :1176], W0[A])
+        assert np.array_equal(T[k, :1176, :1176], T0[A])
+        assert not W[k, 1176:].any() and not W[k, :, 1176:].any()
+
+
+def test_gpu_ev_dump_matches_reference(mtt, torch_dev):
+    z = np.load(os.path.join(GOLDEN, "ev_dump_slice.npz"))
+    idx, vals = z["flop_index"], z["values"]
+    for k, f in enumerate(idx[:16]):
+        out = np.zeros((1, 4, 2, 169), np.float32)
+        mtt.call("mtt_ev_dump_tensor", int(f), 1, out.ctypes.data)
+        assert np.array_equal(out[0], vals[k]), f
+
+
+def test_gpu_turn_matrices(mtt, oracle, torch_dev):
+    import torch
+    flop = np.array([50, 33, 2], np.uint8)
+    combos = torch.from_numpy(_combo_struct(mtt, flop)).to(torch_dev)
+    deck = torch.from_numpy(_deck(flop)).to(torch_dev)
+    dfl = torch.from_numpy(flop).to(torch_dev)
+    out = torch.zeros(49 * 1216 * 1216, dtype=torch.float32, device=torch_dev)
+    mtt.call("mtt_gen_turn_matrices", combos.data_ptr(), deck.data_ptr(), dfl.data_ptr(), out.data_ptr(), None)
+    torch.cuda.synchronize()
+    E = out.view(49, 1216, 1216)
+    dk = _deck(flop)
+    for t in (0, 17, 48):
+        want = oracle.turn_matrix(flop, int(dk[t]))
+        assert np.array_equal(E[t].cpu().numpy(), want), t
+
+
+def test_gpu_river_matrices(mtt, oracle, torch_dev):
+    import torch
+    flop = np.array([0, 21, 42], np.uint8)
+    combos = torch.from_numpy(_combo_struct(mtt, flop)).to(torch_dev)
+    dk = _deck(flop)
+    deck = torch.from_numpy(dk).to(torch_dev)
+    dfl = torch.from_numpy(flop).to(torch_dev)
+    boards = np.array([[0, 1], [10, 40], [48, 3]], np.int16)
+    out = torch.zeros(len(boards) * 1216 * 1216, dtype=torch.float32, device=torch_dev)
+    mtt.call("mtt_gen_river_matrices", combos.data_ptr(), deck.data_ptr(), dfl.data_ptr(),
+             boards.ctypes.data, len(boards), out.data_ptr(), None)
+    torch.cuda.synchronize()
+    E = out.view(len(boards), 1216, 1216).cpu().numpy()
+    al = _alive(flop)
+    for k, (t, r) in enumerate(boards):
+        bc = {int(dk[t]), int(dk[r])}
+        cards = np.array([[*flop, dk[t], dk[r], a, b] for a, b in al], np.uint8)
+        s = oracle.eval7_batch(cards).astype(np.int64)
+        ok = np.array([a not in bc and b not in bc for a, b in al])
+        m = np.array([(1 << a) | (1 << b) for a, b in al], dtype=object)
+        blk = np.array([[bool(m[i] & m[j]) for j in range(len(al))] for i in range(len(al))])
+        # E[j][h]: 1 if h beats j, 1/2 tie
+        want = np.where(s[None, :] > s[:, None], 1.0, np.where(s[None, :] == s[:, None], 0.5, 0.0)).astype(np.float32)
+        want[blk] = 0
+        want[~ok, :] = 0
+        want[:, ~ok] = 0
+        assert np.array_equal(E[k, :1176, :1176], want)
+        assert not E[k, 1176:].any() and not E[k, :, 1176:].any()
