@@ -24,12 +24,15 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#ifndef MTT_NO_HIP_TYPES
+#include <hip/hip_runtime_api.h>
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
 
 #ifndef MTT_NO_HIP_TYPES
-#include <hip/hip_runtime_api.h>
 typedef hipStream_t mtt_stream_t;
 #else
 typedef void* mtt_stream_t;
@@ -62,12 +65,15 @@ const char* mtt_version(void);
 enum { MTT_NODE_DECISION = 0, MTT_NODE_FOLD = 1, MTT_NODE_SHOWDOWN = 2, MTT_NODE_CHANCE = 3 };
 
 /* HU tree node, honest_multistreet_solver.cu:16-21 (32 bytes, same offsets).
- * river_idx sits in the reference's padding at offset 14: for a chance node
- * turn_idx < 0 deals the turn (children = the 49 flop-deck cards in order)
- * and turn_idx >= 0 deals the river (48 children, deck order without the
- * turn); a showdown with river_idx < 0 is a turn all-in (turn matrix
- * turn_idx), otherwise the river board (turn_idx, river_idx), both deck
- * indices; a single-board river game uses turn_idx = river_idx = -1. */
+ * Chance nodes deal from the deck of cards not on the root board, in order:
+ * turn_idx < 0 deals every deck card (num_actions children), turn_idx >= 0
+ * is the deck index of the turn already dealt, which is skipped.  Showdown
+ * nodes keep the reference meaning of turn_idx: the index of their equity
+ * matrix.  river_idx (in the reference's padding at offset 14) records the
+ * cards dealt below the root on the node's path: 0 none, d1+1 for one deck
+ * index d1, (d1+1) + 64*(d2+1) for two; combos holding a dealt card are
+ * dead at the node.  Decision, fold and chance nodes set turn_idx to d1 (or
+ * -1).  Fold: active_mask has the winner's bit (0b01 = player 0 wins). */
 typedef struct mtt_flat_node {
     uint8_t type, player, num_actions, active_mask;
     int32_t children_offset, infoset_offset;
@@ -142,6 +148,82 @@ int mtt_gen_turn_matrices(const mtt_alive_combo* d_combos, const uint8_t* d_deck
 int mtt_gen_river_matrices(const mtt_alive_combo* d_combos, const uint8_t* d_deck,
                            const uint8_t* d_flop, const int16_t* boards, int n_boards,
                            float* d_out, mtt_stream_t stream);
+
+/* ------------------------------------------------------- HU game trees -- */
+/* Bet menu of one street.  The reference has no bet-menu config in code
+ * (menus exist only as README.md:10 prose and MAX_ACTIONS); this struct
+ * carries the menus the benchmark configurations describe.  Unopened: check,
+ * each size (pot fraction) that stays below the stack, all-in (if allin).
+ * Facing a bet: fold, call and, while fewer than raise_cap raises were made
+ * this street, a raise to raise_mult x the bet (0 = none) and (if allin) an
+ * all-in raise.  At most MTT_MAX_ACTIONS actions per node. */
+typedef struct mtt_street_menu {
+    int n_sizes;
+    float sizes[3];
+    float raise_mult;
+    int raise_cap;
+    int allin;          /* 1: offer all-in (unopened, and as a raise) */
+} mtt_street_menu;
+
+/* Heads-up subgame.  n_board cards are known at the root; root_chance = 1
+ * makes the root a chance node dealing the next card (the turn for a flop
+ * board).  Streets are 0 flop, 1 turn, 2 river; betting runs from the first
+ * street to last_street, then showdown over any cards still to come.
+ * pot = dead money at the root, stack = effective stack behind. */
+typedef struct mtt_hu_game {
+    uint8_t board[5];
+    int n_board;
+    int root_chance;
+    int last_street;
+    float pot, stack;
+    mtt_street_menu menu[3];
+} mtt_hu_game;
+
+/* Size of the flat tree (honest_multistreet_solver.cu:16-21 nodes; children
+ * contiguous from children_offset, decision infosets numbered in order). */
+int mtt_hu_tree_size(const mtt_hu_game* game, int* n_nodes, int* n_infosets, int* n_matrices);
+int mtt_hu_tree_build(const mtt_hu_game* game, mtt_flat_node* nodes, int capacity, int* n_nodes);
+
+/* ---------------------------------------------------- seeded inputs -- */
+/* Project PRNG (SplitMix64): n cards without replacement. */
+int mtt_seeded_cards(uint64_t seed, int n, uint8_t* out);
+/* Board (n_board cards) then player 0 and player 1 weights over the 1326
+ * combos from one seed; dist 0 = Uniform[0,1] on unblocked combos, 1 =
+ * Dirichlet(alpha) over all combos with blocked ones zeroed; normalised. */
+int mtt_seeded_inputs(uint64_t seed, int n_board, int dist, float alpha, uint8_t* board_out,
+                      float* range0_out, float* range1_out);
+
+/* -------------------------------------------------------- HU solver -- */
+/* Solve entry points (new: the reference's HU main is a stub,
+ * honest_multistreet_solver.cu:251-254).  The handle owns all device state
+ * on the current device; handles are not shared across threads. */
+enum { MTT_MODE_PLAIN_FP32 = 0, MTT_MODE_BF16_SPLIT = 1 };
+enum { MTT_SCHEME_CFR_PLUS = 0, MTT_SCHEME_DCFR = 1 };
+typedef struct mtt_solver_cfg {
+    int mode;          /* MTT_MODE_* */
+    int scheme;        /* MTT_SCHEME_*: CFR+ (R floored, weights t) or DCFR */
+    float alpha, beta, gamma;  /* DCFR discounts; beta <= -1e30 floors negatives */
+} mtt_solver_cfg;
+typedef struct mtt_solver mtt_solver;
+
+/* ranges: 1326 floats each (global combo order), blocked combos ignored. */
+int mtt_solver_create(const mtt_hu_game* game, const float* range0, const float* range1,
+                      const mtt_solver_cfg* cfg, mtt_solver** out);
+int mtt_solver_run(mtt_solver* h, int iters);
+int mtt_solver_iterations(const mtt_solver* h);
+/* Average strategy in the reference layout [infoset][MTT_MAX_ACTIONS][1216]
+ * (normalised, uniform where the strategy sum is 0); n_floats must be at
+ * least n_infosets*MTT_MAX_ACTIONS*1216. */
+int mtt_solver_get_strategy(mtt_solver* h, float* host_out, size_t n_floats);
+/* FP64 best response on the GPU against the average strategy:
+ * out[0] = BR0 value, out[1] = BR1 value, out[2], out[3] = the profile's
+ * own values, out[4] = pair mass M, out[5] = exploitability in % of the
+ * starting pot = 100*(BR0 + BR1 - pot*M) / (2*M*pot). */
+int mtt_solver_exploitability_fp64(mtt_solver* h, double* out);
+/* info: [0] nodes [1] infosets [2] alive combos [3] device bytes
+ * [4] decision nodes [5] showdown nodes [6] fold nodes [7] matrices */
+int mtt_solver_info(mtt_solver* h, int64_t* info);
+int mtt_solver_destroy(mtt_solver* h);
 
 #ifdef __cplusplus
 }

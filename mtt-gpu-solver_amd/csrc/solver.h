@@ -1,0 +1,66 @@
+// Solver handle internals shared by the plain, batched and best-response
+// code paths.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "hu_kernels.h"
+#include "hu_tree.h"
+
+struct mtt_solver;  // opaque in the C ABI
+
+namespace mtt {
+
+struct BatchedState;  // hu_batched.hip
+
+struct SolverImpl {
+    mtt_hu_game game;
+    mtt_solver_cfg cfg;
+    HuTree tree;
+    hipStream_t stream = nullptr;
+    // root-alive combos
+    int n_alive = 0;
+    uint8_t c1[1326], c2[1326];
+    int gidx[1326];
+    std::vector<uint8_t> deck;            // cards not on the root board
+    std::vector<float> range[2];          // root-alive layout (n_alive)
+    double pair_mass = 0;                 // sum_ij r0 r1 nb
+    // tree topology
+    std::vector<std::vector<int>> depth_lists;  // decision + chance nodes
+    std::vector<int> terminals;
+    int iters_done = 0;
+    // device (plain + shared)
+    DeviceArena arena;
+    mtt_flat_node* d_nodes = nullptr;
+    uint8_t* d_deck = nullptr;
+    mtt_alive_combo* d_combos = nullptr;
+    std::vector<int*> d_depth;            // per depth
+    float* d_reach = nullptr;             // [nodes][2][1216]
+    float* d_cfv = nullptr;
+    float* d_R = nullptr;                 // [infosets][A][1216]
+    float* d_S = nullptr;
+    float* d_sigma = nullptr;
+    float* d_mats = nullptr;              // [n_mat][1216][1216]
+    float* d_sums = nullptr;              // [terms][2][64]
+    PlainTerminalPlan plan;
+    // batched path
+    BatchedState* batched = nullptr;       // owned; freed by batched_destroy
+    ~SolverImpl();
+};
+
+int solver_gen_matrices(SolverImpl& s, float* d_out);
+int solver_alloc_plain(SolverImpl& s);
+int solver_run_plain(SolverImpl& s, int iters);
+void iteration_weights(const mtt_solver_cfg& cfg, int t, float& wp, float& wn, float& ws);
+
+// batched path (hu_batched.hip)
+int batched_create(SolverImpl& s);
+int batched_run(SolverImpl& s, int iters);
+int batched_strategy_sums(SolverImpl& s, float* d_S_ref);  // scatter into the reference layout
+void batched_destroy(BatchedState* b);
+
+// FP64 best response (br.hip): exploitability of the average strategy held
+// (normalised, reference layout) in d_strat.
+int br_exploitability(SolverImpl& s, const float* d_strat, double* out6);
+
+}  // namespace mtt

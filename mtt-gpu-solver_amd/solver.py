@@ -1,0 +1,186 @@
+"""Host-side helpers over the solver C ABI: game descriptions of the
+benchmark configurations, seeded inputs, and a small Solver wrapper.
+
+Loaded by the tests and the benchmark next to mtt.py (same directory).
+"""
+import ctypes
+import importlib.util
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+if "mtt" in sys.modules and hasattr(sys.modules["mtt"], "lib"):
+    mtt = sys.modules["mtt"]
+else:
+    _spec = importlib.util.spec_from_file_location("mtt", os.path.join(_HERE, "mtt.py"))
+    mtt = importlib.util.module_from_spec(_spec)
+    sys.modules["mtt"] = mtt
+    _spec.loader.exec_module(mtt)
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+
+MODE_PLAIN_FP32, MODE_BF16_SPLIT = 0, 1
+SCHEME_CFR_PLUS, SCHEME_DCFR = 0, 1
+
+
+class Menu(ctypes.Structure):
+    _fields_ = [("n_sizes", ctypes.c_int), ("sizes", ctypes.c_float * 3), ("raise_mult", ctypes.c_float),
+                ("raise_cap", ctypes.c_int), ("allin", ctypes.c_int)]
+
+
+class Game(ctypes.Structure):
+    _fields_ = [("board", ctypes.c_uint8 * 5), ("n_board", ctypes.c_int), ("root_chance", ctypes.c_int),
+                ("last_street", ctypes.c_int), ("pot", ctypes.c_float), ("stack", ctypes.c_float),
+                ("menu", Menu * 3)]
+
+
+class SolverCfg(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("scheme", ctypes.c_int), ("alpha", ctypes.c_float),
+                ("beta", ctypes.c_float), ("gamma", ctypes.c_float)]
+
+
+for name, args in {
+    "mtt_seeded_cards": [ctypes.c_uint64, _I, _P],
+    "mtt_seeded_inputs": [ctypes.c_uint64, _I, _I, ctypes.c_float, _P, _P, _P],
+    "mtt_hu_tree_size": [_P, _P, _P, _P],
+    "mtt_hu_tree_build": [_P, _P, _I, _P],
+    "mtt_solver_create": [_P, _P, _P, _P, _P],
+    "mtt_solver_run": [_P, _I],
+    "mtt_solver_iterations": [_P],
+    "mtt_solver_get_strategy": [_P, _P, ctypes.c_size_t],
+    "mtt_solver_exploitability_fp64": [_P, _P],
+    "mtt_solver_info": [_P, _P],
+    "mtt_solver_destroy": [_P],
+    "mtt_hu_forward_sweep": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P],
+    "mtt_hu_evaluate_terminals": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P],
+    "mtt_hu_backward_sweep": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_float, _I, _P],
+    "mtt_hu_backward_sweep_dcfr": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_float, _I, _P],
+}.items():
+    mtt.register(name, args)
+
+FLAT_DTYPE = np.dtype([("type", "u1"), ("player", "u1"), ("num_actions", "u1"), ("active_mask", "u1"),
+                       ("children_offset", "<i4"), ("infoset_offset", "<i4"), ("turn_idx", "<i2"),
+                       ("river_idx", "<i2"), ("pot_size", "<f4"), ("committed", "<f4", (2,)), ("_pad", "V4")])
+assert FLAT_DTYPE.itemsize == 32
+
+
+def menu(sizes=(), raise_mult=0.0, raise_cap=0, allin=1):
+    m = Menu()
+    m.n_sizes = len(sizes)
+    for k, s in enumerate(sizes):
+        m.sizes[k] = s
+    m.raise_mult = raise_mult
+    m.raise_cap = raise_cap
+    m.allin = allin
+    return m
+
+
+def seeded_inputs(seed, n_board, dist, alpha=1.0):
+    board = np.zeros(5, np.uint8)
+    r0 = np.zeros(1326, np.float32)
+    r1 = np.zeros(1326, np.float32)
+    mtt.call("mtt_seeded_inputs", seed, n_board, dist, alpha, board.ctypes.data, r0.ctypes.data, r1.ctypes.data)
+    return board[:n_board].copy(), r0, r1
+
+
+def make_game(board, pot, stack, menus, root_chance=0, last_street=2):
+    g = Game()
+    for k, c in enumerate(board):
+        g.board[k] = int(c)
+    g.n_board = len(board)
+    g.root_chance = root_chance
+    g.last_street = last_street
+    g.pot = pot
+    g.stack = stack
+    for s, m in menus.items():
+        g.menu[s] = m
+    return g
+
+
+def config1(seed=0):
+    """HU river: 5-card seeded board, uniform[0,1] ranges over unblocked
+    combos, pot 10, stack 20, bet 50%/100%/all-in, one raise (all-in)."""
+    board, r0, r1 = seeded_inputs(seed, 5, 0)
+    g = make_game(board, 10.0, 20.0, {2: menu((0.5, 1.0), 0.0, 1, 1)}, 0, 2)
+    return g, r0, r1
+
+
+def config3(seed=0, menu_turn=None, menu_river=None):
+    """HU turn+river: seeded flop, root deals the turn; pot 6, stack 40; per
+    street 33%/75%, one 2.5x raise, all-in; Dirichlet(0.5) ranges."""
+    board, r0, r1 = seeded_inputs(seed, 3, 1, 0.5)
+    m = menu((0.33, 0.75), 2.5, 1, 1)
+    g = make_game(board, 6.0, 40.0, {1: menu_turn or m, 2: menu_river or m}, 1, 2)
+    return g, r0, r1
+
+
+def config3_small(seed=0):
+    """Same street structure as config 3 with an all-in-only menu (for
+    oracle-sized multi-street parity)."""
+    m = menu((), 0.0, 1, 1)
+    return config3(seed, m, m)
+
+
+def tree(game):
+    n, ni, nm = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    mtt.call("mtt_hu_tree_size", ctypes.byref(game), ctypes.byref(n), ctypes.byref(ni), ctypes.byref(nm))
+    arr = np.zeros(n.value, FLAT_DTYPE)
+    nn = ctypes.c_int()
+    mtt.call("mtt_hu_tree_build", ctypes.byref(game), arr.ctypes.data, n.value, ctypes.byref(nn))
+    return arr, ni.value, nm.value
+
+
+def alive_index(board):
+    """Root-alive combo list (lexicographic) and its 1326 global indices."""
+    bs = set(int(c) for c in board)
+    pairs = [(a, b) for a in range(52) for b in range(a + 1, 52)]
+    alive = [(k, a, b) for k, (a, b) in enumerate(pairs) if a not in bs and b not in bs]
+    return np.array([k for k, _, _ in alive]), np.array([[a, b] for _, a, b in alive], np.uint8)
+
+
+class Solver:
+    def __init__(self, game, r0, r1, mode=MODE_PLAIN_FP32, scheme=SCHEME_CFR_PLUS, alpha=1.5, beta=0.0, gamma=2.0):
+        self.game = game
+        cfg = SolverCfg(mode, scheme, alpha, beta, gamma)
+        h = ctypes.c_void_p()
+        r0 = np.ascontiguousarray(r0, np.float32)
+        r1 = np.ascontiguousarray(r1, np.float32)
+        mtt.call("mtt_solver_create", ctypes.byref(game), r0.ctypes.data, r1.ctypes.data, ctypes.byref(cfg),
+                 ctypes.byref(h))
+        self.h = h
+        info = np.zeros(8, np.int64)
+        mtt.call("mtt_solver_info", self.h, info.ctypes.data)
+        self.info = info
+
+    def run(self, iters):
+        mtt.call("mtt_solver_run", self.h, iters)
+
+    @property
+    def iterations(self):
+        return mtt.lib().mtt_solver_iterations(self.h)
+
+    def strategy(self):
+        n = int(self.info[1]) * mtt.MAX_ACTIONS * mtt.N_PAD
+        out = np.zeros(n, np.float32)
+        mtt.call("mtt_solver_get_strategy", self.h, out.ctypes.data, n)
+        return out
+
+    def exploitability(self):
+        out = np.zeros(6, np.float64)
+        mtt.call("mtt_solver_exploitability_fp64", self.h, out.ctypes.data)
+        return out
+
+    def close(self):
+        if self.h:
+            mtt.lib().mtt_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

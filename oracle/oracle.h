@@ -32,6 +32,17 @@
 extern "C" {
 #endif
 
+/* Tree node layout of the reference (honest_multistreet_solver.cu:16-21),
+ * restated here byte for byte; river_idx occupies the padding at offset 14. */
+#define MTT_MAX_ACTIONS 4
+enum { MTT_NODE_DECISION = 0, MTT_NODE_FOLD = 1, MTT_NODE_SHOWDOWN = 2, MTT_NODE_CHANCE = 3 };
+typedef struct mtt_flat_node {
+    uint8_t type, player, num_actions, active_mask;
+    int32_t children_offset, infoset_offset;
+    int16_t turn_idx, river_idx;
+    float pot_size, committed[2];
+} __attribute__((aligned(16))) mtt_flat_node;
+
 uint32_t or_eval7(const uint8_t c[7]);
 void or_eval7_batch(const uint8_t* cards, int64_t n, uint32_t* out);
 void or_category_counts(uint64_t out[9]);
@@ -41,6 +52,19 @@ void or_ev_dump(int first, int count, float* out);
 int or_combo_index(int a, int b);
 void or_flop_counts(const uint8_t flop[3], int32_t* W, int32_t* T);
 void or_turn_matrix(const uint8_t flop[3], int turn_card, float* E);
+
+/* heads-up CFR (or_cfr.c) */
+typedef struct or_hu_s or_hu;
+or_hu* or_hu_create(const mtt_flat_node* nodes, int n_nodes, int n_infosets, const uint8_t* board, int n_board,
+                    const float* range0, const float* range1, int precision);
+void or_hu_destroy(or_hu* h);
+int or_hu_n_combos(const or_hu* h);
+void or_hu_iterate(or_hu* h, int iters, int scheme, double alpha, double beta, double gamma);
+void or_hu_avg_strategy(const or_hu* h, float* out);
+void or_hu_state(const or_hu* h, float* R_out, float* S_out);
+void or_hu_exploitability(const or_hu* h, const float* strat, double pot, double* out);
+void or_hu_terminal(or_hu* h, int node, const uint8_t* dealt, int nd, const double* r0, const double* r1,
+                    double* v0, double* v1);
 
 #ifdef __cplusplus
 }

@@ -77,3 +77,68 @@ def turn_matrix(flop, turn_card):
     E = np.zeros((1216, 1216), np.float32)
     lib().or_turn_matrix(_p(f), int(turn_card), _p(E))
     return E
+
+
+class HU:
+    """Oracle heads-up CFR over a FlatNode tree (numpy structured array with
+    the 32-byte node layout).  precision 0 = FP64, 1 = FP32."""
+
+    def __init__(self, nodes, n_infosets, board, r0_alive, r1_alive, precision=0):
+        L = lib()
+        L.or_hu_create.restype = _P
+        L.or_hu_create.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, _P, _P, ctypes.c_int]
+        L.or_hu_destroy.argtypes = [_P]
+        L.or_hu_n_combos.argtypes = [_P]
+        L.or_hu_iterate.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.or_hu_avg_strategy.argtypes = [_P, _P]
+        L.or_hu_state.argtypes = [_P, _P, _P]
+        L.or_hu_exploitability.argtypes = [_P, _P, ctypes.c_double, _P]
+        L.or_hu_terminal.argtypes = [_P, ctypes.c_int, _P, ctypes.c_int, _P, _P, _P, _P]
+        self.nodes = np.ascontiguousarray(nodes)
+        self.n_infosets = n_infosets
+        self.board = np.ascontiguousarray(board, np.uint8)
+        self.r0 = np.ascontiguousarray(r0_alive, np.float32)
+        self.r1 = np.ascontiguousarray(r1_alive, np.float32)
+        self.h = L.or_hu_create(self.nodes.ctypes.data, len(self.nodes), n_infosets, self.board.ctypes.data,
+                                len(self.board), self.r0.ctypes.data, self.r1.ctypes.data, precision)
+        self.N = L.or_hu_n_combos(self.h)
+
+    def iterate(self, n, scheme=0, alpha=1.5, beta=0.0, gamma=2.0):
+        lib().or_hu_iterate(self.h, n, scheme, alpha, beta, gamma)
+
+    def strategy(self):
+        out = np.zeros(self.n_infosets * 4 * 1216, np.float32)
+        lib().or_hu_avg_strategy(self.h, _p(out))
+        return out
+
+    def state(self):
+        R = np.zeros(self.n_infosets * 4 * 1216, np.float32)
+        S = np.zeros_like(R)
+        lib().or_hu_state(self.h, _p(R), _p(S))
+        return R, S
+
+    def exploitability(self, strat, pot):
+        out = np.zeros(6, np.float64)
+        s = np.ascontiguousarray(strat, np.float32)
+        lib().or_hu_exploitability(self.h, _p(s), pot, _p(out))
+        return out
+
+    def terminal(self, node, dealt, r0, r1):
+        d = np.ascontiguousarray(dealt, np.uint8)
+        a = np.ascontiguousarray(r0, np.float64)
+        b = np.ascontiguousarray(r1, np.float64)
+        v0 = np.zeros(1216)
+        v1 = np.zeros(1216)
+        lib().or_hu_terminal(self.h, node, _p(d) if len(d) else None, len(d), _p(a), _p(b), _p(v0), _p(v1))
+        return v0, v1
+
+    def close(self):
+        if self.h:
+            lib().or_hu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
