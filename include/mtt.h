@@ -220,6 +220,12 @@ int mtt_solver_get_strategy(mtt_solver* h, float* host_out, size_t n_floats);
  * own values, out[4] = pair mass M, out[5] = exploitability in % of the
  * starting pot = 100*(BR0 + BR1 - pot*M) / (2*M*pot). */
 int mtt_solver_exploitability_fp64(mtt_solver* h, double* out);
+/* Raw regrets and strategy sums of the plain path (reference layout). */
+int mtt_solver_get_state(mtt_solver* h, float* R_out, float* S_out, size_t n_floats);
+/* Per-kernel timing with HIP events on the handle's stream (for benchmarks):
+ * enable clears the totals; kernel_times returns entry idx (-1 past the end). */
+int mtt_solver_timing(mtt_solver* h, int enable);
+int mtt_solver_kernel_times(mtt_solver* h, int idx, char* name, int name_cap, double* total_ms, int64_t* launches);
 /* info: [0] nodes [1] infosets [2] alive combos [3] device bytes
  * [4] decision nodes [5] showdown nodes [6] fold nodes [7] matrices */
 int mtt_solver_info(mtt_solver* h, int64_t* info);

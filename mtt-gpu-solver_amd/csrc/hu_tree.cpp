@@ -283,6 +283,12 @@ int build_hu_tree(const mtt_hu_game& g, HuTree& out, bool flat) {
         x.children_offset = -1;
         x.infoset_offset = -1;
         x.active_mask = 3;
+        if ((int)out.flat_tmpl.size() < (int)F.size()) {
+            out.flat_tmpl.resize(F.size(), -1);
+            out.flat_tnode.resize(F.size(), -1);
+        }
+        out.flat_tmpl[fi] = tid;
+        out.flat_tnode[fi] = ni;
         if (n.type == T_DECISION) {
             x.type = MTT_NODE_DECISION;
             x.player = n.player;

@@ -40,6 +40,7 @@ struct HuTree {
     int first_street = 0;
     // flat expansion (reference FlatNode layout)
     std::vector<mtt_flat_node> flat;
+    std::vector<int32_t> flat_tmpl, flat_tnode;  // template and template node of each flat node
     int n_infosets = 0;
     int n_matrices = 0;                 // matrices referenced by showdown turn_idx
     int board_cards = 0;                // cards known at the root

@@ -2,6 +2,7 @@
 // code paths.
 #pragma once
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "hu_kernels.h"
@@ -12,6 +13,50 @@ struct mtt_solver;  // opaque in the C ABI
 namespace mtt {
 
 struct BatchedState;  // hu_batched.hip
+
+// Per-kernel timing with HIP events recorded on the solver's stream.
+struct KernelTimer {
+    bool enabled = false;
+    struct Mark { const char* name; hipEvent_t a, b; };
+    std::vector<Mark> pending;
+    std::vector<std::string> names;
+    std::vector<double> ms;
+    std::vector<long> count;
+    void begin(const char* name, hipStream_t s, Mark& m) {
+        m.name = name;
+        (void)hipEventCreate(&m.a);
+        (void)hipEventCreate(&m.b);
+        (void)hipEventRecord(m.a, s);
+    }
+    void end(Mark& m, hipStream_t s) {
+        (void)hipEventRecord(m.b, s);
+        pending.push_back(m);
+    }
+    void collect() {
+        for (auto& m : pending) {
+            float t = 0.0f;
+            (void)hipEventSynchronize(m.b);
+            (void)hipEventElapsedTime(&t, m.a, m.b);
+            size_t k = 0;
+            while (k < names.size() && names[k] != m.name) ++k;
+            if (k == names.size()) { names.push_back(m.name); ms.push_back(0.0); count.push_back(0); }
+            ms[k] += t;
+            count[k]++;
+            (void)hipEventDestroy(m.a);
+            (void)hipEventDestroy(m.b);
+        }
+        pending.clear();
+    }
+};
+
+// Launches a kernel and, when timing is on, brackets it with events.
+#define MTT_TIMED(S, NAME, ...)                                   \
+    do {                                                          \
+        ::mtt::KernelTimer::Mark _m;                              \
+        if ((S).timer.enabled) (S).timer.begin(NAME, (S).stream, _m); \
+        __VA_ARGS__;                                              \
+        if ((S).timer.enabled) (S).timer.end(_m, (S).stream);     \
+    } while (0)
 
 struct SolverImpl {
     mtt_hu_game game;
@@ -29,6 +74,7 @@ struct SolverImpl {
     std::vector<std::vector<int>> depth_lists;  // decision + chance nodes
     std::vector<int> terminals;
     int iters_done = 0;
+    KernelTimer timer;
     // device (plain + shared)
     DeviceArena arena;
     mtt_flat_node* d_nodes = nullptr;

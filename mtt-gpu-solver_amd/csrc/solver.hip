@@ -211,15 +211,22 @@ int solver_run_plain(SolverImpl& s, int iters) {
         float wp, wn, ws;
         iteration_weights(s.cfg, t, wp, wn, ws);
         int rc = MTT_OK;
+        KernelTimer::Mark m;
+        if (s.timer.enabled) s.timer.begin("plain_forward", s.stream, m);
         for (int d = 0; d < D && !rc; ++d)
             rc = launch_hu_forward(s.d_depth[d], (int)s.depth_lists[d].size(), s.d_nodes, s.d_deck, s.d_combos, s.d_R,
                                    s.d_sigma, s.d_reach, s.n_alive, s.stream);
+        if (s.timer.enabled) s.timer.end(m, s.stream);
+        if (s.timer.enabled) s.timer.begin("plain_terminals", s.stream, m);
         if (!rc)
             rc = launch_hu_terminals_grouped(s.plan, s.d_nodes, s.d_deck, s.d_combos, s.d_reach, s.d_mats, s.d_sums,
                                              s.d_cfv, s.n_alive, s.stream);
+        if (s.timer.enabled) s.timer.end(m, s.stream);
+        if (s.timer.enabled) s.timer.begin("plain_backward", s.stream, m);
         for (int d = D - 1; d >= 0 && !rc; --d)
             rc = launch_hu_backward(s.d_depth[d], (int)s.depth_lists[d].size(), s.d_nodes, s.d_deck, s.d_combos,
                                     s.d_sigma, s.d_reach, s.d_cfv, s.d_R, s.d_S, wp, wn, ws, s.n_alive, s.stream);
+        if (s.timer.enabled) s.timer.end(m, s.stream);
         if (rc) return rc;
         s.iters_done = t;
     }
@@ -397,6 +404,39 @@ int mtt_solver_exploitability_fp64(mtt_solver* h, double* out) {
     rc = rc ? rc : avg_strategy_device(h->impl, d);
     rc = rc ? rc : br_exploitability(h->impl, d, out);
     return rc;
+}
+
+int mtt_solver_get_state(mtt_solver* h, float* R_out, float* S_out, size_t n_floats) {
+    MTT_ARG(h, "mtt_solver_get_state: null handle");
+    size_t need = (size_t)h->impl.tree.n_infosets * AS * NP;
+    MTT_ARG(n_floats >= need, "mtt_solver_get_state: need %zu floats", need);
+    MTT_ARG(h->impl.cfg.mode == MTT_MODE_PLAIN_FP32, "mtt_solver_get_state: plain mode only");
+    if (R_out) MTT_HIP(hipMemcpy(R_out, h->impl.d_R, need * sizeof(float), hipMemcpyDeviceToHost));
+    if (S_out) MTT_HIP(hipMemcpy(S_out, h->impl.d_S, need * sizeof(float), hipMemcpyDeviceToHost));
+    return MTT_OK;
+}
+
+int mtt_solver_timing(mtt_solver* h, int enable) {
+    MTT_ARG(h, "mtt_solver_timing: null handle");
+    h->impl.timer.collect();
+    h->impl.timer.enabled = enable != 0;
+    if (enable) {
+        h->impl.timer.names.clear();
+        h->impl.timer.ms.clear();
+        h->impl.timer.count.clear();
+    }
+    return MTT_OK;
+}
+
+int mtt_solver_kernel_times(mtt_solver* h, int idx, char* name, int name_cap, double* total_ms, int64_t* launches) {
+    MTT_ARG(h && name && total_ms && launches && name_cap > 0, "mtt_solver_kernel_times: bad arguments");
+    MTT_HIP(hipStreamSynchronize(h->impl.stream));
+    h->impl.timer.collect();
+    if (idx < 0 || idx >= (int)h->impl.timer.names.size()) return -1;
+    snprintf(name, name_cap, "%s", h->impl.timer.names[idx].c_str());
+    *total_ms = h->impl.timer.ms[idx];
+    *launches = h->impl.timer.count[idx];
+    return MTT_OK;
 }
 
 int mtt_solver_info(mtt_solver* h, int64_t* info) {

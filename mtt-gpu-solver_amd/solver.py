@@ -53,6 +53,9 @@ for name, args in {
     "mtt_solver_get_strategy": [_P, _P, ctypes.c_size_t],
     "mtt_solver_exploitability_fp64": [_P, _P],
     "mtt_solver_info": [_P, _P],
+    "mtt_solver_get_state": [_P, _P, _P, ctypes.c_size_t],
+    "mtt_solver_timing": [_P, _I],
+    "mtt_solver_kernel_times": [_P, _I, ctypes.c_char_p, _I, _P, _P],
     "mtt_solver_destroy": [_P],
     "mtt_hu_forward_sweep": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P],
     "mtt_hu_evaluate_terminals": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P],
@@ -167,6 +170,31 @@ class Solver:
         n = int(self.info[1]) * mtt.MAX_ACTIONS * mtt.N_PAD
         out = np.zeros(n, np.float32)
         mtt.call("mtt_solver_get_strategy", self.h, out.ctypes.data, n)
+        return out
+
+    def state(self):
+        n = int(self.info[1]) * mtt.MAX_ACTIONS * mtt.N_PAD
+        R = np.zeros(n, np.float32)
+        S = np.zeros(n, np.float32)
+        mtt.call("mtt_solver_get_state", self.h, R.ctypes.data, S.ctypes.data, n)
+        return R, S
+
+    def timing(self, enable):
+        mtt.call("mtt_solver_timing", self.h, 1 if enable else 0)
+
+    def kernel_times(self):
+        """{kernel name: (total ms, launches)} recorded since timing(True)."""
+        out = {}
+        k = 0
+        while True:
+            name = ctypes.create_string_buffer(64)
+            ms = ctypes.c_double()
+            n = ctypes.c_int64()
+            rc = mtt.lib().mtt_solver_kernel_times(self.h, k, name, 64, ctypes.byref(ms), ctypes.byref(n))
+            if rc != 0:
+                break
+            out[name.value.decode()] = (ms.value, n.value)
+            k += 1
         return out
 
     def exploitability(self):

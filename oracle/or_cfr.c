@@ -52,6 +52,11 @@ struct or_hu_s {
 };
 typedef struct or_hu_s or_hu;
 
+static int cmp_u64(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : (x > y);
+}
+
 static const or_board* or_get_board(const or_hu* hc, const uint8_t* dealt, int nd) {
     or_hu* h = (or_hu*)hc;
     int key = (nd >= 1 ? dealt[0] + 1 : 0) * 50 + (nd >= 2 ? dealt[1] + 1 : 0);
@@ -72,13 +77,12 @@ static const or_board* or_get_board(const or_hu* hc, const uint8_t* dealt, int n
         b->score[i] = or_eval7(hand);
         b->order[b->n++] = i;
     }
-    /* insertion sort by score (boards are built once) */
-    for (int a = 1; a < b->n; ++a) {
-        int32_t x = b->order[a];
-        int k = a - 1;
-        while (k >= 0 && b->score[b->order[k]] > b->score[x]) { b->order[k + 1] = b->order[k]; k--; }
-        b->order[k + 1] = x;
-    }
+    /* stable sort by score: pack (score, index) and sort as 64-bit keys */
+    uint64_t* kv = (uint64_t*)malloc(sizeof(uint64_t) * (b->n ? b->n : 1));
+    for (int a = 0; a < b->n; ++a) kv[a] = ((uint64_t)b->score[b->order[a]] << 32) | (uint32_t)b->order[a];
+    qsort(kv, b->n, sizeof(uint64_t), cmp_u64);
+    for (int a = 0; a < b->n; ++a) b->order[a] = (int32_t)(kv[a] & 0xFFFFFFFFu);
+    free(kv);
     h->boards[key] = b;
     return b;
 }
@@ -202,6 +206,17 @@ or_hu* or_hu_create(const mtt_flat_node* nodes, int n_nodes, int n_infosets, con
         for (int i = 0; i < n; ++i) h->range[p][i] = src[i];
     }
     h->max_depth = tree_depth(nodes, 0) + 2;
+    /* every 5-card board reachable below the root, built up front so that
+     * the parallel iteration only reads the cache */
+    if (n_board == 5) {
+        or_get_board(h, NULL, 0);
+    } else if (n_board == 4) {
+        for (int a = 0; a < h->deck_n; ++a) { uint8_t d[1] = {(uint8_t)a}; or_get_board(h, d, 1); }
+    } else {
+        for (int a = 0; a < h->deck_n; ++a)
+            for (int b = 0; b < h->deck_n; ++b)
+                if (a != b) { uint8_t d[2] = {(uint8_t)a, (uint8_t)b}; or_get_board(h, d, 2); }
+    }
     h->prec = precision;
     size_t sz = (size_t)n_infosets * h->A * n;
     size_t el = precision == 0 ? sizeof(double) : sizeof(float);
@@ -222,44 +237,12 @@ void or_hu_destroy(or_hu* h) {
 
 int or_hu_n_combos(const or_hu* h) { return h->N; }
 
-#define ALLOC_BUF(B, T, h)                                                      \
-    do {                                                                        \
-        (B).A = (h)->A; (B).N = (h)->N; (B).R = (T*)(h)->R; (B).S = (T*)(h)->S; \
-        (B).max_depth = (h)->max_depth;                                         \
-        (B).reach = (T**)malloc(sizeof(T*) * (B).max_depth);                    \
-        (B).vals = (T**)malloc(sizeof(T*) * (B).max_depth);                     \
-        (B).sig = (T**)malloc(sizeof(T*) * (B).max_depth);                      \
-        (B).acc = (T**)malloc(sizeof(T*) * (B).max_depth);                      \
-        for (int _d = 0; _d < (B).max_depth; ++_d) {                            \
-            (B).reach[_d] = (T*)malloc(sizeof(T) * 2 * (h)->N);                 \
-            (B).vals[_d] = (T*)malloc(sizeof(T) * (h)->A * 2 * (h)->N);         \
-            (B).sig[_d] = (T*)malloc(sizeof(T) * (h)->A * (h)->N);              \
-            (B).acc[_d] = (T*)malloc(sizeof(T) * 2 * (h)->N);                   \
-        }                                                                       \
-    } while (0)
-#define FREE_BUF(B)                                                             \
-    do {                                                                        \
-        for (int _d = 0; _d < (B).max_depth; ++_d) {                            \
-            free((B).reach[_d]); free((B).vals[_d]); free((B).sig[_d]); free((B).acc[_d]); \
-        }                                                                       \
-        free((B).reach); free((B).vals); free((B).sig); free((B).acc);          \
-    } while (0)
-
 /* scheme 0 = CFR+ (R floored, linear averaging), 1 = DCFR(alpha,beta,gamma)
  * with beta = -INFINITY meaning negatives are floored (the reference form,
  * preflop_grid_generator.cu:402). */
 void or_hu_iterate(or_hu* h, int iters, int scheme, double alpha, double beta, double gamma) {
-    if (h->prec == 0) {
-        cfr_buf_d B;
-        ALLOC_BUF(B, double, h);
-        cfr_iterations_d(h, &B, iters, h->iters_done, scheme, alpha, beta, gamma);
-        FREE_BUF(B);
-    } else {
-        cfr_buf_f B;
-        ALLOC_BUF(B, float, h);
-        cfr_iterations_f(h, &B, iters, h->iters_done, scheme, alpha, beta, gamma);
-        FREE_BUF(B);
-    }
+    if (h->prec == 0) cfr_iterations_d(h, iters, h->iters_done, scheme, alpha, beta, gamma);
+    else cfr_iterations_f(h, iters, h->iters_done, scheme, alpha, beta, gamma);
     h->iters_done += iters;
 }
 

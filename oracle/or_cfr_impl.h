@@ -218,12 +218,31 @@ static void FN(cfr_rec)(or_hu* h, FN(cfr_buf) * B, int node, int depth, uint8_t*
     }
 }
 
-static void FN(cfr_iterations)(or_hu* h, FN(cfr_buf) * B, int iters, int t0, int scheme, double alpha,
-                               double beta, double gamma) {
+static void FN(buf_init)(FN(cfr_buf) * B, or_hu* h) {
+    B->A = h->A; B->N = h->N; B->R = (REAL*)h->R; B->S = (REAL*)h->S;
+    B->max_depth = h->max_depth;
+    B->reach = (REAL**)malloc(sizeof(REAL*) * B->max_depth);
+    B->vals = (REAL**)malloc(sizeof(REAL*) * B->max_depth);
+    B->sig = (REAL**)malloc(sizeof(REAL*) * B->max_depth);
+    B->acc = (REAL**)malloc(sizeof(REAL*) * B->max_depth);
+    for (int d = 0; d < B->max_depth; ++d) {
+        B->reach[d] = (REAL*)malloc(sizeof(REAL) * 2 * h->N);
+        B->vals[d] = (REAL*)malloc(sizeof(REAL) * h->A * 2 * h->N);
+        B->sig[d] = (REAL*)malloc(sizeof(REAL) * h->A * h->N);
+        B->acc[d] = (REAL*)malloc(sizeof(REAL) * 2 * h->N);
+    }
+}
+
+static void FN(buf_free)(FN(cfr_buf) * B) {
+    for (int d = 0; d < B->max_depth; ++d) { free(B->reach[d]); free(B->vals[d]); free(B->sig[d]); free(B->acc[d]); }
+    free(B->reach); free(B->vals); free(B->sig); free(B->acc);
+}
+
+/* Iterations; a chance root is split over OpenMP threads (its subtrees own
+ * disjoint regrets and strategy sums, and boards are precomputed). */
+static void FN(cfr_iterations)(or_hu* h, int iters, int t0, int scheme, double alpha, double beta, double gamma) {
     int N = h->N;
-    REAL* r = (REAL*)malloc(sizeof(REAL) * 2 * N);
-    REAL* v = (REAL*)malloc(sizeof(REAL) * 2 * N);
-    uint8_t dealt[4];
+    const mtt_flat_node* root = &h->nodes[0];
     for (int it = 1; it <= iters; ++it) {
         double t = (double)(t0 + it);
         double wp, wn, ws;
@@ -234,8 +253,36 @@ static void FN(cfr_iterations)(or_hu* h, FN(cfr_buf) * B, int iters, int t0, int
             wn = isinf(beta) && beta < 0 ? 0 : tb / (tb + 1);
             ws = pow(t, gamma);
         }
-        for (int i = 0; i < N; ++i) { r[i] = (REAL)h->range[0][i]; r[N + i] = (REAL)h->range[1][i]; }
-        FN(cfr_rec)(h, B, 0, 0, dealt, 0, r, r + N, v, v + N, wp, wn, ws);
+        if (root->type != MTT_NODE_CHANCE) {
+            FN(cfr_buf) B;
+            FN(buf_init)(&B, h);
+            REAL* r = (REAL*)malloc(sizeof(REAL) * 4 * N);
+            for (int i = 0; i < N; ++i) { r[i] = (REAL)h->range[0][i]; r[N + i] = (REAL)h->range[1][i]; }
+            uint8_t dealt[4];
+            FN(cfr_rec)(h, &B, 0, 0, dealt, 0, r, r + N, r + 2 * N, r + 3 * N, wp, wn, ws);
+            free(r);
+            FN(buf_free)(&B);
+            continue;
+        }
+#pragma omp parallel
+        {
+            FN(cfr_buf) B;
+            FN(buf_init)(&B, h);
+            REAL* r = (REAL*)malloc(sizeof(REAL) * 4 * N);
+            uint8_t dealt[4];
+#pragma omp for schedule(dynamic, 1)
+            for (int k = 0; k < root->num_actions; ++k) {
+                uint64_t cb = 1ull << h->deck[k];
+                for (int i = 0; i < N; ++i) {
+                    int dead = (h->cmask[i] & cb) != 0;
+                    r[i] = dead ? 0 : (REAL)h->range[0][i];
+                    r[N + i] = dead ? 0 : (REAL)h->range[1][i];
+                }
+                dealt[0] = (uint8_t)k;
+                FN(cfr_rec)(h, &B, root->children_offset + k, 1, dealt, 1, r, r + N, r + 2 * N, r + 3 * N, wp, wn, ws);
+            }
+            free(r);
+            FN(buf_free)(&B);
+        }
     }
-    free(r); free(v);
 }

@@ -1,0 +1,202 @@
+"""GPU parity of the heads-up solver paths (through the C ABI) against the
+CPU oracle: strategies and FP64 exploitability on the same seeded inputs."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# Tolerances.  Both sides run the same recurrence; FP32 sums differ in order
+# (GPU vs CPU), and the differences grow slowly with iterations.
+STRAT_LINF_FP32 = 2e-3      # plain FP32 GPU vs FP32 oracle, average strategy
+STRAT_LINF_BF16 = 5e-3      # BF16 hi+lo showdown (16-bit reach mantissa) vs FP32 oracle
+EXPL_ABS = 2e-3             # % pot, GPU FP64 BR vs oracle FP64 BR of the same strategy
+
+
+def _setup(solver_api, oracle, g, r0, r1, prec):
+    nodes, ni, _ = solver_api.tree(g)
+    board = list(g.board)[:g.n_board]
+    gi, cards = solver_api.alive_index(board)
+    return oracle.HU(nodes, ni, board, r0[gi], r1[gi], prec), nodes, ni
+
+
+def _live_linf(a, b, nodes, n_alive, S_ref):
+    """L-inf of the average-strategy difference over (infoset, combo) entries
+    that carry strategy mass: sum_a S >= 1e-5 of the infoset's largest.
+    Entries below are combos whose reach underflows (Dirichlet(0.5) ranges
+    have weights near 0); their average strategy is numerically arbitrary and
+    does not enter the exploitability, which is compared separately."""
+    a = a.reshape(-1, 4, 1216)
+    b = b.reshape(-1, 4, 1216)
+    w = S_ref.reshape(-1, 4, 1216).sum(1)
+    d = 0.0
+    for x in nodes[nodes["type"] == 0]:
+        k, na = int(x["infoset_offset"]), int(x["num_actions"])
+        m = w[k, :n_alive]
+        if m.max() <= 0:
+            continue
+        keep = m >= 1e-5 * m.max()
+        d = max(d, float(np.abs(a[k, :na, :n_alive] - b[k, :na, :n_alive])[:, keep].max()))
+    return d
+
+
+def test_plain_config1_matches_oracle(solver_api, oracle, torch_dev):
+    g, r0, r1 = solver_api.config1(0)
+    print("config1 plain fp32, seed 0", flush=True)
+    s = solver_api.Solver(g, r0, r1, solver_api.MODE_PLAIN_FP32)
+    h32, nodes, ni = _setup(solver_api, oracle, g, r0, r1, 1)
+    s.run(200)
+    h32.iterate(200)
+    st = s.strategy()
+    n_alive = int(s.info[2])
+    assert _live_linf(st, h32.strategy(), nodes, n_alive, h32.state()[1]) < STRAT_LINF_FP32
+    e_gpu = s.exploitability()
+    e_cpu = h32.exploitability(st, 10.0)
+    assert np.allclose(e_gpu[:5], e_cpu[:5], rtol=1e-9, atol=1e-12)
+    assert abs(e_gpu[5] - e_cpu[5]) < EXPL_ABS
+    s.run(800)
+    e = s.exploitability()
+    assert e[5] < 0.5 and e[5] < e_gpu[5]
+    s.close()
+
+
+def test_plain_multistreet_matches_oracle(solver_api, oracle, torch_dev):
+    """Turn chance, turn betting, river chance, river betting, turn all-in and
+    river showdowns (all-in-only menu, full 49x48 boards)."""
+    g, r0, r1 = solver_api.config3_small(1)
+    print("config3-small plain fp32, seed 1", flush=True)
+    s = solver_api.Solver(g, r0, r1, solver_api.MODE_PLAIN_FP32)
+    h32, nodes, ni = _setup(solver_api, oracle, g, r0, r1, 1)
+    s.run(30)
+    h32.iterate(30)
+    st = s.strategy()
+    assert _live_linf(st, h32.strategy(), nodes, int(s.info[2]), h32.state()[1]) < STRAT_LINF_FP32
+    e_gpu = s.exploitability()
+    e_cpu = h32.exploitability(st, 6.0)
+    assert np.allclose(e_gpu[:5], e_cpu[:5], rtol=1e-7)
+    assert abs(e_gpu[5] - e_cpu[5]) < EXPL_ABS
+    s.close()
+
+
+def test_dropin_kernels_manual_iteration(solver_api, oracle, torch_dev, mtt):
+    """A host loop over the reference-style kernel entry points (forward per
+    depth, per-terminal evaluation, backward per depth) reproduces the
+    oracle's first iterations."""
+    import torch
+    g, r0, r1 = solver_api.config1(2)
+    nodes, ni, _ = solver_api.tree(g)
+    board = list(g.board)[:5]
+    gi, cards = solver_api.alive_index(board)
+    n_alive = len(gi)
+    NP, A = 1216, 4
+    dev = torch_dev
+    combos = np.zeros(NP, dtype=[("c1", "u1"), ("c2", "u1"), ("pad", "V6"), ("mask", "<u8")])
+    combos["c1"][:n_alive] = cards[:, 0]
+    combos["c2"][:n_alive] = cards[:, 1]
+    combos["mask"][:n_alive] = (1 << cards[:, 0].astype(np.uint64)) | (1 << cards[:, 1].astype(np.uint64))
+    combos["mask"][n_alive:] = np.uint64(2**64 - 1)
+    d_combos = torch.from_numpy(combos.view(np.uint8).copy()).to(dev)
+    deck = np.array([c for c in range(52) if c not in board], np.uint8)
+    d_deck = torch.from_numpy(deck).to(dev)
+    d_nodes = torch.from_numpy(nodes.view(np.uint8).copy()).to(dev)
+    nn = len(nodes)
+    reach = torch.zeros(nn * 2 * NP, device=dev)
+    cfv = torch.zeros(nn * 2 * NP, device=dev)
+    R = torch.zeros(ni * A * NP, device=dev)
+    S = torch.zeros_like(R)
+    sig = torch.zeros_like(R)
+    reach[:n_alive] = torch.from_numpy(r0[gi]).to(dev)
+    reach[NP:NP + n_alive] = torch.from_numpy(r1[gi]).to(dev)
+    # depth lists
+    depth = {0: [0]}
+    q, dmap = [0], {0: 0}
+    terms = []
+    while q:
+        i = q.pop(0)
+        x = nodes[i]
+        if x["type"] in (1, 2):
+            terms.append(i)
+            continue
+        depth.setdefault(dmap[i], [])
+        if i not in depth[dmap[i]]:
+            depth[dmap[i]].append(i)
+        for k in range(int(x["num_actions"])):
+            c = int(x["children_offset"]) + k
+            dmap[c] = dmap[i] + 1
+            q.append(c)
+    levels = [torch.tensor(depth[d], dtype=torch.int32, device=dev) for d in sorted(depth)]
+    d_terms = torch.tensor(terms, dtype=torch.int32, device=dev)
+    # the single showdown matrix through the same generator as the solver
+    mats = torch.zeros(NP * NP, device=dev)
+    cm = np.zeros(NP, dtype=combos.dtype)
+    # board with the root's five cards: E[j][h] over root-alive combos
+    from itertools import combinations
+    sc = oracle.eval7_batch(np.array([[*board, a, b] for a, b in cards], np.uint8)).astype(np.int64)
+    m = (1 << cards[:, 0].astype(object)) | (1 << cards[:, 1].astype(object))
+    E = np.zeros((NP, NP), np.float32)
+    blk = np.array([[bool(m[i] & m[j]) for i in range(n_alive)] for j in range(n_alive)])
+    e = (sc[None, :] > sc[:, None]) + 0.5 * (sc[None, :] == sc[:, None])
+    e[blk] = 0
+    E[:n_alive, :n_alive] = e
+    mats.copy_(torch.from_numpy(E.ravel()).to(dev))
+    h = oracle.HU(nodes, ni, board, r0[gi], r1[gi], 1)
+    L = mtt.lib()
+    for t in range(1, 4):
+        for lv in levels:
+            mtt.check(L.mtt_hu_forward_sweep(lv.data_ptr(), lv.numel(), d_nodes.data_ptr(), d_deck.data_ptr(),
+                                             d_combos.data_ptr(), R.data_ptr(), sig.data_ptr(), reach.data_ptr(),
+                                             n_alive, None), "forward")
+        mtt.check(L.mtt_hu_evaluate_terminals(d_terms.data_ptr(), d_terms.numel(), d_nodes.data_ptr(),
+                                              d_deck.data_ptr(), d_combos.data_ptr(), reach.data_ptr(),
+                                              mats.data_ptr(), cfv.data_ptr(), n_alive, None), "terminals")
+        for lv in reversed(levels):
+            mtt.check(L.mtt_hu_backward_sweep(lv.data_ptr(), lv.numel(), d_nodes.data_ptr(), d_deck.data_ptr(),
+                                              d_combos.data_ptr(), sig.data_ptr(), reach.data_ptr(), cfv.data_ptr(),
+                                              R.data_ptr(), S.data_ptr(), ctypes.c_float(float(t)), n_alive, None),
+                      "backward")
+        torch.cuda.synchronize()
+        h.iterate(1)
+        Rc, Sc = h.state()
+        Rg = R.cpu().numpy()
+        Sg = S.cpu().numpy()
+        scale = max(1.0, np.abs(Rc).max())
+        assert np.abs(Rg - Rc).max() < 1e-4 * scale, t
+        assert np.abs(Sg - Sc).max() < 1e-4 * max(1.0, np.abs(Sc).max()), t
+
+
+def test_batched_config1_matches_oracle(solver_api, oracle, torch_dev):
+    """BF16 hi+lo split showdown GEMM path on the river subgame."""
+    g, r0, r1 = solver_api.config1(0)
+    print("config1 batched bf16-split, seed 0", flush=True)
+    s = solver_api.Solver(g, r0, r1, solver_api.MODE_BF16_SPLIT)
+    h32, nodes, ni = _setup(solver_api, oracle, g, r0, r1, 1)
+    s.run(200)
+    h32.iterate(200)
+    st = s.strategy()
+    assert _live_linf(st, h32.strategy(), nodes, int(s.info[2]), h32.state()[1]) < STRAT_LINF_BF16
+    e_gpu = s.exploitability()
+    e_cpu = h32.exploitability(st, 10.0)
+    assert np.allclose(e_gpu[:5], e_cpu[:5], rtol=1e-9, atol=1e-12)
+    e_ref = h32.exploitability(h32.strategy(), 10.0)
+    assert abs(e_gpu[5] - e_ref[5]) < 0.01
+    s.close()
+
+
+def test_batched_multistreet_matches_oracle(solver_api, oracle, torch_dev):
+    """Turn + river street kernels, river GEMM and turn all-in showdowns on
+    the all-in-only turn+river game."""
+    g, r0, r1 = solver_api.config3_small(1)
+    print("config3-small batched bf16-split, seed 1", flush=True)
+    s = solver_api.Solver(g, r0, r1, solver_api.MODE_BF16_SPLIT)
+    h32, nodes, ni = _setup(solver_api, oracle, g, r0, r1, 1)
+    s.run(30)
+    h32.iterate(30)
+    st = s.strategy()
+    assert _live_linf(st, h32.strategy(), nodes, int(s.info[2]), h32.state()[1]) < STRAT_LINF_BF16
+    e_gpu = s.exploitability()
+    e_cpu = h32.exploitability(st, 6.0)
+    assert np.allclose(e_gpu[:5], e_cpu[:5], rtol=1e-7)
+    e_ref = h32.exploitability(h32.strategy(), 6.0)
+    assert abs(e_gpu[5] - e_ref[5]) < 0.01
+    s.close()
