@@ -193,6 +193,40 @@ int mtt_seeded_cards(uint64_t seed, int n, uint8_t* out);
 int mtt_seeded_inputs(uint64_t seed, int n_board, int dist, float alpha, uint8_t* board_out,
                       float* range0_out, float* range1_out);
 
+/* ------------------------------------------- HU tree-sweep kernels -- */
+/* Per-depth kernels with the reference's argument meaning, for a host loop
+ * that drives the sweeps itself.  Device pointers; layouts:
+ * regrets/strategy/strategy sums [infoset][MTT_MAX_ACTIONS][1216],
+ * reach/cfv [node][2][1216], matrices [n][1216][1216] (E[j][h] = equity of
+ * hero h against villain j, 0 where blocked).  n_alive = live combos of the
+ * root board (<= 1216).  stream may be NULL (the null stream). */
+/* Regret matching + reach propagation for the decision nodes of one depth;
+ * replaces kernel_forward_sweep, honest_multistreet_solver.cu:88-126. */
+int mtt_hu_forward_sweep(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node* d_nodes,
+                         const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_cumulative_regrets,
+                         float* d_current_strategy, float* d_reach_probs, int n_alive, hipStream_t stream);
+/* Showdown and fold values of terminal nodes (card removal, exact chance
+ * normalisation); replaces kernel_evaluate_terminals,
+ * honest_multistreet_solver.cu:128-196. */
+int mtt_hu_evaluate_terminals(const int* d_terminal_node_indices, int num_terminals, const mtt_flat_node* d_nodes,
+                              const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_reach_probs,
+                              const float* d_turn_matrices, float* d_cfvs, int n_alive, hipStream_t stream);
+/* CFR+ backward pass for one depth: node values, regrets floored at 0,
+ * strategy sums weighted by iter_weight; replaces kernel_backward_sweep,
+ * honest_multistreet_solver.cu:198-249. */
+int mtt_hu_backward_sweep(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node* d_nodes,
+                          const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_current_strategy,
+                          const float* d_reach_probs, float* d_cfvs, float* d_cumulative_regrets,
+                          float* d_cumulative_strategy_sums, float iter_weight, int n_alive, hipStream_t stream);
+/* Same pass with DCFR discounting fused: R <- (R + r) * (R + r > 0 ? w_pos :
+ * w_neg), S += w_strat * reach * sigma; the reference's DCFR step is
+ * kernel_dcfr_step, preflop_grid_generator.cu:387-410. */
+int mtt_hu_backward_sweep_dcfr(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node* d_nodes,
+                               const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_current_strategy,
+                               const float* d_reach_probs, float* d_cfvs, float* d_cumulative_regrets,
+                               float* d_cumulative_strategy_sums, float w_pos, float w_neg, float w_strat, int n_alive,
+                               hipStream_t stream);
+
 /* -------------------------------------------------------- HU solver -- */
 /* Solve entry points (new: the reference's HU main is a stub,
  * honest_multistreet_solver.cu:251-254).  The handle owns all device state
@@ -220,7 +254,7 @@ int mtt_solver_get_strategy(mtt_solver* h, float* host_out, size_t n_floats);
  * own values, out[4] = pair mass M, out[5] = exploitability in % of the
  * starting pot = 100*(BR0 + BR1 - pot*M) / (2*M*pot). */
 int mtt_solver_exploitability_fp64(mtt_solver* h, double* out);
-/* Raw regrets and strategy sums of the plain path (reference layout). */
+/* Raw regrets and strategy sums in the reference layout (either path). */
 int mtt_solver_get_state(mtt_solver* h, float* R_out, float* S_out, size_t n_floats);
 /* Per-kernel timing with HIP events on the handle's stream (for benchmarks):
  * enable clears the totals; kernel_times returns entry idx (-1 past the end). */

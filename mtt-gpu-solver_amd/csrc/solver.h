@@ -87,7 +87,7 @@ struct SolverImpl {
     float* d_S = nullptr;
     float* d_sigma = nullptr;
     float* d_mats = nullptr;              // [n_mat][1216][1216]
-    float* d_sums = nullptr;              // [terms][2][64]
+    double* d_sums = nullptr;             // [terms][2][64] card sums (FP64)
     PlainTerminalPlan plan;
     // batched path
     BatchedState* batched = nullptr;       // owned; freed by batched_destroy
@@ -103,6 +103,7 @@ void iteration_weights(const mtt_solver_cfg& cfg, int t, float& wp, float& wn, f
 int batched_create(SolverImpl& s);
 int batched_run(SolverImpl& s, int iters);
 int batched_strategy_sums(SolverImpl& s, float* d_S_ref);  // scatter into the reference layout
+int batched_export(SolverImpl& s, float* d_out, int which);  // 0 strategy sums, 1 regrets
 void batched_destroy(BatchedState* b);
 
 // FP64 best response (br.hip): exploitability of the average strategy held

@@ -410,10 +410,21 @@ int mtt_solver_get_state(mtt_solver* h, float* R_out, float* S_out, size_t n_flo
     MTT_ARG(h, "mtt_solver_get_state: null handle");
     size_t need = (size_t)h->impl.tree.n_infosets * AS * NP;
     MTT_ARG(n_floats >= need, "mtt_solver_get_state: need %zu floats", need);
-    MTT_ARG(h->impl.cfg.mode == MTT_MODE_PLAIN_FP32, "mtt_solver_get_state: plain mode only");
-    if (R_out) MTT_HIP(hipMemcpy(R_out, h->impl.d_R, need * sizeof(float), hipMemcpyDeviceToHost));
-    if (S_out) MTT_HIP(hipMemcpy(S_out, h->impl.d_S, need * sizeof(float), hipMemcpyDeviceToHost));
-    return MTT_OK;
+    if (h->impl.cfg.mode == MTT_MODE_PLAIN_FP32) {
+        if (R_out) MTT_HIP(hipMemcpy(R_out, h->impl.d_R, need * sizeof(float), hipMemcpyDeviceToHost));
+        if (S_out) MTT_HIP(hipMemcpy(S_out, h->impl.d_S, need * sizeof(float), hipMemcpyDeviceToHost));
+        return MTT_OK;
+    }
+    DeviceArena tmp;
+    float* d = nullptr;
+    int rc = tmp.alloc(need, &d);
+    for (int w = 0; w < 2 && !rc; ++w) {
+        float* dst = w ? R_out : S_out;
+        if (!dst) continue;
+        rc = batched_export(h->impl, d, w);
+        if (!rc) MTT_HIP(hipMemcpy(dst, d, need * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return rc;
 }
 
 int mtt_solver_timing(mtt_solver* h, int enable) {
