@@ -51,18 +51,28 @@ constexpr int MAXE = 256;      // DFS events per template
 constexpr int MAXSLOT = 64;    // action slots per template
 constexpr double FX = 281474976710656.0;  // 2^48: fixed-point scale of card sums
 
+// One DFS event of a template with its node record, 32 bytes, read by the
+// sweeps through scalar loads.
+struct EvRec {
+    DNode nd;
+    int kind;              // EV_*
+};
+static_assert(sizeof(EvRec) == 32, "EvRec layout");
+
 struct LevelDev {
     int np, nslot, nterm, ncol, ncolp, ntmpl, nboards;
     int maxterm;               // largest terminal count of one template
-    const DNode* nodes;        // level nodes, templates concatenated (DFS preorder)
-    const int* tnode0;         // [ntmpl+1]
+    int maxcolt;               // largest showdown column count of one template
+    const EvRec* ev;           // DFS events of every template, concatenated
     const int* tev0;           // [ntmpl+1]
+    const EvRec* fev;          // the same without LEAVE events (preorder nodes)
+    const int* tfev0;          // [ntmpl+1]
     const int* tslot0;         // [ntmpl+1] first slot of each template
     const int* tterm0;         // [ntmpl+1] first terminal of each template
-    const int* events;         // (local node << 2) | kind
+    const int* tcol0;          // [ntmpl+1] first showdown column of each template
     float* R;                  // [board][nslot][np]
     float* S;
-    double* sums;              // [board][nterm][2][56] card sums, slot 52 = total
+    unsigned long long* part;  // [board][nterm][half][2][52] card sums of reach, 2^48 fixed point
     const uint8_t* cc;         // [board][np][2] cards of combo
     const int* nlive;          // [board]
 };
@@ -74,6 +84,7 @@ struct BatchedDev {
     const int* leaf_tmpl;      // turn leaf -> river template
     const int* tmpl_leaf;      // river template -> turn leaf
     float* leafreach;          // [49][nleaf][2][NPT]
+    float* leafval;            // [49][nleaf][2][NPT] turn-leaf values (mean over rivers)
     float* rootval;            // [boards1][ntmpl1][2][NPR]
     float* rootval0;           // [49][2][NPT]
     const uint16_t* rmap;      // [boards1][NPR] river combo -> turn combo
@@ -90,36 +101,59 @@ struct BatchedDev {
     float pot0;
 };
 
-// Template tables are identical for every lane: after staging them in LDS,
-// read them into wave-uniform registers so depth / slot indices select
-// registers directly.
-__device__ __forceinline__ DNode uniform_node(const DNode* p) {
-    const int* w = reinterpret_cast<const int*>(p);
-    int v[7];
+typedef float f32x32_t __attribute__((ext_vector_type(32)));
+typedef const __attribute__((address_space(4))) int* cint_p;
+
+// Wave-uniform reads of small per-level tables through the scalar data cache.
+__device__ __forceinline__ int ldu(const int* p) { return *(cint_p)p; }
+
+// Event records are staged in LDS per workgroup; the walk keeps the next
+// record in vector registers (issued one event ahead) and makes the current
+// one wave-uniform with readfirstlane.
+struct EvRaw {
+    int4 a, b;
+};
+__device__ __forceinline__ EvRaw ev_fetch(const EvRec* lds, int e) {
+    const int4* q = reinterpret_cast<const int4*>(lds + e);
+    EvRaw r;
+    r.a = q[0];
+    r.b = q[1];
+    return r;
+}
+__device__ __forceinline__ EvRec ev_uniform(const EvRaw& r) {
+    int v[8] = {r.a.x, r.a.y, r.a.z, r.a.w, r.b.x, r.b.y, r.b.z, r.b.w};
 #pragma unroll
-    for (int k = 0; k < 7; ++k) v[k] = __builtin_amdgcn_readfirstlane(w[k]);
-    DNode d;
-    __builtin_memcpy(&d, v, sizeof(d));
-    return d;
+    for (int k = 0; k < 8; ++k) v[k] = __builtin_amdgcn_readfirstlane(v[k]);
+    EvRec e;
+    __builtin_memcpy(&e, v, sizeof(e));
+    return e;
+}
+__device__ __forceinline__ void ev_stage(EvRec* lds, const EvRec* __restrict__ g, int n) {
+    const int4* src = reinterpret_cast<const int4*>(g);
+    int4* dst = reinterpret_cast<int4*>(lds);
+    for (int e = threadIdx.x; e < 2 * n; e += blockDim.x) dst[e] = src[e];
 }
 
-// Select element q (wave-uniform) of two 32-register arrays: a switch on a
-// uniform value becomes scalar branches and every array access keeps a
-// constant index, so the arrays stay in registers.
-__device__ __forceinline__ float sel64(const float (&a)[32], const float (&b)[32], int q) {
-    switch (q) {
-#define MTT_C(i) case i: return a[i]; case i + 32: return b[i];
-        MTT_C(0) MTT_C(1) MTT_C(2) MTT_C(3) MTT_C(4) MTT_C(5) MTT_C(6) MTT_C(7)
-        MTT_C(8) MTT_C(9) MTT_C(10) MTT_C(11) MTT_C(12) MTT_C(13) MTT_C(14) MTT_C(15)
-        MTT_C(16) MTT_C(17) MTT_C(18) MTT_C(19) MTT_C(20) MTT_C(21) MTT_C(22) MTT_C(23)
-        MTT_C(24) MTT_C(25) MTT_C(26) MTT_C(27) MTT_C(28) MTT_C(29) MTT_C(30) MTT_C(31)
-#undef MTT_C
-        default: return 0.0f;
-    }
+// 1/n for an action count n in 1..4 (wave-uniform; exact constants).
+__device__ __forceinline__ float uniform_share(int n) {
+    return n == 2 ? 0.5f : (n == 3 ? 1.0f / 3.0f : (n == 4 ? 0.25f : 1.0f));
 }
 
-__device__ __forceinline__ float mass_of(const double* s, int a, int b, float self) {
-    return (float)(s[52] - s[a] - s[b] + (double)self);
+// Element q (wave-uniform, < 64) of a 64-entry register file held as two
+// 32-register vectors: both reads share one indexed-register window.
+__device__ __forceinline__ float rsel(const f32x32_t& a, const f32x32_t& b, int q) {
+    const float va = a[q & 31], vb = b[q & 31];
+    return q < 32 ? va : vb;
+}
+
+// Non-blocked reach mass of the other player for a combo holding cards
+// a, b: total - sum[a] - sum[b] + own (the pair sharing both cards is the
+// combo itself and was subtracted twice), exact in 2^48 fixed point.  The
+// first difference can be negative (by at most own), so it is taken as a
+// two's-complement signed value.
+__device__ __forceinline__ float mass_u(const unsigned long long* s, int a, int b, float self) {
+    const long long m = (long long)(s[52] - s[a] - s[b]);
+    return (float)((double)m * (1.0 / FX) + (double)self);
 }
 
 __device__ __forceinline__ float root_reach(const BatchedDev& D, const float* __restrict__ ranges, int level, int b,
@@ -132,206 +166,234 @@ __device__ __forceinline__ float root_reach(const BatchedDev& D, const float* __
     return ranges[((size_t)b * 2 + p) * np + k];
 }
 
-// ------------------------------------------------------------ forward ----
-// One workgroup per (board, template); each thread owns two combos (tid and
-// tid + blockDim), so the whole board's range is in one workgroup and the
-// card sums of every terminal reach column are reduced in LDS with 64-bit
-// fixed-point atomics (exact and order independent).  Reaches and
-// strategies live in register stacks indexed by the uniform template depth.
-template <int LEVEL, int DM>
-__global__ __launch_bounds__(576) void k_bt_forward(BatchedDev D, const float* __restrict__ ranges) {
-    const LevelDev& L = D.L[LEVEL];
-    extern __shared__ unsigned long long bins[];  // [2*maxterm][52]
-    __shared__ DNode tn[MAXN];
-    const int inst = blockIdx.x;
-    const int b = inst / L.ntmpl, tm = inst % L.ntmpl;
-    const int tid = threadIdx.x, NT = blockDim.x;
-    const int np = L.np;
-    const int nlive = L.nlive[b];
-    const int n0 = L.tnode0[tm], nn = L.tnode0[tm + 1] - n0;
-    const int t0 = L.tterm0[tm], ncolt = 2 * (L.tterm0[tm + 1] - t0);
-    for (int e = tid; e < nn; e += NT) tn[e] = L.nodes[n0 + e];
-    for (int e = tid; e < ncolt * 52; e += NT) bins[e] = 0ull;
-    const int kk[2] = {tid, tid + NT};
-    const bool live[2] = {kk[0] < nlive, kk[1] < nlive};
-    int cA[2], cB[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        cA[q] = L.cc[((size_t)b * np + kk[q]) * 2];
-        cB[q] = L.cc[((size_t)b * np + kk[q]) * 2 + 1];
-    }
-    float rA0[DM], rA1[DM], rB0[DM], rB1[DM];
-    float sA[DM * 4], sB[DM * 4];
-    float root[2][2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int p = 0; p < 2; ++p) root[q][p] = live[q] ? root_reach(D, ranges, LEVEL, b, tm, np, kk[q], p) : 0.0f;
-    __syncthreads();
-    const float* Rb = L.R + (size_t)b * L.nslot * np;
-    for (int n = 0; n < nn; ++n) {
-        const DNode nd = uniform_node(&tn[n]);
-        const int d = nd.depth;
-        float r[2][2];
-        if (n == 0) {
-            r[0][0] = root[0][0]; r[0][1] = root[0][1]; r[1][0] = root[1][0]; r[1][1] = root[1][1];
-        } else {
-            const float pa0 = rA0[d - 1], pa1 = rA1[d - 1], pb0 = rB0[d - 1], pb1 = rB1[d - 1];
-            const float sa = sA[(d - 1) * 4 + nd.act], sb = sB[(d - 1) * 4 + nd.act];
-            r[0][0] = nd.pad == 0 ? pa0 * sa : pa0;
-            r[0][1] = nd.pad == 1 ? pa1 * sa : pa1;
-            r[1][0] = nd.pad == 0 ? pb0 * sb : pb0;
-            r[1][1] = nd.pad == 1 ? pb1 * sb : pb1;
-        }
-        if (nd.type == DN_DEC) {
-            rA0[d] = r[0][0]; rA1[d] = r[0][1]; rB0[d] = r[1][0]; rB1[d] = r[1][1];
-            float pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0}, suma = 0.0f, sumb = 0.0f;
-            const float* Rk = Rb + (size_t)nd.slot * np;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                if (a < nd.nact) {
-                    if (live[0]) pa[a] = fmaxf(Rk[(size_t)a * np + kk[0]], 0.0f);
-                    if (live[1]) pb[a] = fmaxf(Rk[(size_t)a * np + kk[1]], 0.0f);
-                }
-                suma += pa[a];
-                sumb += pb[a];
-            }
-            const float ia = suma > 1e-7f ? 1.0f / suma : 0.0f, ib = sumb > 1e-7f ? 1.0f / sumb : 0.0f;
-            const float un = 1.0f / (float)nd.nact;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                sA[d * 4 + a] = suma > 1e-7f ? pa[a] * ia : un;
-                sB[d * 4 + a] = sumb > 1e-7f ? pb[a] * ib : un;
-            }
-            continue;
-        }
-        if (nd.type == DN_LEAF) {
-            if (LEVEL == 0) {
-#pragma unroll
-                for (int q = 0; q < 2; ++q)
-#pragma unroll
-                    for (int p = 0; p < 2; ++p)
-                        D.leafreach[(((size_t)b * D.nleaf + nd.leaf) * 2 + p) * NPT + kk[q]] = live[q] ? r[q][p] : 0.0f;
-            }
-            continue;
-        }
-        if (nd.type == DN_SD) {
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int p = 0; p < 2; ++p) {
-                    const float ro = live[q] ? r[q][1 - p] : 0.0f;  // player p's column holds the opponent's reach
-                    const size_t col = (size_t)nd.col + p;
-                    if (LEVEL == 1) {
-                        const __hip_bfloat16 hi = __float2bfloat16(ro);
-                        D.Bh[((size_t)b * L.ncolp + col) * NPR + kk[q]] = hi;
-                        D.Bl[((size_t)b * L.ncolp + col) * NPR + kk[q]] = __float2bfloat16(ro - __bfloat162float(hi));
-                    } else {
-                        D.B0[((size_t)b * L.ncolp + col) * NPT + kk[q]] = ro;
-                    }
-                }
-        }
-        // fold or showdown: card sums of both players' reaches at this node
-        const int tl = nd.sidx - t0;
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                if (!live[q]) continue;
-                const unsigned long long v = (unsigned long long)__double2ull_rn((double)r[q][p] * FX);
-                if (v) {
-                    atomicAdd(&bins[(2 * tl + p) * 52 + cA[q]], v);
-                    atomicAdd(&bins[(2 * tl + p) * 52 + cB[q]], v);
-                }
-            }
-    }
-    __syncthreads();
-    double* out = L.sums + ((size_t)b * L.nterm + t0) * 2 * 56;
-    for (int e = tid; e < ncolt * 53; e += NT) {
-        const int col = e / 53, c = e % 53;
-        double v;
-        if (c < 52) v = (double)bins[col * 52 + c] / FX;
-        else {
-            unsigned long long t = 0;
-            for (int x = 0; x < 52; ++x) t += bins[col * 52 + x];
-            v = 0.5 * ((double)t / FX);
-        }
-        out[(size_t)col * 56 + c] = v;
-    }
-}
+constexpr int SWEEP_NT = 576;   // threads per sweep workgroup; two workgroups cover one board's combos
 
-// ----------------------------------------------------------- backward ----
-// Two workgroups per (board, template), one combo per thread.  Template
-// nodes, DFS events and the template's card sums are staged in LDS; the
-// regrets of all the template's slots are loaded up front.  Values flow up
-// through register stacks; regret matching, the DCFR-generalised regret
-// update and the strategy sum happen in the same pass.
+// ------------------------------------------------------------ forward ----
+// Two workgroups per (board, template), one combo per thread.  The regrets
+// of every slot of the template are loaded up front into an indexed
+// register file; the walk over the template's preorder events reads node
+// records through scalar loads (one event ahead).  At each decision node:
+// regret matching, reach propagation, and the strategy-sum update
+// S += w * reach * sigma as a no-return float atomic (one writer per entry).
+// At each terminal the reach of both players is added into per-card bins in
+// LDS with 64-bit fixed-point atomics (exact, order independent); each
+// workgroup writes its half's bins, and the backward adds the two halves.
 template <int LEVEL, int DM>
-__global__ __launch_bounds__(576) void k_bt_backward(BatchedDev D, const float* __restrict__ ranges, float wp,
-                                                     float wn, float ws) {
+__global__ __launch_bounds__(SWEEP_NT) void k_bt_forward(BatchedDev D, const float* __restrict__ ranges, float ws) {
+    typedef float vS __attribute__((ext_vector_type(DM * 4)));
+    typedef float vD __attribute__((ext_vector_type(DM)));
     const LevelDev& L = D.L[LEVEL];
-    extern __shared__ double ssum[];  // [maxterm][2][56]
-    __shared__ DNode tn[MAXN];
-    __shared__ int tev[MAXE];
-    const int np = L.np;
-    const int NT = blockDim.x;
-    const int inst = blockIdx.x >> 1;
+    extern __shared__ unsigned long long bins[];  // [nt][2][52]
+    __shared__ EvRec sev[MAXE];
+    const int inst = blockIdx.x >> 1, half = blockIdx.x & 1;
     const int b = inst / L.ntmpl, tm = inst % L.ntmpl;
     const int tid = threadIdx.x;
-    const int k = (blockIdx.x & 1) * NT + tid;
-    const bool live = k < L.nlive[b];
-    const int n0 = L.tnode0[tm], nn = L.tnode0[tm + 1] - n0;
-    const int e0 = L.tev0[tm], ne = L.tev0[tm + 1] - e0;
-    const int s0 = L.tslot0[tm], nsl = L.tslot0[tm + 1] - s0;
-    const int t0 = L.tterm0[tm], nt = L.tterm0[tm + 1] - t0;
-    for (int e = tid; e < nn; e += NT) tn[e] = L.nodes[n0 + e];
-    for (int e = tid; e < ne; e += NT) tev[e] = L.events[e0 + e];
-    const double* gs = L.sums + ((size_t)b * L.nterm + t0) * 2 * 56;
-    for (int e = tid; e < nt * 2 * 56; e += NT) ssum[e] = gs[e];
-    const int c1 = L.cc[((size_t)b * np + k) * 2], c2 = L.cc[((size_t)b * np + k) * 2 + 1];
-    // regrets of every slot of the template (old values, also used by the update)
-    float Ra[32], Rb2[32];
-    float* Rg = L.R + ((size_t)b * L.nslot + s0) * np + k;
-    float* Sg = L.S + ((size_t)b * L.nslot + s0) * np + k;
+    const int np = L.np;
+    const int k = half * SWEEP_NT + tid;
+    const bool inb = k < np;
+    const bool live = k < ldu(L.nlive + b);
+    const int e0 = ldu(L.tfev0 + tm), e1 = ldu(L.tfev0 + tm + 1);
+    const int s0 = ldu(L.tslot0 + tm), nsl = ldu(L.tslot0 + tm + 1) - s0;
+    const int t0 = ldu(L.tterm0 + tm), nt = ldu(L.tterm0 + tm + 1) - t0;
+    for (int e = tid; e < nt * 104; e += SWEEP_NT) bins[e] = 0ull;
+    ev_stage(sev, L.fev + e0, e1 - e0);
+    int c1 = 0, c2 = 0;
+    float root0 = 0.0f, root1 = 0.0f;
+    f32x32_t Ra, Rb;
+    const size_t rbase = ((size_t)b * L.nslot + s0) * np + k;
 #pragma unroll
     for (int x = 0; x < 32; ++x) {
-        Ra[x] = (live && x < nsl) ? Rg[(size_t)x * np] : 0.0f;
-        Rb2[x] = (live && x + 32 < nsl) ? Rg[(size_t)(x + 32) * np] : 0.0f;
+        Ra[x] = (live && x < nsl) ? L.R[rbase + (size_t)x * np] : 0.0f;
+        Rb[x] = (live && x + 32 < nsl) ? L.R[rbase + (size_t)(x + 32) * np] : 0.0f;
     }
-    float root0 = 0.0f, root1 = 0.0f;
     if (live) {
+        c1 = L.cc[((size_t)b * np + k) * 2];
+        c2 = L.cc[((size_t)b * np + k) * 2 + 1];
         root0 = root_reach(D, ranges, LEVEL, b, tm, np, k, 0);
         root1 = root_reach(D, ranges, LEVEL, b, tm, np, k, 1);
     }
     __syncthreads();
-    float r0[DM], r1[DM], sg[DM * 4], cv[DM * 4], op[DM], oS[DM * 4];
-    float out0 = 0.0f, out1 = 0.0f;
+    vD rs0, rs1;
+    vS sg;
+    const int ne = e1 - e0;
+    EvRaw nx = ev_fetch(sev, 0);
     for (int e = 0; e < ne; ++e) {
-        const int ev = __builtin_amdgcn_readfirstlane(tev[e]);
-        const int kind = ev & 3;
-        const DNode nd = uniform_node(&tn[ev >> 2]);
+        const EvRec ec = ev_uniform(nx);
+        if (e + 1 < ne) nx = ev_fetch(sev, e + 1);
+        const DNode& nd = ec.nd;
         const int d = nd.depth;
-        float x0, x1;  // reach of the node
-        if (d == 0) { x0 = root0; x1 = root1; }
-        else {
+        float x0, x1;
+        if (d == 0) {
+            x0 = root0;
+            x1 = root1;
+        } else {
             const float sv = sg[(d - 1) * 4 + nd.act];
-            x0 = nd.pad == 0 ? r0[d - 1] * sv : r0[d - 1];
-            x1 = nd.pad == 1 ? r1[d - 1] * sv : r1[d - 1];
+            const float p0 = rs0[d - 1], p1 = rs1[d - 1];
+            x0 = nd.pad == 0 ? p0 * sv : p0;
+            x1 = nd.pad == 1 ? p1 * sv : p1;
         }
-        if (kind == EV_ENTER) {
-            r0[d] = x0;
-            r1[d] = x1;
+        if (nd.type == DN_DEC) {
+            rs0[d] = x0;
+            rs1[d] = x1;
             const int ls = nd.slot - s0;
             float pos[4], sum = 0.0f;
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
-                const int q = ls + a;
-                const float rv = a < nd.nact ? sel64(Ra, Rb2, q) : 0.0f;
-                pos[a] = fmaxf(rv, 0.0f);
+                pos[a] = a < nd.nact ? fmaxf(rsel(Ra, Rb, ls + a), 0.0f) : 0.0f;
                 sum += pos[a];
-                oS[d * 4 + a] = (live && a < nd.nact) ? Sg[(size_t)q * np] : 0.0f;
             }
-            const float inv = sum > 1e-7f ? 1.0f / sum : 0.0f, un = 1.0f / (float)nd.nact;
+            const float inv = sum > 1e-7f ? __builtin_amdgcn_rcpf(sum) : 0.0f, un = uniform_share(nd.nact);
+            const float wr = ws * (nd.player == 0 ? x0 : x1);
+            float* Sg = L.S + ((size_t)b * L.nslot + nd.slot) * np + k;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const float sa = a < nd.nact ? (sum > 1e-7f ? pos[a] * inv : un) : 0.0f;
+                sg[d * 4 + a] = sa;
+                if (live && a < nd.nact) unsafeAtomicAdd(Sg + (size_t)a * np, wr * sa);
+            }
+            continue;
+        }
+        if (nd.type == DN_LEAF) {
+            if (LEVEL == 0 && inb) {
+                float* lr = D.leafreach + ((size_t)b * D.nleaf + nd.leaf) * 2 * NPT + k;
+                lr[0] = live ? x0 : 0.0f;
+                lr[NPT] = live ? x1 : 0.0f;
+            }
+            continue;
+        }
+        if (nd.type == DN_SD && inb) {
+            // player p's column holds the opponent's reach
+            const float o0 = live ? x1 : 0.0f, o1 = live ? x0 : 0.0f;
+            if (LEVEL == 1) {
+                const size_t cb = ((size_t)b * L.ncolp + nd.col) * NPR + k;
+                const __hip_bfloat16 h0 = __float2bfloat16(o0), h1 = __float2bfloat16(o1);
+                D.Bh[cb] = h0;
+                D.Bl[cb] = __float2bfloat16(o0 - __bfloat162float(h0));
+                D.Bh[cb + NPR] = h1;
+                D.Bl[cb + NPR] = __float2bfloat16(o1 - __bfloat162float(h1));
+            } else {
+                const size_t cb = ((size_t)b * L.ncolp + nd.col) * NPT + k;
+                D.B0[cb] = o0;
+                D.B0[cb + NPT] = o1;
+            }
+        }
+        if (live) {  // fold or showdown: card sums of both players' reaches
+            unsigned long long* bn = bins + (size_t)(nd.sidx - t0) * 104;
+            const unsigned long long v0 = (unsigned long long)__double2ull_rn((double)x0 * FX);
+            const unsigned long long v1 = (unsigned long long)__double2ull_rn((double)x1 * FX);
+            if (v0) {
+                atomicAdd(&bn[c1], v0);
+                atomicAdd(&bn[c2], v0);
+            }
+            if (v1) {
+                atomicAdd(&bn[52 + c1], v1);
+                atomicAdd(&bn[52 + c2], v1);
+            }
+        }
+    }
+    __syncthreads();
+    unsigned long long* out = L.part + ((size_t)b * L.nterm + t0) * 208 + half * 104;
+    for (int e = tid; e < nt * 104; e += SWEEP_NT) out[(size_t)(e / 104) * 208 + e % 104] = bins[e];
+}
+
+// ----------------------------------------------------------- backward ----
+// NH workgroups of NTB threads per (board, template), one combo per
+// thread.  The card sums of the template's terminals (both forward halves
+// added) and the showdown values of the template's columns (this
+// workgroup's combos) are
+// staged in LDS; the regrets of every slot of the template are loaded up
+// front into an indexed register file.  The walk over the template's DFS
+// events recomputes reaches top-down, takes terminal values from the
+// showdown values and the card sums, and applies the DCFR-generalised
+// regret update R <- (R + r) * (R + r > 0 ? wp : wn) at each decision node
+// on the way up.
+template <int LEVEL, int DM, int NTB>
+__global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* __restrict__ ranges, float wp, float wn) {
+    constexpr int NH = (NPT + NTB - 1) / NTB;  // workgroups per (board, template)
+    typedef float vS __attribute__((ext_vector_type(DM * 4)));
+    typedef float vD __attribute__((ext_vector_type(DM)));
+    const LevelDev& L = D.L[LEVEL];
+    extern __shared__ unsigned long long ssum[];  // [nt][2][53] (slot 52 = total), then float [ncolt][NTB]
+    __shared__ EvRec sev[MAXE];
+    const int inst = blockIdx.x / NH, half = blockIdx.x % NH;
+    const int b = inst / L.ntmpl, tm = inst % L.ntmpl;
+    const int tid = threadIdx.x;
+    const int np = L.np;
+    const int k = half * NTB + tid;
+    const bool live = k < ldu(L.nlive + b);
+    const int e0 = ldu(L.tev0 + tm), e1 = ldu(L.tev0 + tm + 1);
+    const int s0 = ldu(L.tslot0 + tm), nsl = ldu(L.tslot0 + tm + 1) - s0;
+    const int t0 = ldu(L.tterm0 + tm), nt = ldu(L.tterm0 + tm + 1) - t0;
+    const int col0 = ldu(L.tcol0 + tm), ncolt = ldu(L.tcol0 + tm + 1) - col0;
+    float* Cs = reinterpret_cast<float*>(ssum + (size_t)L.maxterm * 106) + tid;
+    {
+        const unsigned long long* P = L.part + ((size_t)b * L.nterm + t0) * 208;
+        for (int e = tid; e < nt * 104; e += NTB) {
+            const int tl = e / 104, r = e % 104;
+            ssum[tl * 106 + (r / 52) * 53 + r % 52] = P[(size_t)tl * 208 + r] + P[(size_t)tl * 208 + 104 + r];
+        }
+        const float* Cb = LEVEL == 1 ? D.C + ((size_t)b * L.ncolp + col0) * NPR + k
+                                     : D.C0 + ((size_t)b * L.ncolp + col0) * NPT + k;
+        const int cstride = LEVEL == 1 ? NPR : NPT;
+        for (int x = 0; x < ncolt; ++x) Cs[x * NTB] = live ? Cb[(size_t)x * cstride] : 0.0f;
+    }
+    ev_stage(sev, L.ev + e0, e1 - e0);
+    int c1 = 0, c2 = 0;
+    float root0 = 0.0f, root1 = 0.0f;
+    f32x32_t Ra, Rb;
+    const size_t rbase = ((size_t)b * L.nslot + s0) * np + k;
+#pragma unroll
+    for (int x = 0; x < 32; ++x) {
+        Ra[x] = (live && x < nsl) ? L.R[rbase + (size_t)x * np] : 0.0f;
+        Rb[x] = (live && x + 32 < nsl) ? L.R[rbase + (size_t)(x + 32) * np] : 0.0f;
+    }
+    if (live) {
+        c1 = L.cc[((size_t)b * np + k) * 2];
+        c2 = L.cc[((size_t)b * np + k) * 2 + 1];
+        root0 = root_reach(D, ranges, LEVEL, b, tm, np, k, 0);
+        root1 = root_reach(D, ranges, LEVEL, b, tm, np, k, 1);
+    }
+    __syncthreads();
+    for (int e = tid; e < nt * 2; e += NTB) {
+        unsigned long long t = 0;
+        const unsigned long long* sp = ssum + (size_t)(e >> 1) * 106 + (e & 1) * 53;
+        for (int c = 0; c < 52; ++c) t += sp[c];
+        ssum[(size_t)(e >> 1) * 106 + (e & 1) * 53 + 52] = t >> 1;  // each combo counted on two cards
+    }
+    __syncthreads();
+    vD rs0, rs1, op;
+    vS sg, cv;
+    float out0 = 0.0f, out1 = 0.0f;
+    float* Rg = L.R + rbase;
+    const int ne = e1 - e0;
+    EvRaw nx = ev_fetch(sev, 0);
+    for (int e = 0; e < ne; ++e) {
+        const EvRec ec = ev_uniform(nx);
+        if (e + 1 < ne) nx = ev_fetch(sev, e + 1);
+        const int kind = ec.kind;
+        const DNode& nd = ec.nd;
+        const int d = nd.depth;
+        float x0, x1;  // reach of the node
+        if (d == 0) {
+            x0 = root0;
+            x1 = root1;
+        } else {
+            const float sv = sg[(d - 1) * 4 + nd.act];
+            const float p0 = rs0[d - 1], p1 = rs1[d - 1];
+            x0 = nd.pad == 0 ? p0 * sv : p0;
+            x1 = nd.pad == 1 ? p1 * sv : p1;
+        }
+        if (kind == EV_ENTER) {
+            rs0[d] = x0;
+            rs1[d] = x1;
+            const int ls = nd.slot - s0;
+            float pos[4], sum = 0.0f;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                pos[a] = a < nd.nact ? fmaxf(rsel(Ra, Rb, ls + a), 0.0f) : 0.0f;
+                sum += pos[a];
+            }
+            const float inv = sum > 1e-7f ? __builtin_amdgcn_rcpf(sum) : 0.0f, un = uniform_share(nd.nact);
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
                 sg[d * 4 + a] = a < nd.nact ? (sum > 1e-7f ? pos[a] * inv : un) : 0.0f;
@@ -342,45 +404,36 @@ __global__ __launch_bounds__(576) void k_bt_backward(BatchedDev D, const float* 
         }
         float v0 = 0.0f, v1 = 0.0f;
         if (kind == EV_TERM) {
-            if (live) {
-                if (nd.type == DN_FOLD) {
-                    const int w = nd.player, l = 1 - w;
-                    const double* sw = ssum + ((size_t)(nd.sidx - t0) * 2 + w) * 56;
-                    const double* sl = ssum + ((size_t)(nd.sidx - t0) * 2 + l) * 56;
-                    const float cw = w == 0 ? nd.c0 : nd.c1, cl = w == 0 ? nd.c1 : nd.c0;
-                    const float xw = w == 0 ? x0 : x1, xl = w == 0 ? x1 : x0;
-                    const float vw = (nd.pot - cw) * mass_of(sl, c1, c2, xl);
-                    const float vl = -cl * mass_of(sw, c1, c2, xw);
-                    v0 = w == 0 ? vw : vl;
-                    v1 = w == 0 ? vl : vw;
-                } else if (nd.type == DN_SD) {
-                    const double* sa = ssum + ((size_t)(nd.sidx - t0) * 2 + 0) * 56;
-                    const double* sb = ssum + ((size_t)(nd.sidx - t0) * 2 + 1) * 56;
-                    const float m0 = mass_of(sb, c1, c2, x1), m1 = mass_of(sa, c1, c2, x0);
-                    if (LEVEL == 1) {
-                        const float* Cb = D.C + ((size_t)b * L.ncolp + nd.col) * NPR + k;
-                        v0 = 0.5f * nd.pot * Cb[0] + 0.5f * D.pot0 * m0;
-                        v1 = 0.5f * nd.pot * Cb[NPR] + 0.5f * D.pot0 * m1;
-                    } else {
-                        const float* Cb = D.C0 + ((size_t)b * L.ncolp + nd.col) * NPT + k;
-                        v0 = nd.pot * Cb[0] - nd.c0 * m0;
-                        v1 = nd.pot * Cb[NPT] - nd.c1 * m1;
-                    }
-                } else if (LEVEL == 0) {  // leaf: mean of the river-line roots over the 44 rivers
-                    const int rt = D.leaf_tmpl[nd.leaf];
-                    const int nt1 = D.L[1].ntmpl;
-                    float a0 = 0.0f, a1 = 0.0f;
-                    for (int rk = 0; rk < 48; ++rk) {
-                        const int bb = b * 48 + rk;
-                        const int ri = D.imap[(size_t)bb * NPT + k];
-                        if (ri == 0xFFFF) continue;
-                        const float* rv = D.rootval + (((size_t)bb * nt1 + rt) * 2) * NPR + ri;
-                        a0 += rv[0];
-                        a1 += rv[NPR];
-                    }
-                    v0 = a0 * (1.0f / 44.0f);
-                    v1 = a1 * (1.0f / 44.0f);
+            if (nd.type == DN_FOLD) {
+                const int w = nd.player;
+                const unsigned long long* sw = ssum + (size_t)(nd.sidx - t0) * 106 + w * 53;
+                const unsigned long long* sl = ssum + (size_t)(nd.sidx - t0) * 106 + (1 - w) * 53;
+                const float cw = w == 0 ? nd.c0 : nd.c1, cl = w == 0 ? nd.c1 : nd.c0;
+                const float xw = w == 0 ? x0 : x1, xl = w == 0 ? x1 : x0;
+                const float vw = (nd.pot - cw) * mass_u(sl, c1, c2, xl);
+                const float vl = -cl * mass_u(sw, c1, c2, xw);
+                v0 = w == 0 ? vw : vl;
+                v1 = w == 0 ? vl : vw;
+            } else if (nd.type == DN_SD) {
+                const unsigned long long* sa = ssum + (size_t)(nd.sidx - t0) * 106;
+                const float m0 = mass_u(sa + 53, c1, c2, x1), m1 = mass_u(sa, c1, c2, x0);
+                const int lc = nd.col - col0;
+                const float q0 = Cs[lc * NTB], q1 = Cs[(lc + 1) * NTB];
+                if (LEVEL == 1) {
+                    v0 = 0.5f * nd.pot * q0 + 0.5f * D.pot0 * m0;
+                    v1 = 0.5f * nd.pot * q1 + 0.5f * D.pot0 * m1;
+                } else {
+                    v0 = nd.pot * q0 - nd.c0 * m0;
+                    v1 = nd.pot * q1 - nd.c1 * m1;
                 }
+            } else if (LEVEL == 0 && live) {  // leaf: mean of the river-line roots over the rivers
+                const float* lv = D.leafval + ((size_t)b * D.nleaf + nd.leaf) * 2 * NPT + k;
+                v0 = lv[0];
+                v1 = lv[NPT];
+            }
+            if (!live) {
+                v0 = 0.0f;
+                v1 = 0.0f;
             }
         } else {  // EV_LEAVE of a decision node
             const int p = nd.player;
@@ -389,16 +442,13 @@ __global__ __launch_bounds__(576) void k_bt_backward(BatchedDev D, const float* 
             for (int a = 0; a < 4; ++a)
                 if (a < nd.nact) node_v = fmaf(sg[d * 4 + a], cv[d * 4 + a], node_v);
             if (live) {
-                const float rp = p == 0 ? x0 : x1;
                 const int ls = nd.slot - s0;
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
-                    if (a >= nd.nact) break;
-                    const int q = ls + a;
-                    const float rold = sel64(Ra, Rb2, q);
-                    const float qv = rold + (cv[d * 4 + a] - node_v);
-                    Rg[(size_t)q * np] = qv * (qv > 0.0f ? wp : wn);
-                    Sg[(size_t)q * np] = fmaf(ws * rp, sg[d * 4 + a], oS[d * 4 + a]);
+                    if (a < nd.nact) {
+                        const float qv = rsel(Ra, Rb, ls + a) + (cv[d * 4 + a] - node_v);
+                        Rg[(size_t)(ls + a) * np] = qv * (qv > 0.0f ? wp : wn);
+                    }
                 }
             }
             v0 = p == 0 ? node_v : op[d];
@@ -429,6 +479,28 @@ __global__ __launch_bounds__(576) void k_bt_backward(BatchedDev D, const float* 
     }
 }
 
+// Turn-leaf values: mean over the 44 rivers that leave the combo alive of
+// the river-line root values (rivers are summed in deck order).
+__global__ void k_bt_leafval(BatchedDev D) {
+    const int t = blockIdx.y / D.nleaf, leaf = blockIdx.y % D.nleaf;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= NPT) return;
+    const int rt = D.leaf_tmpl[leaf];
+    const int nt1 = D.L[1].ntmpl;
+    float a0 = 0.0f, a1 = 0.0f;
+    for (int rk = 0; rk < 48; ++rk) {
+        const int bb = t * 48 + rk;
+        const int ri = D.imap[(size_t)bb * NPT + k];
+        if (ri == 0xFFFF) continue;
+        const float* rv = D.rootval + (((size_t)bb * nt1 + rt) * 2) * NPR + ri;
+        a0 += rv[0];
+        a1 += rv[NPR];
+    }
+    float* lv = D.leafval + ((size_t)t * D.nleaf + leaf) * 2 * NPT + k;
+    lv[0] = a0 * (1.0f / 44.0f);
+    lv[NPT] = a1 * (1.0f / 44.0f);
+}
+
 // ------------------------------------------------------- river GEMM -------
 // C[col][i] = sum_j S[i][j] r_col[j] for one board and a 128-hero block.
 // MFMA operands: A = reach columns (hi or lo, from LDS), B = S^T generated
@@ -440,87 +512,9 @@ constexpr int GK = 32;          // K chunk (two MFMA k-steps)
 constexpr int GCOLS = 288;      // columns per board, padded (9 tiles of 32)
 constexpr int GROW = 40;        // LDS row stride in bf16 (80 bytes)
 
-__device__ __forceinline__ uint32_t bf16pair(int sa, int sb) {
-    // sign in {-1,0,1} -> bf16 bits (1.0 = 0x3F80, -1.0 = 0xBF80)
-    uint32_t a = sa > 0 ? 0x3F80u : (sa < 0 ? 0xBF80u : 0u);
-    uint32_t b2 = sb > 0 ? 0x3F80u : (sb < 0 ? 0xBF80u : 0u);
-    return a | (b2 << 16);
-}
-
 __device__ __forceinline__ bool conflict(uint32_t x, uint32_t y) {
     uint32_t v = x ^ y;  // a zero byte means a shared card
     return ((v - 0x01010101u) & ~v & 0x80808080u) != 0;
-}
-
-template <int NTILE>
-__global__ __launch_bounds__(256) void k_bt_gemm(BatchedDev D) {
-    constexpr int ncolp = NTILE * 32;
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][GCOLS][GROW];
-    __shared__ uint16_t gj_s[GK];
-    __shared__ uint32_t cy_s[GK];
-    const int b = blockIdx.y;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 128 + wave * 32 + (lane & 31);
-    const bool row_ok = i < NPR;
-    const int ic = row_ok ? i : 0;
-    const uint16_t gi = D.grank[(size_t)b * NPR + ic];
-    const uint32_t xi = D.cx[(size_t)b * NPR + ic];
-    const int half = lane >> 5;
-    f32x16_t acc[NTILE];
-#pragma unroll
-    for (int n = 0; n < NTILE; ++n)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[n][r] = 0.0f;
-    const __hip_bfloat16* Bh = D.Bh + (size_t)b * ncolp * NPR;
-    const __hip_bfloat16* Bl = D.Bl + (size_t)b * ncolp * NPR;
-    for (int j0 = 0; j0 < NPR; j0 += GK) {
-        // stage 32 K x ncolp columns (hi, lo): 4 x 16-byte pieces per column
-        for (int e = threadIdx.x; e < ncolp * 8; e += 256) {
-            int col = e >> 3, part = e & 7;  // part: hl*4 + quarter
-            int hl = part >> 2, qt = part & 3;
-            const __hip_bfloat16* src = (hl ? Bl : Bh) + (size_t)col * NPR + j0 + qt * 8;
-            *reinterpret_cast<uint4*>(&Bs[hl][col][qt * 8]) = *reinterpret_cast<const uint4*>(src);
-        }
-        if (threadIdx.x < GK) {
-            gj_s[threadIdx.x] = D.grank[(size_t)b * NPR + j0 + threadIdx.x];
-            cy_s[threadIdx.x] = D.cy[(size_t)b * NPR + j0 + threadIdx.x];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            // B operand: S^T[j][i] for j = j0 + 16s + 8*half + e
-            uint32_t w[4];
-#pragma unroll
-            for (int e2 = 0; e2 < 4; ++e2) {
-                int ja = 16 * s + 8 * half + 2 * e2;
-                int sa = conflict(xi, cy_s[ja]) ? 0 : ((int)gi > (int)gj_s[ja]) - ((int)gi < (int)gj_s[ja]);
-                int sb = conflict(xi, cy_s[ja + 1]) ? 0 : ((int)gi > (int)gj_s[ja + 1]) - ((int)gi < (int)gj_s[ja + 1]);
-                w[e2] = bf16pair(sa, sb);
-            }
-            bf16x8_t bop;
-            __builtin_memcpy(&bop, w, 16);
-#pragma unroll
-            for (int n = 0; n < NTILE; ++n) {
-                const int col = n * 32 + (lane & 31);
-                bf16x8_t ah, al;
-                __builtin_memcpy(&ah, &Bs[0][col][16 * s + 8 * half], 16);
-                __builtin_memcpy(&al, &Bs[1][col][16 * s + 8 * half], 16);
-                acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bop, acc[n], 0, 0, 0);
-                acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bop, acc[n], 0, 0, 0);
-            }
-        }
-        __syncthreads();
-    }
-    if (!row_ok) return;
-    float* Cb = D.C + (size_t)b * ncolp * NPR;
-#pragma unroll
-    for (int n = 0; n < NTILE; ++n) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            int col = n * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
-            Cb[(size_t)col * NPR + i] = acc[n][r];
-        }
-    }
 }
 
 // Sign-blocker matrix of every river board as bytes, precomputed once.
@@ -733,6 +727,15 @@ __global__ void k_bt_export(BatchedDev D, const ExportRec* __restrict__ recs, in
 }  // namespace
 
 // ================================================================ host ===
+// Backward sweep workgroup size: two workgroups of 576 cover the 1152
+// padded turn combos (and the 1088 river combos).  (384, three per board:
+// two fit a CU, but the extra staging of card sums costs more than the
+// overlap gains.)
+constexpr int BWD_NT = 576;
+constexpr int BWD_NH = (NPT + BWD_NT - 1) / BWD_NT;
+// LDS of the backward sweep: card sums, then the showdown values.
+static size_t bwd_lds(const LevelDev& L) { return (size_t)L.maxterm * 106 * 8 + (size_t)L.maxcolt * BWD_NT * 4; }
+
 struct BatchedState {
     BatchedDev D;
     DeviceArena arena;
@@ -942,39 +945,59 @@ int batched_create(SolverImpl& s) {
         if (L.ncolp == 0) L.ncolp = 32;
         L.ntmpl = (int)H.tnode0.size() - 1;
         int r = MTT_OK;
-        DNode* dn;
-        int *t0, *e0, *ev, *nl, *ts, *tt;
-        uint8_t* c;
-        std::vector<int> slot0 = H.slot0, term0 = H.term0;
+        std::vector<int> slot0 = H.slot0, term0 = H.term0, col0 = H.col0;
         slot0.push_back(H.nslot);
         term0.push_back(H.nterm);
+        col0.push_back(H.ncol);
         L.maxterm = 0;
+        L.maxcolt = 0;
         for (int t = 0; t < L.ntmpl; ++t) {
             L.maxterm = std::max(L.maxterm, term0[t + 1] - term0[t]);
+            L.maxcolt = std::max(L.maxcolt, col0[t + 1] - col0[t]);
             MTT_ARG(H.tnode0[t + 1] - H.tnode0[t] <= MAXN, "template has more than %d nodes", MAXN);
             MTT_ARG(H.tev0[t + 1] - H.tev0[t] <= MAXE, "template has more than %d events", MAXE);
             MTT_ARG(slot0[t + 1] - slot0[t] <= MAXSLOT, "template has more than %d action slots", MAXSLOT);
         }
-        r = r ? r : A.upload(H.nodes, &dn);
-        r = r ? r : A.upload(H.tnode0, &t0);
+        // event records: node + kind, in DFS order per template
+        std::vector<EvRec> recs(H.events.size());
+        for (int t = 0; t < L.ntmpl; ++t)
+            for (int e = H.tev0[t]; e < H.tev0[t + 1]; ++e) {
+                memset(&recs[e], 0, sizeof(EvRec));
+                recs[e].nd = H.nodes[H.tnode0[t] + (H.events[e] >> 2)];
+                recs[e].kind = H.events[e] & 3;
+            }
+        std::vector<EvRec> frecs;
+        std::vector<int> tf0{0};
+        for (int t = 0; t < L.ntmpl; ++t) {
+            for (int e = H.tev0[t]; e < H.tev0[t + 1]; ++e)
+                if (recs[e].kind != EV_LEAVE) frecs.push_back(recs[e]);
+            tf0.push_back((int)frecs.size());
+        }
+        EvRec *dev, *dfev;
+        int *e0, *f0, *nl, *ts, *tt, *tc;
+        uint8_t* c;
+        r = r ? r : A.upload(recs, &dev);
+        r = r ? r : A.upload(frecs, &dfev);
+        r = r ? r : A.upload(tf0, &f0);
         r = r ? r : A.upload(H.tev0, &e0);
-        r = r ? r : A.upload(H.events, &ev);
         r = r ? r : A.upload(slot0, &ts);
         r = r ? r : A.upload(term0, &tt);
+        r = r ? r : A.upload(col0, &tc);
         r = r ? r : A.upload(cc, &c);
         r = r ? r : A.upload(nlive, &nl);
         if (r) return r;
-        L.nodes = dn; L.tnode0 = t0; L.tev0 = e0; L.events = ev; L.tslot0 = ts; L.tterm0 = tt; L.cc = c; L.nlive = nl;
+        MTT_ARG(bwd_lds(L) <= 160 * 1024, "template needs %zu bytes of LDS in the backward sweep", bwd_lds(L));
+        L.ev = dev; L.tev0 = e0; L.fev = dfev; L.tfev0 = f0; L.tslot0 = ts; L.tterm0 = tt; L.tcol0 = tc; L.cc = c; L.nlive = nl;
         size_t ns = (size_t)nboards * L.nslot * np;
         r = r ? r : A.alloc(ns, &L.R);
         r = r ? r : A.alloc(ns, &L.S);
-        r = r ? r : A.alloc((size_t)nboards * std::max(L.nterm, 1) * 2 * 56, &L.sums);
+        r = r ? r : A.alloc((size_t)nboards * std::max(L.nterm, 1) * 208, &L.part);
         if (r) return r;
         MTT_HIP(hipMemsetAsync(L.R, 0, ns * sizeof(float), s.stream));
         MTT_HIP(hipMemsetAsync(L.S, 0, ns * sizeof(float), s.stream));
         return MTT_OK;
     };
-    auto pick = [](int d) { return d + 1 <= 4 ? 4 : (d + 1 <= 6 ? 6 : 8); };
+    auto pick = [](int d) { return d + 1 <= 4 ? 4 : 8; };
     B.dm[0] = pick(H0.maxdepth);
     B.dm[1] = pick(H1.maxdepth);
     rc = up_level(H1, B.D.L[1], NPR, nb1, cc1, nlive1);
@@ -1020,6 +1043,7 @@ int batched_create(SolverImpl& s) {
         rc = rc ? rc : A.upload(leaf_tmpl, &d_lt);
         rc = rc ? rc : A.upload(tmpl_leaf, &d_tl);
         rc = rc ? rc : A.alloc((size_t)49 * B.D.nleaf * 2 * NPT, &B.D.leafreach);
+        rc = rc ? rc : A.alloc((size_t)49 * B.D.nleaf * 2 * NPT, &B.D.leafval);
         rc = rc ? rc : A.alloc((size_t)nb1 * L1.ntmpl * 2 * NPR, &B.D.rootval);
         rc = rc ? rc : A.alloc((size_t)49 * 2 * NPT, &B.D.rootval0);
         rc = rc ? rc : A.alloc((size_t)49 * L0.ncolp * NPT, &B.D.B0);
@@ -1029,6 +1053,7 @@ int batched_create(SolverImpl& s) {
         if (rc) return rc;
         B.D.leaf_tmpl = d_lt; B.D.tmpl_leaf = d_tl; B.D.Eturn = d_E;
         MTT_HIP(hipMemsetAsync(B.D.leafreach, 0, (size_t)49 * B.D.nleaf * 2 * NPT * 4, s.stream));
+        MTT_HIP(hipMemsetAsync(B.D.leafval, 0, (size_t)49 * B.D.nleaf * 2 * NPT * 4, s.stream));
         MTT_HIP(hipMemsetAsync(B.D.B0, 0, (size_t)49 * L0.ncolp * NPT * 4, s.stream));
         // turn-compact -> root inverse, and the turn equity matrices
         std::vector<uint16_t> inv0((size_t)49 * kPadFlop, 0xFFFF);
@@ -1108,20 +1133,18 @@ int batched_run(SolverImpl& s, int iters) {
         const float* ranges = B.d_ranges;
         if (B.two_level) {
             const LevelDev& L0 = B.D.L[0];
-            const size_t sh = (size_t)2 * L0.maxterm * 52 * 8;
+            const size_t sh = (size_t)L0.maxterm * 104 * 8;
+            const dim3 g(49 * L0.ntmpl * 2);
             MTT_TIMED(s, "turn_forward",
-                      if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_forward<0, 4>), dim3(49 * L0.ntmpl), dim3(NPT / 2), sh, s.stream, B.D, ranges);
-                      else if (B.dm[0] == 6) hipLaunchKernelGGL((k_bt_forward<0, 6>), dim3(49 * L0.ntmpl), dim3(NPT / 2), sh, s.stream, B.D,
-                                                                 ranges);
-                      else hipLaunchKernelGGL((k_bt_forward<0, 8>), dim3(49 * L0.ntmpl), dim3(NPT / 2), sh, s.stream, B.D, ranges));
+                      if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_forward<0, 4>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws);
+                      else hipLaunchKernelGGL((k_bt_forward<0, 8>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws));
         }
         {
-            const size_t sh = (size_t)2 * L1.maxterm * 52 * 8;
+            const size_t sh = (size_t)L1.maxterm * 104 * 8;
+            const dim3 g(B.nb1 * L1.ntmpl * 2);
             MTT_TIMED(s, "river_forward",
-                      if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_forward<1, 4>), dim3(B.nb1 * L1.ntmpl), dim3(NPR / 2), sh, s.stream, B.D, ranges);
-                      else if (B.dm[1] == 6) hipLaunchKernelGGL((k_bt_forward<1, 6>), dim3(B.nb1 * L1.ntmpl), dim3(NPR / 2), sh, s.stream, B.D,
-                                                                 ranges);
-                      else hipLaunchKernelGGL((k_bt_forward<1, 8>), dim3(B.nb1 * L1.ntmpl), dim3(NPR / 2), sh, s.stream, B.D, ranges));
+                      if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_forward<1, 4>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws);
+                      else hipLaunchKernelGGL((k_bt_forward<1, 8>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws));
         }
         ::mtt::KernelTimer::Mark gm;
         if (s.timer.enabled) s.timer.begin("river_showdown_gemm", s.stream, gm);
@@ -1142,28 +1165,24 @@ int batched_run(SolverImpl& s, int iters) {
         }
         if (s.timer.enabled) s.timer.end(gm, s.stream);
         {
-            const size_t sh = (size_t)L1.maxterm * 2 * 56 * 8;
+            const size_t sh = bwd_lds(L1);
+            const dim3 g(B.nb1 * L1.ntmpl * BWD_NH);
             MTT_TIMED(s, "river_backward",
-                      if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_backward<1, 4>), dim3(B.nb1 * L1.ntmpl * 2), dim3(NPR / 2), sh, s.stream, B.D, ranges, wp,
-                                                           wn, ws);
-                      else if (B.dm[1] == 6) hipLaunchKernelGGL((k_bt_backward<1, 6>), dim3(B.nb1 * L1.ntmpl * 2), dim3(NPR / 2), sh, s.stream, B.D,
-                                                                 ranges, wp, wn, ws);
-                      else hipLaunchKernelGGL((k_bt_backward<1, 8>), dim3(B.nb1 * L1.ntmpl * 2), dim3(NPR / 2), sh, s.stream, B.D, ranges, wp, wn,
-                                              ws));
+                      if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_backward<1, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn);
+                      else hipLaunchKernelGGL((k_bt_backward<1, 8, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn));
         }
         if (B.two_level) {
             const LevelDev& L0 = B.D.L[0];
+            MTT_TIMED(s, "turn_leafval",
+                      hipLaunchKernelGGL(k_bt_leafval, dim3(NPT / 256 + 1, 49 * B.D.nleaf), dim3(256), 0, s.stream, B.D));
             MTT_TIMED(s, "turn_showdown",
                       hipLaunchKernelGGL(k_bt_turn_sd<16>, dim3(NPT / 256 + 1, 49), dim3(256), 0, s.stream, B.D,
                                          L0.ncol));
-            const size_t sh = (size_t)L0.maxterm * 2 * 56 * 8;
+            const size_t sh = bwd_lds(L0);
+            const dim3 g(49 * L0.ntmpl * BWD_NH);
             MTT_TIMED(s, "turn_backward",
-                      if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_backward<0, 4>), dim3(49 * L0.ntmpl * 2), dim3(NPT / 2), sh, s.stream, B.D, ranges, wp,
-                                                           wn, ws);
-                      else if (B.dm[0] == 6) hipLaunchKernelGGL((k_bt_backward<0, 6>), dim3(49 * L0.ntmpl * 2), dim3(NPT / 2), sh, s.stream, B.D,
-                                                                 ranges, wp, wn, ws);
-                      else hipLaunchKernelGGL((k_bt_backward<0, 8>), dim3(49 * L0.ntmpl * 2), dim3(NPT / 2), sh, s.stream, B.D, ranges, wp, wn,
-                                              ws));
+                      if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_backward<0, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn);
+                      else hipLaunchKernelGGL((k_bt_backward<0, 8, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn));
         }
         MTT_HIP(hipGetLastError());
         s.iters_done = t;
