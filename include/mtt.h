@@ -265,6 +265,105 @@ int mtt_solver_kernel_times(mtt_solver* h, int idx, char* name, int name_cap, do
 int mtt_solver_info(mtt_solver* h, int64_t* info);
 int mtt_solver_destroy(mtt_solver* h);
 
+/* -------------------------------------------------- fused DCFR step -- */
+/* kernel_dcfr_step, preflop_grid_generator.cu:387-410, over n entries with
+ * action stride `stride` (the reference: n = 169 classes, stride 192):
+ * v = sum_a sigma_a cfv_a; R_a <- (R_a + cfv_a - v) * (> 0 ? w_pos : w_neg);
+ * S_a += w_strat * reach * sigma_a; sigma <- R+ / sum R+ (uniform when the
+ * sum is <= 1e-7).  w_neg = 0 is the reference's floor-then-scale form.
+ * Device pointers: R, sigma, S, cfv [num_actions][stride], reach [n]. */
+int mtt_dcfr_step(int num_actions, float* d_regrets, float* d_current_strat, float* d_strat_sums,
+                  const float* d_action_cfvs, const float* d_reach, float w_pos, float w_strat, float w_neg, int n,
+                  int stride, hipStream_t stream);
+
+/* ------------------------------------------------------- 4-way engine -- */
+/* Single-street 4-player flop game (multiway_4way_engine.cu).  Per-action
+ * state uses the reference's stride of 3 actions:
+ * regrets/strategy/sums [infoset][3][1216], reach/cfv [node][4][1216]. */
+#define MTT_4WAY_MAX_ACTIONS 3
+typedef struct mtt_4way_game {
+    float pot0[4];        /* chips each seat has in the pot before the street */
+    float stack[4];       /* chips behind at the start of the street */
+    mtt_street_menu menu;
+} mtt_4way_game;
+/* Betting tree (seat order, bets capped at the smallest stack in the hand,
+ * breadth-first layout, one infoset per decision node). */
+int mtt_4way_tree_size(const mtt_4way_game* game, int* n_nodes, int* n_infosets);
+int mtt_4way_tree_build(const mtt_4way_game* game, mtt_flat_node4* nodes, int capacity, int* n_nodes);
+
+/* Flop board tables for the multiway showdown: the reference's
+ * combo_rank_pos (multiway_4way_engine.cu:85,137-146) plus what the card
+ * removal and ties need.  Strength = the reference's score of the 5 flop
+ * cards; positions ascend in strength. */
+typedef struct mtt_board4 {
+    uint8_t c1[1216], c2[1216];        /* combo cards (1176 alive) */
+    uint16_t rank_pos[1216];           /* combo -> position (= combo_rank_pos) */
+    uint16_t pos_combo[1216];          /* position -> combo */
+    uint16_t grp_lo[1216], grp_hi[1216];  /* tie group of a combo: [lo, hi) positions */
+    uint16_t card_list[52][48];        /* combos holding a card, ascending position */
+    uint8_t n_card[52];
+    uint8_t card_lt[1216][2];          /* entries of each card's list strictly weaker */
+    uint8_t card_le[1216][2];          /* entries weaker or tied */
+} mtt_board4;
+int mtt_board4_build(const uint8_t flop[3], mtt_board4* out);
+
+/* kernel_4way_forward_sweep, multiway_4way_engine.cu:41-83. */
+int mtt_4way_forward_sweep(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node4* d_nodes,
+                           const float* d_cumulative_regrets, float* d_current_strategy, float* d_reach_probs,
+                           hipStream_t stream);
+/* kernel_4way_evaluate_terminals, multiway_4way_engine.cu:85-217, with
+ * card removal against the hero's hand and split pots for ties:
+ * fold: pay * prod_{o != p} W_o; showdown, active p:
+ *   inact * (g * prod F_o + sum_{tied T} share_T * prod_T E_o prod_rest F_o
+ *            + l * (prod W_o - prod G_o)),
+ * F/G/E/W = opponent reach strictly weaker / weaker-or-tied / tied / total,
+ * each without combos that share a card with the hero;
+ * g = pot - c_p, l = -bf[p][p] c_p, share = pot/(|T|+1) - c_p (scaled by
+ * bf[p][p] when negative); inactive p: -bf[p][p] c_p prod W_o.
+ * d_board: device copy of mtt_board4; d_bubble_factors 4x4 floats. */
+int mtt_4way_evaluate_terminals(const int* d_terminal_node_indices, int num_terminals, const mtt_flat_node4* d_nodes,
+                                const float* d_reach_probs, const mtt_board4* d_board, const float* d_bubble_factors,
+                                float* d_cfvs, hipStream_t stream);
+/* kernel_4way_backward_cfr_plus, multiway_4way_engine.cu:219-257. */
+int mtt_4way_backward_cfr_plus(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node4* d_nodes,
+                               const float* d_current_strategy, const float* d_reach_probs, float* d_cfvs,
+                               float* d_cumulative_regrets, float* d_cumulative_strategy_sums, float iter_weight,
+                               hipStream_t stream);
+/* The same with the DCFR-generalised update (w_pos, w_neg, w_strat). */
+int mtt_4way_backward_dcfr(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node4* d_nodes,
+                           const float* d_current_strategy, const float* d_reach_probs, float* d_cfvs,
+                           float* d_cumulative_regrets, float* d_cumulative_strategy_sums, float w_pos, float w_neg,
+                           float w_strat, hipStream_t stream);
+
+/* 4-way solves of one tree over many flops (new: the reference's main is a
+ * stub, multiway_4way_engine.cu:259-262).  ranges: [n_flops][4][1216] in
+ * each flop's alive-combo order; bubble_factors 4x4 (row p, column q). */
+typedef struct mtt_multi mtt_multi;
+int mtt_multi_create(const mtt_4way_game* game, int n_flops, const uint8_t* flops, const float* ranges,
+                     const float* bubble_factors, const mtt_solver_cfg* cfg, mtt_multi** out);
+int mtt_multi_run(mtt_multi* h, int iters);
+/* Raw regrets/strategy sums of one flop [infoset][3][1216] and its root
+ * values [4][1216]. */
+int mtt_multi_get_state(mtt_multi* h, int flop, float* R_out, float* S_out, float* root_cfv_out);
+int mtt_multi_timing(mtt_multi* h, int enable);
+int mtt_multi_kernel_times(mtt_multi* h, int idx, char* name, int name_cap, double* total_ms, int64_t* launches);
+int mtt_multi_destroy(mtt_multi* h);
+
+/* ------------------------------------------------------------- ICM -- */
+/* Malmuth-Harville equities in FP64 on the GPU (absent from the reference,
+ * whose bubble_factors are an input, multiway_4way_engine.cu:85,128,201,204).
+ * Host arrays: stacks[n] (n <= 11), payouts[k]; out_equity[n]. */
+int mtt_icm_mh_fp64(const double* stacks, int n, const double* payouts, int k, double* out_equity);
+/* Batched on device: d_stacks [count][n] -> d_equity [count][n]. */
+int mtt_icm_mh_fp64_batch(const double* d_stacks, int count, int n, const double* d_payouts, int k,
+                          double* d_equity, hipStream_t stream);
+/* Bubble factors of the m players in_pot[] (host arrays): for p != q with
+ * e = min(S_p, S_q), bf[p][q] = (EQ_p - EQ_p after losing e to q) /
+ * (EQ_p after winning e from q - EQ_p); bf[p][p] = mean of bf[p][q] over
+ * the others in the pot.  out_bf [m][m]. */
+int mtt_icm_bubble_factors_fp64(const double* stacks, int n, const double* payouts, int k, const int* in_pot, int m,
+                                double* out_bf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,37 @@ void or_hu_exploitability(const or_hu* h, const float* strat, double pot, double
 void or_hu_terminal(or_hu* h, int node, const uint8_t* dealt, int nd, const double* r0, const double* r1,
                     double* v0, double* v1);
 
+/* 4-way engine (or_multi.c).  Node layout of multiway_4way_engine.cu:20-24. */
+typedef struct or_node4 {
+    uint8_t type, player, num_actions, active_mask;
+    int32_t children_offset, infoset_offset;
+    float pot_size, committed[4];
+} __attribute__((aligned(16))) or_node4;
+/* Flop board tables: combos, strength order, tie groups, per-card lists. */
+typedef struct or_board4 {
+    uint8_t c1[1216], c2[1216];
+    uint16_t rank_pos[1216], pos_combo[1216], grp_lo[1216], grp_hi[1216];
+    uint16_t card_list[52][48];
+    uint8_t n_card[52];
+    uint8_t card_lt[1216][2], card_le[1216][2];
+} or_board4;
+uint32_t or_evaln(const uint8_t* c, int n);
+void or_4way_board(const uint8_t flop[3], or_board4* B);
+void or_4way_terminal(const or_board4* B, const or_node4* nd, const float* reach, const double* bf, double* out);
+void or_4way_terminal_brute(const or_board4* B, const or_node4* nd, const float* reach, const double* bf,
+                            const uint32_t* score, double* out);
+void or_dcfr_step(int num_actions, int n, int stride, float* R, float* sigma, float* S, const float* cfv,
+                  const float* reach, float w_pos, float w_neg, float w_strat);
+void or_icm(const double* stacks, int n, const double* payouts, int k, double* eq);
+void or_icm_brute(const double* stacks, int n, const double* payouts, int k, double* eq);
+void or_bubble_factors(const double* stacks, int n, const double* payouts, int k, const int* in_pot, int m, double* bf);
+typedef struct or_4way_s or_4way;
+or_4way* or_4way_create(const or_node4* nodes, int n_nodes, int n_infosets, const uint8_t flop[3], const float* ranges,
+                        const double* bf);
+void or_4way_destroy(or_4way* h);
+void or_4way_iterate(or_4way* h, int iters, int t0, float alpha, float beta, float gamma, int scheme);
+void or_4way_state(const or_4way* h, float* R, float* S, float* root_cfv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -142,3 +142,133 @@ class HU:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ 4-way (or_multi.c)
+BOARD4_DTYPE = np.dtype([("c1", "u1", (1216,)), ("c2", "u1", (1216,)), ("rank_pos", "<u2", (1216,)),
+                         ("pos_combo", "<u2", (1216,)), ("grp_lo", "<u2", (1216,)), ("grp_hi", "<u2", (1216,)),
+                         ("card_list", "<u2", (52, 48)), ("n_card", "u1", (52,)), ("card_lt", "u1", (1216, 2)),
+                         ("card_le", "u1", (1216, 2))])
+
+
+def _multi_sigs():
+    L = lib()
+    if getattr(L, "_multi_ready", False):
+        return L
+    L.or_evaln.argtypes = [_P, ctypes.c_int]
+    L.or_evaln.restype = ctypes.c_uint32
+    L.or_4way_board.argtypes = [_P, _P]
+    L.or_4way_terminal.argtypes = [_P, _P, _P, _P, _P]
+    L.or_4way_terminal_brute.argtypes = [_P, _P, _P, _P, _P, _P]
+    L.or_dcfr_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float]
+    L.or_icm.argtypes = [_P, ctypes.c_int, _P, ctypes.c_int, _P]
+    L.or_icm_brute.argtypes = [_P, ctypes.c_int, _P, ctypes.c_int, _P]
+    L.or_bubble_factors.argtypes = [_P, ctypes.c_int, _P, ctypes.c_int, _P, ctypes.c_int, _P]
+    L.or_4way_create.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
+    L.or_4way_create.restype = _P
+    L.or_4way_destroy.argtypes = [_P]
+    L.or_4way_iterate.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                  ctypes.c_int]
+    L.or_4way_state.argtypes = [_P, _P, _P, _P]
+    L._multi_ready = True
+    return L
+
+
+def evaln(cards):
+    c = np.ascontiguousarray(cards, np.uint8)
+    return int(_multi_sigs().or_evaln(c.ctypes.data, len(c)))
+
+
+def board4(flop):
+    b = np.zeros(1, BOARD4_DTYPE)
+    f = np.ascontiguousarray(flop, np.uint8)
+    _multi_sigs().or_4way_board(f.ctypes.data, b.ctypes.data)
+    return b
+
+
+def terminal4(board, node, reach, bf):
+    """node: one record of the 4-way node dtype; reach [4][1216] float32;
+    bf [4][4] float64.  Returns [4][1216] float64."""
+    out = np.zeros(4 * 1216, np.float64)
+    nd = np.ascontiguousarray(node)
+    r = np.ascontiguousarray(reach, np.float32)
+    b = np.ascontiguousarray(bf, np.float64)
+    _multi_sigs().or_4way_terminal(board.ctypes.data, nd.ctypes.data, r.ctypes.data, b.ctypes.data, out.ctypes.data)
+    return out.reshape(4, 1216)
+
+
+def terminal4_brute(board, node, reach, bf, score):
+    out = np.zeros(4 * 1216, np.float64)
+    nd = np.ascontiguousarray(node)
+    r = np.ascontiguousarray(reach, np.float32)
+    b = np.ascontiguousarray(bf, np.float64)
+    sc = np.ascontiguousarray(score, np.uint32)
+    _multi_sigs().or_4way_terminal_brute(board.ctypes.data, nd.ctypes.data, r.ctypes.data, b.ctypes.data,
+                                         sc.ctypes.data, out.ctypes.data)
+    return out.reshape(4, 1216)
+
+
+def dcfr_step(num_actions, R, sigma, S, cfv, reach, w_pos, w_neg, w_strat):
+    """In place on float32 arrays [num_actions][stride]; reach [n]."""
+    n = len(reach)
+    stride = R.shape[1]
+    _multi_sigs().or_dcfr_step(num_actions, n, stride, R.ctypes.data, sigma.ctypes.data, S.ctypes.data,
+                               cfv.ctypes.data, reach.ctypes.data, w_pos, w_neg, w_strat)
+
+
+def icm(stacks, payouts, brute=False):
+    st = np.ascontiguousarray(stacks, np.float64)
+    pay = np.ascontiguousarray(payouts, np.float64)
+    out = np.zeros(len(st), np.float64)
+    fn = _multi_sigs().or_icm_brute if brute else _multi_sigs().or_icm
+    fn(st.ctypes.data, len(st), pay.ctypes.data, len(pay), out.ctypes.data)
+    return out
+
+
+def bubble_factors(stacks, payouts, in_pot):
+    st = np.ascontiguousarray(stacks, np.float64)
+    pay = np.ascontiguousarray(payouts, np.float64)
+    ip = np.ascontiguousarray(in_pot, np.int32)
+    out = np.zeros((len(ip), len(ip)), np.float64)
+    _multi_sigs().or_bubble_factors(st.ctypes.data, len(st), pay.ctypes.data, len(pay), ip.ctypes.data, len(ip),
+                                    out.ctypes.data)
+    return out
+
+
+class FourWay:
+    """FP32 4-way CFR+/DCFR loop of one flop (or_multi.c)."""
+
+    def __init__(self, nodes, n_infosets, flop, ranges, bf):
+        L = _multi_sigs()
+        self._nodes = np.ascontiguousarray(nodes)
+        self.n_info = n_infosets
+        f = np.ascontiguousarray(flop, np.uint8)
+        r = np.ascontiguousarray(ranges, np.float32)
+        b = np.ascontiguousarray(bf, np.float64)
+        self.h = L.or_4way_create(self._nodes.ctypes.data, len(self._nodes), n_infosets, f.ctypes.data, r.ctypes.data,
+                                  b.ctypes.data)
+        self.t = 0
+
+    def iterate(self, n, scheme=1, alpha=1.5, beta=0.0, gamma=2.0):
+        _multi_sigs().or_4way_iterate(self.h, n, self.t + 1, alpha, beta, gamma, scheme)
+        self.t += n
+
+    def state(self):
+        k = self.n_info * 3 * 1216
+        R = np.zeros(k, np.float32)
+        S = np.zeros(k, np.float32)
+        root = np.zeros(4 * 1216, np.float32)
+        _multi_sigs().or_4way_state(self.h, R.ctypes.data, S.ctypes.data, root.ctypes.data)
+        return R, S, root
+
+    def close(self):
+        if self.h:
+            _multi_sigs().or_4way_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
