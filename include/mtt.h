@@ -149,6 +149,21 @@ int mtt_gen_river_matrices(const mtt_alive_combo* d_combos, const uint8_t* d_dec
                            const uint8_t* d_flop, const int16_t* boards, int n_boards,
                            float* d_out, mtt_stream_t stream);
 
+/* kernel_gen_100_flop_matrices, preflop_grid_generator.cu:130-183: for
+ * each flop f, hero h and villain j among the caller's first 1176 combos,
+ * eq = (2W + T/2)/(2n) over the sampled runouts of the hero's 47-card deck
+ * (turn index 0,2,..,46; river r1+1, r1+4, .. < 47; runouts holding a
+ * villain card skipped), the reference's quarter weight of ties kept.
+ * Output transposed: d_flop_matrices[(f*1216 + j)*1216 + h]; blocked
+ * villains 0; heroes holding a flop card are not written (as in the
+ * reference).  preflop_grid_generator.cu:30 flop record: */
+typedef struct mtt_subset_flop {
+    uint8_t c1, c2, c3, pad;
+    float weight;
+} mtt_subset_flop;
+int mtt_gen_flop_matrices(const mtt_subset_flop* d_flops, int n_flops, const mtt_alive_combo* d_combos,
+                          float* d_flop_matrices, mtt_stream_t stream);
+
 /* ------------------------------------------------------- HU game trees -- */
 /* Bet menu of one street.  The reference has no bet-menu config in code
  * (menus exist only as README.md:10 prose and MAX_ACTIONS); this struct

@@ -356,6 +356,67 @@ __global__ void k_river_matrix(const mtt_alive_combo* __restrict__ combos, const
 using namespace mtt;
 
 // ================================================================ C ABI ===
+
+// ---------------------------------------------- preflop flop matrices ----
+// kernel_gen_100_flop_matrices (preflop_grid_generator.cu:130-183): one
+// workgroup per (hero, flop).  The hero's 47-card deck and its scores on
+// the sampled runouts (turn deck index 0,2,..,46; river r1+1, r1+4, .. < 47)
+// are staged in LDS once; threads stride over villains and evaluate only
+// the villain's hand.  eq = (2W + T/2) / (2n) as in the reference (a tie
+// counts a quarter).  Heroes that hold a flop card write nothing; blocked
+// villains get 0; output transposed E^T[f][j][h].
+constexpr int FM_PTS = 200;
+__global__ __launch_bounds__(256) void k_flop_matrices(const mtt_subset_flop* __restrict__ flops,
+                                                       const mtt_alive_combo* __restrict__ combos,
+                                                       float* __restrict__ out) {
+    __shared__ uint32_t hs[FM_PTS];
+    __shared__ uint64_t pm[FM_PTS];
+    __shared__ uint64_t pb[FM_PTS];
+    __shared__ uint8_t deck[47];
+    __shared__ int npts;
+    const int h = blockIdx.x, f = blockIdx.y;
+    const mtt_subset_flop fl = flops[f];
+    const uint64_t fmask = (1ull << fl.c1) | (1ull << fl.c2) | (1ull << fl.c3);
+    const mtt_alive_combo hero = combos[h];
+    if (hero.mask & fmask) return;
+    const uint64_t fb = card_bits16(fl.c1) | card_bits16(fl.c2) | card_bits16(fl.c3);
+    if (threadIdx.x == 0) {
+        int d = 0;
+        for (int c = 0; c < 52; ++c)
+            if (!((hero.mask | fmask) & (1ull << c))) deck[d++] = (uint8_t)c;
+        int n = 0;
+        for (int r1 = 0; r1 < 47; r1 += 2)
+            for (int r2 = r1 + 1; r2 < 47; r2 += 3) {
+                pm[n] = (1ull << deck[r1]) | (1ull << deck[r2]);
+                pb[n] = card_bits16(deck[r1]) | card_bits16(deck[r2]);
+                n++;
+            }
+        npts = n;
+    }
+    __syncthreads();
+    const int np = npts;
+    for (int p = threadIdx.x; p < np; p += blockDim.x)
+        hs[p] = score_packed(fb | card_bits16(hero.c1) | card_bits16(hero.c2) | pb[p]);
+    __syncthreads();
+    for (int j = threadIdx.x; j < kAliveFlop; j += blockDim.x) {
+        const mtt_alive_combo v = combos[j];
+        float eq = 0.0f;
+        if (!((hero.mask & v.mask) || (v.mask & fmask))) {
+            const uint64_t vb = fb | card_bits16(v.c1) | card_bits16(v.c2);
+            int wins = 0, ties = 0, total = 0;
+            for (int p = 0; p < np; ++p) {
+                if (v.mask & pm[p]) continue;
+                const uint32_t sv = score_packed(vb | pb[p]), sh = hs[p];
+                if (sh > sv) wins += 2;
+                else if (sh == sv) ties += 1;
+                total += 2;
+            }
+            eq = total > 0 ? ((float)wins + 0.5f * (float)ties) / (float)total : 0.0f;
+        }
+        out[((size_t)f * kPadFlop + j) * kPadFlop + h] = eq;
+    }
+}
+
 extern "C" {
 
 const char* mtt_last_error(void) { return g_err.c_str(); }
@@ -599,6 +660,17 @@ int mtt_gen_river_matrices(const mtt_alive_combo* d_combos, const uint8_t* d_dec
     hipFree(d_sc);
     hipFree(d_b);
     if (e != hipSuccess) return hip_fail(e, "mtt_gen_river_matrices", __FILE__, __LINE__);
+    return MTT_OK;
+}
+
+int mtt_gen_flop_matrices(const mtt_subset_flop* d_flops, int n_flops, const mtt_alive_combo* d_combos,
+                          float* d_flop_matrices, hipStream_t stream) {
+    MTT_ARG(n_flops >= 0 && n_flops <= 65535, "mtt_gen_flop_matrices: n_flops %d out of 0..65535", n_flops);
+    if (!n_flops) return MTT_OK;
+    MTT_ARG(d_flops && d_combos && d_flop_matrices, "mtt_gen_flop_matrices: null pointer");
+    hipLaunchKernelGGL(k_flop_matrices, dim3(kAliveFlop, n_flops), dim3(256), 0, stream, d_flops, d_combos,
+                       d_flop_matrices);
+    MTT_HIP(hipGetLastError());
     return MTT_OK;
 }
 

@@ -75,6 +75,7 @@ _SIGS = {
     "mtt_ev_dump": [_I, ctypes.c_char_p],
     "mtt_gen_turn_matrices": [_P, _P, _P, _P, _P],
     "mtt_gen_river_matrices": [_P, _P, _P, _P, _I, _P, _P],
+    "mtt_gen_flop_matrices": [_P, _I, _P, _P, _P],
 }
 _RESTYPES = {"mtt_last_error": ctypes.c_char_p, "mtt_version": ctypes.c_char_p,
              "mtt_eval7": ctypes.c_uint32}

@@ -352,3 +352,33 @@ void or_turn_matrix(const uint8_t flop[3], int turn_card, float* E /* [1216][121
     }
     free(sc);
 }
+
+/* One entry of kernel_gen_100_flop_matrices (preflop_grid_generator.cu:
+ * 130-183): the hero's deck is the 47 cards off the flop and the hero's
+ * hand; turn index 0,2,..,46, river index r1+1 step 3; runouts that hold a
+ * villain card are skipped; wins += 2, ties += 1, total += 2 per runout and
+ * eq = (wins + 0.5*ties) / total (so a tie counts a quarter).  Returns 0
+ * for blocked pairs. */
+float or_flop_matrix_entry(const uint8_t flop[3], int hc1, int hc2, int vc1, int vc2) {
+    const uint64_t fm = (1ull << flop[0]) | (1ull << flop[1]) | (1ull << flop[2]);
+    const uint64_t hm = (1ull << hc1) | (1ull << hc2), vm = (1ull << vc1) | (1ull << vc2);
+    if ((hm & vm) || (vm & fm) || (hm & fm)) return 0.0f;
+    uint8_t deck[47];
+    int d = 0;
+    for (int c = 0; c < 52; ++c)
+        if (!((hm | fm) & (1ull << c))) deck[d++] = (uint8_t)c;
+    int wins = 0, ties = 0, total = 0;
+    for (int r1 = 0; r1 < d; r1 += 2) {
+        if (vm & (1ull << deck[r1])) continue;
+        for (int r2 = r1 + 1; r2 < d; r2 += 3) {
+            if (vm & (1ull << deck[r2])) continue;
+            uint8_t h7[7] = {flop[0], flop[1], flop[2], (uint8_t)hc1, (uint8_t)hc2, deck[r1], deck[r2]};
+            uint8_t v7[7] = {flop[0], flop[1], flop[2], (uint8_t)vc1, (uint8_t)vc2, deck[r1], deck[r2]};
+            uint32_t sh = or_eval7(h7), sv = or_eval7(v7);
+            if (sh > sv) wins += 2;
+            else if (sh == sv) ties += 1;
+            total += 2;
+        }
+    }
+    return total > 0 ? ((float)wins + 0.5f * (float)ties) / (float)total : 0.0f;
+}

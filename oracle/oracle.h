@@ -15,6 +15,9 @@
  *   or_flop_counts      W+W^T+T = 990 on unblocked pairs, T symmetric, brute
  *                       force on sampled pairs, suit-permutation invariance.
  *   or_turn_matrix      equals the mean of river sign matrices (brute force).
+ *   or_flop_matrix_entry  identical-rank hands that can never differ give
+ *                       exactly 1/4 (the reference's quarter-weight tie);
+ *                       sampled-grid counts checked against or_eval7.
  *   HU CFR (or_cfr.c)   river clairvoyance toy game (closed form), brute-force
  *                       pair enumeration of showdown/fold values, zero-sum
  *                       accounting, dense vs sorted showdown.
@@ -52,6 +55,7 @@ void or_ev_dump(int first, int count, float* out);
 int or_combo_index(int a, int b);
 void or_flop_counts(const uint8_t flop[3], int32_t* W, int32_t* T);
 void or_turn_matrix(const uint8_t flop[3], int turn_card, float* E);
+float or_flop_matrix_entry(const uint8_t flop[3], int hc1, int hc2, int vc1, int vc2);
 
 /* heads-up CFR (or_cfr.c) */
 typedef struct or_hu_s or_hu;

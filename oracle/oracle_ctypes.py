@@ -272,3 +272,11 @@ class FourWay:
             self.close()
         except Exception:
             pass
+
+
+def flop_matrix_entry(flop, hero, villain):
+    L = lib()
+    L.or_flop_matrix_entry.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.or_flop_matrix_entry.restype = ctypes.c_float
+    f = np.ascontiguousarray(flop, np.uint8)
+    return float(L.or_flop_matrix_entry(f.ctypes.data, int(hero[0]), int(hero[1]), int(villain[0]), int(villain[1])))

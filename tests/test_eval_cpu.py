@@ -163,3 +163,17 @@ def test_library_exports_declared_symbols(mtt):
     L = mtt.lib()
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, missing
+
+
+def test_flop_matrix_entry_quarter_tie(oracle):
+    """preflop_grid_generator.cu:176-180 counts wins += 2, ties += 1 and
+    divides (wins + 0.5*ties) by total (+= 2 per runout): two hands with the
+    same ranks that can never make different hands tie on every runout, so
+    the entry is exactly 1/4; a blocked pair is 0."""
+    c = lambda s: "23456789TJQKA".index(s[0]) * 4 + "cdhs".index(s[1])
+    flop = sorted([c("2h"), c("7s"), c("9h")])  # no clubs or diamonds: neither hand can flush
+    assert oracle.flop_matrix_entry(flop, (c("Ac"), c("Kc")), (c("Ad"), c("Kd"))) == 0.25
+    assert oracle.flop_matrix_entry(flop, (c("Ac"), c("Kc")), (c("Ac"), c("Qd"))) == 0.0
+    # aces against a hand drawing dead to nothing but runner-runner: between 0 and 1
+    e = oracle.flop_matrix_entry(flop, (c("Ac"), c("Ad")), (c("3c"), c("4d")))
+    assert 0.5 < e < 1.0

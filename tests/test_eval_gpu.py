@@ -131,3 +131,37 @@ def test_gpu_river_matrices(mtt, oracle, torch_dev):
         want[:, ~ok] = 0
         assert np.array_equal(E[k, :1176, :1176], want)
         assert not E[k, 1176:].any() and not E[k, :, 1176:].any()
+
+
+def test_gpu_flop_matrices(mtt, oracle, torch_dev):
+    """kernel_gen_100_flop_matrices restated: GPU entries bit-equal to the
+    oracle on sampled pairs; rows of heroes that hold a flop card untouched."""
+    import torch
+    combos_np = np.zeros(1216, dtype=[("c1", "u1"), ("c2", "u1"), ("pad", "V6"), ("mask", "<u8")])
+    k = 0
+    for a in range(52):
+        for b in range(a + 1, 52):
+            if k < 1176:  # the reference's preflop list: the first 1176 of the 1326 lexicographic combos
+                combos_np[k] = (a, b, b"\0" * 6, (1 << a) | (1 << b))
+                k += 1
+    flops = np.zeros(2, dtype=[("c1", "u1"), ("c2", "u1"), ("c3", "u1"), ("pad", "u1"), ("w", "<f4")])
+    flops[0] = (3, 22, 45, 0, 1.0)
+    flops[1] = (0, 1, 2, 0, 1.0)
+    d_c = torch.from_numpy(combos_np.view(np.uint8).copy()).to(torch_dev)
+    d_f = torch.from_numpy(flops.view(np.uint8).copy()).to(torch_dev)
+    out = torch.full((2 * 1216 * 1216,), -7.0, device=torch_dev)
+    mtt.call("mtt_gen_flop_matrices", d_f.data_ptr(), 2, d_c.data_ptr(), out.data_ptr(), None)
+    torch.cuda.synchronize()
+    E = out.view(2, 1216, 1216).cpu().numpy()
+    rng = np.random.default_rng(0)
+    for f in range(2):
+        fl = [int(flops[f]["c1"]), int(flops[f]["c2"]), int(flops[f]["c3"])]
+        for _ in range(300):
+            h, j = rng.integers(0, 1176, 2)
+            hero = (int(combos_np[h]["c1"]), int(combos_np[h]["c2"]))
+            vil = (int(combos_np[j]["c1"]), int(combos_np[j]["c2"]))
+            if set(hero) & set(fl):
+                assert E[f, j, h] == -7.0  # not written
+                continue
+            assert E[f, j, h] == np.float32(oracle.flop_matrix_entry(fl, hero, vil)), (f, h, j)
+    assert (E[:, 1176:, :] == -7.0).all() and (E[:, :, 1176:] == -7.0).all()

@@ -163,3 +163,38 @@ def test_dcfr_step_gpu_matches_oracle(mtt, oracle, torch_dev):
     torch.cuda.synchronize()
     for g, c in zip(d[:3], (R, sg, S)):
         assert np.allclose(g.cpu().numpy(), c, rtol=1e-5, atol=1e-6)
+
+
+def test_config5_full_size(M, oracle, torch_dev):
+    """The 4-way benchmark configuration at full size: 100 seeded flops,
+    DCFR(1.5, 0, 2), 200 iterations, bubble factors from FP64 ICM of eight
+    stacks.  Flops 0 and 57 match the oracle after 10 iterations; after 200
+    the root strategy sums equal sum_t t^2 * range (every iteration reaches
+    the root infoset with the range itself)."""
+    game, flops, ranges, bf, st8, pay = M.config5(0, 100)
+    assert np.all(bf >= 1.0)
+    print("config5: 100 flops, DCFR 200 iterations", flush=True)
+    mm = M.Multi(game, flops, ranges, bf)
+    mm.run(10)
+    nodes, ni = mm.nodes, mm.n_info
+    for f in (0, 57):
+        h = oracle.FourWay(nodes, ni, flops[f], ranges[f], bf.astype(np.float64))
+        h.iterate(10, scheme=1, alpha=1.5, beta=0.0, gamma=2.0)
+        Rc, Sc, root_c = h.state()
+        Rg, Sg, root_g = mm.state(f)
+        assert np.abs(Rg - Rc).max() <= 1e-3 * np.abs(Rc).max(), f
+        assert np.abs(Sg - Sc).max() <= 1e-3 * np.abs(Sc).max(), f
+    mm.timing(True)
+    mm.run(190)
+    kt = mm.kernel_times()
+    ms = sum(v[0] for v in kt.values()) / 190
+    print(f"config5 iteration: {ms:.3f} ms over 100 flops ({1e3 / ms:.1f} iterations/s); "
+          + ", ".join(f"{k} {v[0] / 190:.3f} ms" for k, v in kt.items()), flush=True)
+    w = sum(float(t) ** 2 for t in range(1, 201))
+    root = nodes[0]
+    for f in (0, 99):
+        R, S, rootv = mm.state(f)
+        assert np.isfinite(R).all() and np.isfinite(rootv).all()
+        Ssum = S.reshape(ni, 3, 1216)[root["infoset_offset"]].sum(0)[:1176]
+        assert np.allclose(Ssum, w * ranges[f, root["player"], :1176], rtol=1e-4)
+    mm.close()
