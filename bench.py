@@ -43,27 +43,46 @@ def load(name, path):
 
 
 def algorithmic_work(S, game):
-    """Per-iteration algorithmic work of the river kernels of the batched path.
+    """Per-iteration algorithmic work of the river kernels of the batched path
+    (n = 1081 live combos per river board, 49x48 boards).
 
     River showdown GEMM: 2 flops per (hero, villain, column) over the live
-    combos of each river board: 2 * 1081^2 * (2 * showdowns per board) * boards
-    (the hi+lo split's second product is not counted).
-    River sweeps: regrets and strategy sums read and written once per
-    iteration (16 bytes per action slot and live combo), plus the regret read
-    of the forward pass (4 bytes), reach columns out (4 bytes per showdown
-    column entry) and GEMM output in (4 bytes)."""
+    combos of each board: 2 * n^2 * (2 * showdowns); the hi+lo split's
+    second product is not counted.
+    River backward: regrets read and written (8 B per action slot and
+    combo), showdown values read (4 B per column entry), card sums of each
+    terminal read (2 players x 53 x 8 B).
+    River forward: regrets read (4 B), strategy sums read-modify-write (8 B),
+    reach columns out as BF16 hi + lo (4 B per column entry), card sums out."""
     nodes, ni, nm = S.tree(game)
-    river_dec = int(((nodes["type"] == 0) & (nodes["river_idx"] >= 64)).sum())
-    river_sd = int(((nodes["type"] == 2) & (nodes["turn_idx"] >= 49)).sum())
+    river = nodes["river_idx"] >= 64
+    river_dec = int(((nodes["type"] == 0) & river).sum())
+    river_sd = int(((nodes["type"] == 2) & river).sum())
+    river_terms = int((((nodes["type"] == 1) | (nodes["type"] == 2)) & river).sum())
     act = nodes["num_actions"].astype(np.int64)
-    river_actions = int(act[(nodes["type"] == 0) & (nodes["river_idx"] >= 64)].sum())
+    river_actions = int(act[(nodes["type"] == 0) & river].sum())
     boards = 49 * 48
     n = 1081
+    sums = river_terms * 2 * 53 * 8
     gemm_flops = 2.0 * n * n * (2 * river_sd)
-    sweep_bytes = river_actions * n * (16 + 4 + 4) + 2 * river_sd * n * (2 * 2 + 4)
-    return dict(boards=boards, river_decisions=river_dec, river_showdowns=river_sd, river_actions=river_actions,
-                gemm_flops=gemm_flops, backward_bytes=river_actions * n * (16 + 4) + 2 * river_sd * n * 4,
-                sweep_bytes=sweep_bytes)
+    backward_bytes = river_actions * n * 8 + 2 * river_sd * n * 4 + sums
+    forward_bytes = river_actions * n * (4 + 8) + 2 * river_sd * n * 4 + sums
+    return dict(boards=boards, river_decisions=river_dec, river_showdowns=river_sd, river_terminals=river_terms,
+                river_actions=river_actions, gemm_flops=gemm_flops, backward_bytes=backward_bytes,
+                forward_bytes=forward_bytes)
+
+
+def hbm_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (profiles/hbm_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        k = t["kernels"][kernel]
+        return k["fetch_bytes"] + k["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(S, O, game, r0, r1, seconds=20.0):
@@ -99,7 +118,21 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=150, help="timed plain FP32 iterations")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (3x): the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 30
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read": n, "bytes_written": n, "launches": 3}}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
                           "n_gpus": args.gpus}))
@@ -154,13 +187,18 @@ def main():
     if dominant == "river_showdown_gemm":
         ach = work["gemm_flops"] / (gemm_ms * 1e-3) / 1e12
         roof = {"kernel": dominant, "bound": "mfma", "achieved": ach, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / BF16_PEAK_TFLOPS, "traffic": None, "avg_ms": gemm_ms}
+                "frac": ach / BF16_PEAK_TFLOPS, "traffic": hbm_traffic(dominant), "avg_ms": gemm_ms}
     else:
         ms = per_iter.get(dominant, float("nan"))
-        byt = work["backward_bytes"] if dominant == "river_backward" else work["sweep_bytes"]
+        byt = work["backward_bytes"] if dominant == "river_backward" else work["forward_bytes"]
         ach = byt / (ms * 1e-3) / 1e9
         roof = {"kernel": dominant, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_ms": ms}
+                "frac": ach / HBM_PEAK_GBS, "traffic": hbm_traffic(dominant), "avg_ms": ms,
+                "algorithmic_bytes": byt}
+    gemm_roof = {"kernel": "river_showdown_gemm", "bound": "mfma",
+                 "achieved": work["gemm_flops"] / (gemm_ms * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS,
+                 "unit": "TFLOP/s", "avg_ms": gemm_ms}
+    gemm_roof["frac"] = gemm_roof["achieved"] / BF16_PEAK_TFLOPS
 
     cpu = None
     if not args.no_cpu:
@@ -193,6 +231,7 @@ def main():
         "plain_fp32": plain,
         "speedup_vs_plain_fp32": (ips_b / plain["iter_per_s"]) if plain else None,
         "roofline": roof,
+        "roofline_gemm": gemm_roof,
         "work_per_iter": work,
         "cpu_baseline": cpu,
     }
