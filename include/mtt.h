@@ -280,6 +280,34 @@ int mtt_solver_kernel_times(mtt_solver* h, int idx, char* name, int name_cap, do
 int mtt_solver_info(mtt_solver* h, int64_t* info);
 int mtt_solver_destroy(mtt_solver* h);
 
+/* ----------------------------------------------- batched flop subgames -- */
+/* Many independent heads-up flop subgames solved together (new: the
+ * reference has no batch driver; SURVEY's mtt_batch_solve).  Every instance
+ * is a game with n_board = 3, root_chance = 0, last_street = 0 (flop
+ * betting, showdown over the 990 turn/river runouts) and its own board,
+ * pot, stack and menu[0].  ranges: [n][2][1326] (global combo order).
+ * Iterations run the per-instance street sweeps over all instances at once
+ * and the showdowns as a batched GEMM of exact W - L counts (2 x BF16)
+ * against BF16 hi+lo reach on MFMA; after the first call, iterations are
+ * replayed from a captured hipGraph.  cfg->mode is ignored. */
+typedef struct mtt_batch mtt_batch;
+int mtt_batch_create(const mtt_hu_game* games, int n, const float* ranges, const mtt_solver_cfg* cfg,
+                     mtt_batch** out);
+int mtt_batch_run(mtt_batch* h, int iters);
+int mtt_batch_iterations(const mtt_batch* h);
+int mtt_batch_tree_info(mtt_batch* h, int instance, int* n_nodes, int* n_infosets);
+/* Raw regrets / strategy sums of one instance in that instance's tree
+ * layout [infoset][MTT_MAX_ACTIONS][1216] (flop-alive combo order). */
+int mtt_batch_get_state(mtt_batch* h, int instance, float* R_out, float* S_out, size_t n_floats);
+/* 0: launch kernels one by one (also while timing is on), 1: graph replay. */
+int mtt_batch_set_graph(mtt_batch* h, int on);
+/* Autotuner's choice for the showdown GEMM (hero tiles per workgroup) and
+ * the two candidates' measured times (0 when read from the tuning file). */
+int mtt_batch_tuning(mtt_batch* h, int* hero_tiles, double* gemm_ms);
+int mtt_batch_timing(mtt_batch* h, int enable);
+int mtt_batch_kernel_times(mtt_batch* h, int idx, char* name, int name_cap, double* total_ms, int64_t* launches);
+int mtt_batch_destroy(mtt_batch* h);
+
 /* -------------------------------------------------- fused DCFR step -- */
 /* kernel_dcfr_step, preflop_grid_generator.cu:387-410, over n entries with
  * action stride `stride` (the reference: n = 169 classes, stride 192):

@@ -33,6 +33,7 @@ namespace {
 
 constexpr int NPR = 1088;   // river combos padded (1081 live)
 constexpr int NPT = 1152;   // turn combos padded (1128 live)
+constexpr int NPF = 1216;   // flop combos padded (1176 live), the reference's padding
 
 enum : uint8_t { DN_DEC = 0, DN_FOLD = 1, DN_SD = 2, DN_LEAF = 3 };
 enum : int { EV_ENTER = 0, EV_TERM = 1, EV_LEAVE = 2 };
@@ -48,7 +49,7 @@ static_assert(sizeof(DNode) == 28, "DNode layout");
 
 constexpr int MAXN = 128;      // nodes per template
 constexpr int MAXE = 256;      // DFS events per template
-constexpr int MAXSLOT = 64;    // action slots per template
+constexpr int MAXSLOT = 96;    // action slots per template (three 32-register vectors)
 constexpr double FX = 281474976710656.0;  // 2^48: fixed-point scale of card sums
 
 // One DFS event of a template with its node record, 32 bytes, read by the
@@ -63,6 +64,7 @@ struct LevelDev {
     int np, nslot, nterm, ncol, ncolp, ntmpl, nboards;
     int maxterm;               // largest terminal count of one template
     int maxcolt;               // largest showdown column count of one template
+    int maxslot;               // largest action-slot count of one template
     const EvRec* ev;           // DFS events of every template, concatenated
     const int* tev0;           // [ntmpl+1]
     const EvRec* fev;          // the same without LEAVE events (preorder nodes)
@@ -75,10 +77,12 @@ struct LevelDev {
     unsigned long long* part;  // [board][nterm][half][2][52] card sums of reach, 2^48 fixed point
     const uint8_t* cc;         // [board][np][2] cards of combo
     const int* nlive;          // [board]
+    const int* btmpl;          // flop instances: board -> its private template (else null)
+    const float* bpot0;        // flop instances: starting pot of each board
 };
 
 struct BatchedDev {
-    LevelDev L[2];             // L[0] turn (two-level games), L[1] river
+    LevelDev L[3];             // L[0] turn (two-level games), L[1] river, L[2] flop instances
     int two_level;
     int nleaf;                 // turn template leaves (= river templates)
     const int* leaf_tmpl;      // turn leaf -> river template
@@ -134,17 +138,25 @@ __device__ __forceinline__ void ev_stage(EvRec* lds, const EvRec* __restrict__ g
     for (int e = threadIdx.x; e < 2 * n; e += blockDim.x) dst[e] = src[e];
 }
 
+// Element q (wave-uniform, < 32 * NRV) of a register file held as NRV
+// 32-register vectors a, b (, c): the reads share one indexed-register
+// window.
+template <int NRV>
+__device__ __forceinline__ float rselv(const f32x32_t& a, const f32x32_t& b, const f32x32_t& c, int q) {
+    const float va = a[q & 31], vb = b[q & 31];
+    if constexpr (NRV == 3) {
+        const float vc = c[q & 31];
+        return q < 32 ? va : (q < 64 ? vb : vc);
+    } else {
+        return q < 32 ? va : vb;
+    }
+}
+
 // 1/n for an action count n in 1..4 (wave-uniform; exact constants).
 __device__ __forceinline__ float uniform_share(int n) {
     return n == 2 ? 0.5f : (n == 3 ? 1.0f / 3.0f : (n == 4 ? 0.25f : 1.0f));
 }
 
-// Element q (wave-uniform, < 64) of a 64-entry register file held as two
-// 32-register vectors: both reads share one indexed-register window.
-__device__ __forceinline__ float rsel(const f32x32_t& a, const f32x32_t& b, int q) {
-    const float va = a[q & 31], vb = b[q & 31];
-    return q < 32 ? va : vb;
-}
 
 // Non-blocked reach mass of the other player for a combo holding cards
 // a, b: total - sum[a] - sum[b] + own (the pair sharing both cards is the
@@ -178,33 +190,39 @@ constexpr int SWEEP_NT = 576;   // threads per sweep workgroup; two workgroups c
 // At each terminal the reach of both players is added into per-card bins in
 // LDS with 64-bit fixed-point atomics (exact, order independent); each
 // workgroup writes its half's bins, and the backward adds the two halves.
-template <int LEVEL, int DM>
-__global__ __launch_bounds__(SWEEP_NT) void k_bt_forward(BatchedDev D, const float* __restrict__ ranges, float ws) {
+template <int LEVEL, int DM, int NT, int NRV = 2>
+__global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __restrict__ ranges, float ws,
+                                                   const float* __restrict__ wdev) {
     typedef float vS __attribute__((ext_vector_type(DM * 4)));
     typedef float vD __attribute__((ext_vector_type(DM)));
     const LevelDev& L = D.L[LEVEL];
     extern __shared__ unsigned long long bins[];  // [nt][2][52]
     __shared__ EvRec sev[MAXE];
     const int inst = blockIdx.x >> 1, half = blockIdx.x & 1;
-    const int b = inst / L.ntmpl, tm = inst % L.ntmpl;
+    const int b = LEVEL == 2 ? inst : inst / L.ntmpl;
+    const int tm = LEVEL == 2 ? ldu(L.btmpl + b) : inst % L.ntmpl;
+    if (wdev) ws = wdev[2];
     const int tid = threadIdx.x;
     const int np = L.np;
-    const int k = half * SWEEP_NT + tid;
+    const int k = half * NT + tid;
     const bool inb = k < np;
     const bool live = k < ldu(L.nlive + b);
     const int e0 = ldu(L.tfev0 + tm), e1 = ldu(L.tfev0 + tm + 1);
     const int s0 = ldu(L.tslot0 + tm), nsl = ldu(L.tslot0 + tm + 1) - s0;
     const int t0 = ldu(L.tterm0 + tm), nt = ldu(L.tterm0 + tm + 1) - t0;
-    for (int e = tid; e < nt * 104; e += SWEEP_NT) bins[e] = 0ull;
+    // flop instances keep per-board state for their own template only
+    const int sb = LEVEL == 2 ? 0 : s0, tb = LEVEL == 2 ? 0 : t0, cb0 = LEVEL == 2 ? ldu(L.tcol0 + tm) : 0;
+    for (int e = tid; e < nt * 104; e += NT) bins[e] = 0ull;
     ev_stage(sev, L.fev + e0, e1 - e0);
     int c1 = 0, c2 = 0;
     float root0 = 0.0f, root1 = 0.0f;
-    f32x32_t Ra, Rb;
-    const size_t rbase = ((size_t)b * L.nslot + s0) * np + k;
+    f32x32_t Ra, Rb, Rc;
+    const size_t rbase = ((size_t)b * L.nslot + sb) * np + k;
 #pragma unroll
     for (int x = 0; x < 32; ++x) {
         Ra[x] = (live && x < nsl) ? L.R[rbase + (size_t)x * np] : 0.0f;
         Rb[x] = (live && x + 32 < nsl) ? L.R[rbase + (size_t)(x + 32) * np] : 0.0f;
+        if constexpr (NRV == 3) Rc[x] = (live && x + 64 < nsl) ? L.R[rbase + (size_t)(x + 64) * np] : 0.0f;
     }
     if (live) {
         c1 = L.cc[((size_t)b * np + k) * 2];
@@ -239,12 +257,12 @@ __global__ __launch_bounds__(SWEEP_NT) void k_bt_forward(BatchedDev D, const flo
             float pos[4], sum = 0.0f;
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
-                pos[a] = a < nd.nact ? fmaxf(rsel(Ra, Rb, ls + a), 0.0f) : 0.0f;
+                pos[a] = a < nd.nact ? fmaxf(rselv<NRV>(Ra, Rb, Rc, ls + a), 0.0f) : 0.0f;
                 sum += pos[a];
             }
             const float inv = sum > 1e-7f ? __builtin_amdgcn_rcpf(sum) : 0.0f, un = uniform_share(nd.nact);
             const float wr = ws * (nd.player == 0 ? x0 : x1);
-            float* Sg = L.S + ((size_t)b * L.nslot + nd.slot) * np + k;
+            float* Sg = L.S + ((size_t)b * L.nslot + (nd.slot - s0 + sb)) * np + k;
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
                 const float sa = a < nd.nact ? (sum > 1e-7f ? pos[a] * inv : un) : 0.0f;
@@ -264,13 +282,14 @@ __global__ __launch_bounds__(SWEEP_NT) void k_bt_forward(BatchedDev D, const flo
         if (nd.type == DN_SD && inb) {
             // player p's column holds the opponent's reach
             const float o0 = live ? x1 : 0.0f, o1 = live ? x0 : 0.0f;
-            if (LEVEL == 1) {
-                const size_t cb = ((size_t)b * L.ncolp + nd.col) * NPR + k;
+            if (LEVEL >= 1) {
+                constexpr int NX = LEVEL == 2 ? NPF : NPR;
+                const size_t cb = ((size_t)b * L.ncolp + (nd.col - cb0)) * NX + k;
                 const __hip_bfloat16 h0 = __float2bfloat16(o0), h1 = __float2bfloat16(o1);
                 D.Bh[cb] = h0;
                 D.Bl[cb] = __float2bfloat16(o0 - __bfloat162float(h0));
-                D.Bh[cb + NPR] = h1;
-                D.Bl[cb + NPR] = __float2bfloat16(o1 - __bfloat162float(h1));
+                D.Bh[cb + NX] = h1;
+                D.Bl[cb + NX] = __float2bfloat16(o1 - __bfloat162float(h1));
             } else {
                 const size_t cb = ((size_t)b * L.ncolp + nd.col) * NPT + k;
                 D.B0[cb] = o0;
@@ -292,8 +311,8 @@ __global__ __launch_bounds__(SWEEP_NT) void k_bt_forward(BatchedDev D, const flo
         }
     }
     __syncthreads();
-    unsigned long long* out = L.part + ((size_t)b * L.nterm + t0) * 208 + half * 104;
-    for (int e = tid; e < nt * 104; e += SWEEP_NT) out[(size_t)(e / 104) * 208 + e % 104] = bins[e];
+    unsigned long long* out = L.part + ((size_t)b * L.nterm + tb) * 208 + half * 104;
+    for (int e = tid; e < nt * 104; e += NT) out[(size_t)(e / 104) * 208 + e % 104] = bins[e];
 }
 
 // ----------------------------------------------------------- backward ----
@@ -307,16 +326,22 @@ __global__ __launch_bounds__(SWEEP_NT) void k_bt_forward(BatchedDev D, const flo
 // showdown values and the card sums, and applies the DCFR-generalised
 // regret update R <- (R + r) * (R + r > 0 ? wp : wn) at each decision node
 // on the way up.
-template <int LEVEL, int DM, int NTB>
-__global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* __restrict__ ranges, float wp, float wn) {
-    constexpr int NH = (NPT + NTB - 1) / NTB;  // workgroups per (board, template)
+template <int LEVEL, int DM, int NTB, int NRV = 2>
+__global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* __restrict__ ranges, float wp, float wn,
+                                                     const float* __restrict__ wdev) {
+    constexpr int NH = ((LEVEL == 2 ? NPF : NPT) + NTB - 1) / NTB;  // workgroups per (board, template)
     typedef float vS __attribute__((ext_vector_type(DM * 4)));
     typedef float vD __attribute__((ext_vector_type(DM)));
     const LevelDev& L = D.L[LEVEL];
     extern __shared__ unsigned long long ssum[];  // [nt][2][53] (slot 52 = total), then float [ncolt][NTB]
     __shared__ EvRec sev[MAXE];
     const int inst = blockIdx.x / NH, half = blockIdx.x % NH;
-    const int b = inst / L.ntmpl, tm = inst % L.ntmpl;
+    const int b = LEVEL == 2 ? inst : inst / L.ntmpl;
+    const int tm = LEVEL == 2 ? ldu(L.btmpl + b) : inst % L.ntmpl;
+    if (wdev) {
+        wp = wdev[0];
+        wn = wdev[1];
+    }
     const int tid = threadIdx.x;
     const int np = L.np;
     const int k = half * NTB + tid;
@@ -325,27 +350,29 @@ __global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* 
     const int s0 = ldu(L.tslot0 + tm), nsl = ldu(L.tslot0 + tm + 1) - s0;
     const int t0 = ldu(L.tterm0 + tm), nt = ldu(L.tterm0 + tm + 1) - t0;
     const int col0 = ldu(L.tcol0 + tm), ncolt = ldu(L.tcol0 + tm + 1) - col0;
+    // flop instances keep per-board state for their own template only
+    const int sb = LEVEL == 2 ? 0 : s0, tb = LEVEL == 2 ? 0 : t0, cbase = LEVEL == 2 ? 0 : col0;
     float* Cs = reinterpret_cast<float*>(ssum + (size_t)L.maxterm * 106) + tid;
     {
-        const unsigned long long* P = L.part + ((size_t)b * L.nterm + t0) * 208;
+        const unsigned long long* P = L.part + ((size_t)b * L.nterm + tb) * 208;
         for (int e = tid; e < nt * 104; e += NTB) {
             const int tl = e / 104, r = e % 104;
             ssum[tl * 106 + (r / 52) * 53 + r % 52] = P[(size_t)tl * 208 + r] + P[(size_t)tl * 208 + 104 + r];
         }
-        const float* Cb = LEVEL == 1 ? D.C + ((size_t)b * L.ncolp + col0) * NPR + k
-                                     : D.C0 + ((size_t)b * L.ncolp + col0) * NPT + k;
-        const int cstride = LEVEL == 1 ? NPR : NPT;
+        constexpr int cstride = LEVEL == 2 ? NPF : (LEVEL == 1 ? NPR : NPT);
+        const float* Cb = (LEVEL >= 1 ? D.C : D.C0) + ((size_t)b * L.ncolp + cbase) * cstride + k;
         for (int x = 0; x < ncolt; ++x) Cs[x * NTB] = live ? Cb[(size_t)x * cstride] : 0.0f;
     }
     ev_stage(sev, L.ev + e0, e1 - e0);
     int c1 = 0, c2 = 0;
     float root0 = 0.0f, root1 = 0.0f;
-    f32x32_t Ra, Rb;
-    const size_t rbase = ((size_t)b * L.nslot + s0) * np + k;
+    f32x32_t Ra, Rb, Rc;
+    const size_t rbase = ((size_t)b * L.nslot + sb) * np + k;
 #pragma unroll
     for (int x = 0; x < 32; ++x) {
         Ra[x] = (live && x < nsl) ? L.R[rbase + (size_t)x * np] : 0.0f;
         Rb[x] = (live && x + 32 < nsl) ? L.R[rbase + (size_t)(x + 32) * np] : 0.0f;
+        if constexpr (NRV == 3) Rc[x] = (live && x + 64 < nsl) ? L.R[rbase + (size_t)(x + 64) * np] : 0.0f;
     }
     if (live) {
         c1 = L.cc[((size_t)b * np + k) * 2];
@@ -390,7 +417,7 @@ __global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* 
             float pos[4], sum = 0.0f;
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
-                pos[a] = a < nd.nact ? fmaxf(rsel(Ra, Rb, ls + a), 0.0f) : 0.0f;
+                pos[a] = a < nd.nact ? fmaxf(rselv<NRV>(Ra, Rb, Rc, ls + a), 0.0f) : 0.0f;
                 sum += pos[a];
             }
             const float inv = sum > 1e-7f ? __builtin_amdgcn_rcpf(sum) : 0.0f, un = uniform_share(nd.nact);
@@ -422,6 +449,10 @@ __global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* 
                 if (LEVEL == 1) {
                     v0 = 0.5f * nd.pot * q0 + 0.5f * D.pot0 * m0;
                     v1 = 0.5f * nd.pot * q1 + 0.5f * D.pot0 * m1;
+                } else if (LEVEL == 2) {  // C = sum (W - L) r over the 990 runouts: E = 1/2 + (W - L)/1980
+                    const float hp = 0.5f * L.bpot0[b];
+                    v0 = nd.pot * (1.0f / 1980.0f) * q0 + hp * m0;
+                    v1 = nd.pot * (1.0f / 1980.0f) * q1 + hp * m1;
                 } else {
                     v0 = nd.pot * q0 - nd.c0 * m0;
                     v1 = nd.pot * q1 - nd.c1 * m1;
@@ -446,7 +477,7 @@ __global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* 
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
                     if (a < nd.nact) {
-                        const float qv = rsel(Ra, Rb, ls + a) + (cv[d * 4 + a] - node_v);
+                        const float qv = rselv<NRV>(Ra, Rb, Rc, ls + a) + (cv[d * 4 + a] - node_v);
                         Rg[(size_t)(ls + a) * np] = qv * (qv > 0.0f ? wp : wn);
                     }
                 }
@@ -464,7 +495,12 @@ __global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* 
         }
     }
     if (k >= np) return;
-    if (LEVEL == 1) {
+    if (LEVEL == 2) {
+        if (D.rootval0) {
+            D.rootval0[((size_t)b * 2 + 0) * NPF + k] = live ? out0 : 0.0f;
+            D.rootval0[((size_t)b * 2 + 1) * NPF + k] = live ? out1 : 0.0f;
+        }
+    } else if (LEVEL == 1) {
         if (D.two_level) {
             float* rv = D.rootval + (((size_t)b * L.ntmpl + tm) * 2) * NPR + k;
             rv[0] = live ? out0 : 0.0f;
@@ -734,7 +770,9 @@ __global__ void k_bt_export(BatchedDev D, const ExportRec* __restrict__ recs, in
 constexpr int BWD_NT = 576;
 constexpr int BWD_NH = (NPT + BWD_NT - 1) / BWD_NT;
 // LDS of the backward sweep: card sums, then the showdown values.
-static size_t bwd_lds(const LevelDev& L) { return (size_t)L.maxterm * 106 * 8 + (size_t)L.maxcolt * BWD_NT * 4; }
+static size_t bwd_lds(const LevelDev& L, int nt = BWD_NT) {
+    return (size_t)L.maxterm * 106 * 8 + (size_t)L.maxcolt * nt * 4;
+}
 
 struct BatchedState {
     BatchedDev D;
@@ -822,6 +860,83 @@ int add_template(const Template& T, LevelHost& L, int leaf_base) {
     L.ncol += 2 * T.n_showdown;
     L.tnode0.push_back((int)L.nodes.size());
     L.tev0.push_back((int)L.events.size());
+    return MTT_OK;
+}
+
+// Uploads one level's templates, tables and state.  priv: every board runs
+// its own private template (flop instances), so per-board state is sized by
+// the largest template instead of the sum over templates.
+int upload_level(DeviceArena& A, hipStream_t st, LevelHost& H, LevelDev& L, int np, int nboards,
+                 const std::vector<uint8_t>& cc, const std::vector<int>& nlive, bool priv, int bwd_nt) {
+    L.np = np;
+    L.nboards = nboards;
+    L.ntmpl = (int)H.tnode0.size() - 1;
+    int r = MTT_OK;
+    std::vector<int> slot0 = H.slot0, term0 = H.term0, col0 = H.col0;
+    slot0.push_back(H.nslot);
+    term0.push_back(H.nterm);
+    col0.push_back(H.ncol);
+    if (priv) {  // each board holds the state of its own template only
+        L.nslot = L.nterm = L.ncol = 0;
+        for (int t = 0; t < L.ntmpl; ++t) {
+            L.nslot = std::max(L.nslot, slot0[t + 1] - slot0[t]);
+            L.nterm = std::max(L.nterm, term0[t + 1] - term0[t]);
+            L.ncol = std::max(L.ncol, col0[t + 1] - col0[t]);
+        }
+    } else {
+        L.nslot = H.nslot;
+        L.nterm = H.nterm;
+        L.ncol = H.ncol;
+    }
+    L.ncolp = ((L.ncol + 31) / 32) * 32;
+    if (L.ncolp == 0) L.ncolp = 32;
+    L.maxterm = 0;
+    L.maxcolt = 0;
+    for (int t = 0; t < L.ntmpl; ++t) {
+        L.maxterm = std::max(L.maxterm, term0[t + 1] - term0[t]);
+        L.maxcolt = std::max(L.maxcolt, col0[t + 1] - col0[t]);
+        MTT_ARG(H.tnode0[t + 1] - H.tnode0[t] <= MAXN, "template has more than %d nodes", MAXN);
+        MTT_ARG(H.tev0[t + 1] - H.tev0[t] <= MAXE, "template has more than %d events", MAXE);
+        MTT_ARG(slot0[t + 1] - slot0[t] <= MAXSLOT, "template has more than %d action slots", MAXSLOT);
+    }
+    // event records: node + kind, in DFS order per template
+    std::vector<EvRec> recs(H.events.size());
+    for (int t = 0; t < L.ntmpl; ++t)
+        for (int e = H.tev0[t]; e < H.tev0[t + 1]; ++e) {
+            memset(&recs[e], 0, sizeof(EvRec));
+            recs[e].nd = H.nodes[H.tnode0[t] + (H.events[e] >> 2)];
+            recs[e].kind = H.events[e] & 3;
+        }
+    std::vector<EvRec> frecs;
+    std::vector<int> tf0{0};
+    for (int t = 0; t < L.ntmpl; ++t) {
+        for (int e = H.tev0[t]; e < H.tev0[t + 1]; ++e)
+            if (recs[e].kind != EV_LEAVE) frecs.push_back(recs[e]);
+        tf0.push_back((int)frecs.size());
+    }
+    EvRec *dev, *dfev;
+    int *e0, *f0, *nl, *ts, *tt, *tc;
+    uint8_t* c;
+    r = r ? r : A.upload(recs, &dev);
+    r = r ? r : A.upload(frecs, &dfev);
+    r = r ? r : A.upload(tf0, &f0);
+    r = r ? r : A.upload(H.tev0, &e0);
+    r = r ? r : A.upload(slot0, &ts);
+    r = r ? r : A.upload(term0, &tt);
+    r = r ? r : A.upload(col0, &tc);
+    r = r ? r : A.upload(cc, &c);
+    r = r ? r : A.upload(nlive, &nl);
+    if (r) return r;
+    MTT_ARG(bwd_lds(L, bwd_nt) <= 152 * 1024, "template needs %zu bytes of LDS in the backward sweep",
+            bwd_lds(L, bwd_nt));
+    L.ev = dev; L.tev0 = e0; L.fev = dfev; L.tfev0 = f0; L.tslot0 = ts; L.tterm0 = tt; L.tcol0 = tc; L.cc = c; L.nlive = nl;
+    size_t ns = (size_t)nboards * L.nslot * np;
+    r = r ? r : A.alloc(ns, &L.R);
+    r = r ? r : A.alloc(ns, &L.S);
+    r = r ? r : A.alloc((size_t)nboards * std::max(L.nterm, 1) * 208, &L.part);
+    if (r) return r;
+    MTT_HIP(hipMemsetAsync(L.R, 0, ns * sizeof(float), st));
+    MTT_HIP(hipMemsetAsync(L.S, 0, ns * sizeof(float), st));
     return MTT_OK;
 }
 
@@ -936,66 +1051,7 @@ int batched_create(SolverImpl& s) {
     // --------------------------------------------------- device arrays --
     auto up_level = [&](LevelHost& H, LevelDev& L, int np, int nboards, const std::vector<uint8_t>& cc,
                         const std::vector<int>& nlive) -> int {
-        L.np = np;
-        L.nboards = nboards;
-        L.nslot = H.nslot;
-        L.nterm = H.nterm;
-        L.ncol = H.ncol;
-        L.ncolp = ((H.ncol + 31) / 32) * 32;
-        if (L.ncolp == 0) L.ncolp = 32;
-        L.ntmpl = (int)H.tnode0.size() - 1;
-        int r = MTT_OK;
-        std::vector<int> slot0 = H.slot0, term0 = H.term0, col0 = H.col0;
-        slot0.push_back(H.nslot);
-        term0.push_back(H.nterm);
-        col0.push_back(H.ncol);
-        L.maxterm = 0;
-        L.maxcolt = 0;
-        for (int t = 0; t < L.ntmpl; ++t) {
-            L.maxterm = std::max(L.maxterm, term0[t + 1] - term0[t]);
-            L.maxcolt = std::max(L.maxcolt, col0[t + 1] - col0[t]);
-            MTT_ARG(H.tnode0[t + 1] - H.tnode0[t] <= MAXN, "template has more than %d nodes", MAXN);
-            MTT_ARG(H.tev0[t + 1] - H.tev0[t] <= MAXE, "template has more than %d events", MAXE);
-            MTT_ARG(slot0[t + 1] - slot0[t] <= MAXSLOT, "template has more than %d action slots", MAXSLOT);
-        }
-        // event records: node + kind, in DFS order per template
-        std::vector<EvRec> recs(H.events.size());
-        for (int t = 0; t < L.ntmpl; ++t)
-            for (int e = H.tev0[t]; e < H.tev0[t + 1]; ++e) {
-                memset(&recs[e], 0, sizeof(EvRec));
-                recs[e].nd = H.nodes[H.tnode0[t] + (H.events[e] >> 2)];
-                recs[e].kind = H.events[e] & 3;
-            }
-        std::vector<EvRec> frecs;
-        std::vector<int> tf0{0};
-        for (int t = 0; t < L.ntmpl; ++t) {
-            for (int e = H.tev0[t]; e < H.tev0[t + 1]; ++e)
-                if (recs[e].kind != EV_LEAVE) frecs.push_back(recs[e]);
-            tf0.push_back((int)frecs.size());
-        }
-        EvRec *dev, *dfev;
-        int *e0, *f0, *nl, *ts, *tt, *tc;
-        uint8_t* c;
-        r = r ? r : A.upload(recs, &dev);
-        r = r ? r : A.upload(frecs, &dfev);
-        r = r ? r : A.upload(tf0, &f0);
-        r = r ? r : A.upload(H.tev0, &e0);
-        r = r ? r : A.upload(slot0, &ts);
-        r = r ? r : A.upload(term0, &tt);
-        r = r ? r : A.upload(col0, &tc);
-        r = r ? r : A.upload(cc, &c);
-        r = r ? r : A.upload(nlive, &nl);
-        if (r) return r;
-        MTT_ARG(bwd_lds(L) <= 160 * 1024, "template needs %zu bytes of LDS in the backward sweep", bwd_lds(L));
-        L.ev = dev; L.tev0 = e0; L.fev = dfev; L.tfev0 = f0; L.tslot0 = ts; L.tterm0 = tt; L.tcol0 = tc; L.cc = c; L.nlive = nl;
-        size_t ns = (size_t)nboards * L.nslot * np;
-        r = r ? r : A.alloc(ns, &L.R);
-        r = r ? r : A.alloc(ns, &L.S);
-        r = r ? r : A.alloc((size_t)nboards * std::max(L.nterm, 1) * 208, &L.part);
-        if (r) return r;
-        MTT_HIP(hipMemsetAsync(L.R, 0, ns * sizeof(float), s.stream));
-        MTT_HIP(hipMemsetAsync(L.S, 0, ns * sizeof(float), s.stream));
-        return MTT_OK;
+        return upload_level(A, s.stream, H, L, np, nboards, cc, nlive, false, BWD_NT);
     };
     auto pick = [](int d) { return d + 1 <= 4 ? 4 : 8; };
     B.dm[0] = pick(H0.maxdepth);
@@ -1004,6 +1060,7 @@ int batched_create(SolverImpl& s) {
     if (rc) return rc;
     LevelDev& L1 = B.D.L[1];
     MTT_ARG(L1.ncolp <= GCOLS, "river showdown columns %d exceed %d", L1.ncol, GCOLS);
+    MTT_ARG(L1.maxslot <= 64, "a river template has %d action slots (the batched path holds 64)", L1.maxslot);
     {
         uint16_t *d_rank, *d_rmap, *d_imap;
         uint32_t *d_cx, *d_cy;
@@ -1038,6 +1095,7 @@ int batched_create(SolverImpl& s) {
         if (rc) return rc;
         LevelDev& L0 = B.D.L[0];
         MTT_ARG(L0.ncol <= 16, "turn showdown columns %d exceed 16", L0.ncol);
+        MTT_ARG(L0.maxslot <= 64, "the turn template has %d action slots (the batched path holds 64)", L0.maxslot);
         B.D.nleaf = (int)leaf_tmpl.size();
         int *d_lt, *d_tl;
         rc = rc ? rc : A.upload(leaf_tmpl, &d_lt);
@@ -1136,15 +1194,15 @@ int batched_run(SolverImpl& s, int iters) {
             const size_t sh = (size_t)L0.maxterm * 104 * 8;
             const dim3 g(49 * L0.ntmpl * 2);
             MTT_TIMED(s, "turn_forward",
-                      if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_forward<0, 4>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws);
-                      else hipLaunchKernelGGL((k_bt_forward<0, 8>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws));
+                      if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_forward<0, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
+                      else hipLaunchKernelGGL((k_bt_forward<0, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
         {
             const size_t sh = (size_t)L1.maxterm * 104 * 8;
             const dim3 g(B.nb1 * L1.ntmpl * 2);
             MTT_TIMED(s, "river_forward",
-                      if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_forward<1, 4>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws);
-                      else hipLaunchKernelGGL((k_bt_forward<1, 8>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws));
+                      if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_forward<1, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
+                      else hipLaunchKernelGGL((k_bt_forward<1, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
         ::mtt::KernelTimer::Mark gm;
         if (s.timer.enabled) s.timer.begin("river_showdown_gemm", s.stream, gm);
@@ -1168,8 +1226,8 @@ int batched_run(SolverImpl& s, int iters) {
             const size_t sh = bwd_lds(L1);
             const dim3 g(B.nb1 * L1.ntmpl * BWD_NH);
             MTT_TIMED(s, "river_backward",
-                      if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_backward<1, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn);
-                      else hipLaunchKernelGGL((k_bt_backward<1, 8, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn));
+                      if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_backward<1, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
+                      else hipLaunchKernelGGL((k_bt_backward<1, 8, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr));
         }
         if (B.two_level) {
             const LevelDev& L0 = B.D.L[0];
@@ -1181,8 +1239,8 @@ int batched_run(SolverImpl& s, int iters) {
             const size_t sh = bwd_lds(L0);
             const dim3 g(49 * L0.ntmpl * BWD_NH);
             MTT_TIMED(s, "turn_backward",
-                      if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_backward<0, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn);
-                      else hipLaunchKernelGGL((k_bt_backward<0, 8, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn));
+                      if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_backward<0, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
+                      else hipLaunchKernelGGL((k_bt_backward<0, 8, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr));
         }
         MTT_HIP(hipGetLastError());
         s.iters_done = t;
@@ -1204,4 +1262,617 @@ int batched_export(SolverImpl& s, float* d_out, int which) {
 
 int batched_strategy_sums(SolverImpl& s, float* d_S_ref) { return batched_export(s, d_S_ref, 0); }
 
+
+// ===================================================== flop instances ===
+// Many independent heads-up flop subgames (each its own board, stacks, pot
+// and bet menu) solved together: one private template per instance in the
+// street sweeps, the showdown over the 990 turn/river runouts as a batched
+// GEMM of exact W - L counts against the reach columns, and iterations
+// replayed from a captured hipGraph.
+namespace {
+
+constexpr int FNCH = NPF / 32;  // K chunks of 32 villains
+
+// W - L of every (hero, villain) pair from exhaustive counts (alive-1216
+// layout), split into two BF16-exact parts: hi = 16 * round(S / 16)
+// (|hi| <= 992, six significant bits) and lo = S - hi (|lo| <= 8).
+// Layout [board][chunk][hero][32] like the sign bytes of the river path.
+__global__ void k_flop_s_split(const int32_t* __restrict__ W, const int32_t* __restrict__ T, const uint8_t* __restrict__ cc,
+                               int b0, __hip_bfloat16* __restrict__ Sh, __hip_bfloat16* __restrict__ Sl) {
+    const int i = blockIdx.x, bl = blockIdx.y, b = b0 + bl;
+    const uint8_t a1 = cc[((size_t)b * NPF + i) * 2], a2 = cc[((size_t)b * NPF + i) * 2 + 1];
+    const int32_t* w = W + ((size_t)bl * NPF + i) * NPF;
+    const int32_t* t = T + ((size_t)bl * NPF + i) * NPF;
+    for (int j = threadIdx.x; j < NPF; j += blockDim.x) {
+        int v = 0;
+        if (i < kAliveFlop && j < kAliveFlop) {
+            const uint8_t b1 = cc[((size_t)b * NPF + j) * 2], b2 = cc[((size_t)b * NPF + j) * 2 + 1];
+            if (a1 != b1 && a1 != b2 && a2 != b1 && a2 != b2) v = 2 * w[j] + t[j] - 990;
+        }
+        const int hi = 16 * ((v + (v >= 0 ? 8 : -8)) / 16);
+        const size_t o = (((size_t)b * FNCH + (j >> 5)) * NPF + i) * 32 + (j & 31);
+        Sh[o] = __float2bfloat16((float)hi);
+        Sl[o] = __float2bfloat16((float)(v - hi));
+    }
+}
+
+// Flop showdown GEMM: C[b][col][i] = sum_j (Sh + Sl)[i][j] * (Bh + Bl)[b][col][j]
+// in four BF16 products per K step.  HT 32-hero tiles per workgroup, one
+// 32-column tile per wave (waves past the board's columns skip the math);
+// S chunks staged in LDS (double-buffered), reach fragments streamed one
+// chunk ahead; XCD-grouped numbering as in the river GEMM.
+// S staging of the flop GEMM: piece e < 2*PIECES of a chunk is uint4 x of
+// the hi (e < PIECES) or lo array; x = hero * 4 + (villains 8q .. 8q+7).
+template <int NT, int PIECES, int SPT>
+__device__ __forceinline__ void flop_load_s(uint4 (&sv)[SPT], const uint4* __restrict__ s0, const uint4* __restrict__ s1,
+                                            int c, int npieces) {
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int e = threadIdx.x + NT * q;
+        const bool lo = e >= PIECES;
+        const int x = lo ? e - PIECES : e;
+        const bool ok = e < 2 * PIECES && x < npieces;
+        sv[q] = (lo ? s1 : s0)[(size_t)c * (NPF * 4) + (ok ? x : 0)];  // every path writes sv (keeps it in registers)
+    }
+}
+template <int NT, int PIECES, int SPT, int HT>
+__device__ __forceinline__ void flop_store_s(const uint4 (&sv)[SPT], uint4 (&dst)[2][HT][2][2][32], int npieces) {
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int e = threadIdx.x + NT * q;
+        const bool lo = e >= PIECES;
+        const int x = lo ? e - PIECES : e;
+        if (e < 2 * PIECES && x < npieces) {
+            const int hero = x >> 2, qq = x & 3;
+            dst[lo ? 1 : 0][hero >> 5][qq >> 1][qq & 1][hero & 31] = sv[q];
+        }
+    }
+}
+
+template <int NW, int HT>
+__global__ __launch_bounds__((NW < 4 ? 4 : NW) * 64) void k_bt_gemm_flop(BatchedDev D, const __hip_bfloat16* __restrict__ Sh,
+                                                                         const __hip_bfloat16* __restrict__ Sl,
+                                                                         const int* __restrict__ bncol, int nlogical) {
+    constexpr int NT = (NW < 4 ? 4 : NW) * 64;  // at least four waves share the S staging
+    constexpr int HB = (NPF + 32 * HT - 1) / (32 * HT);
+    constexpr int PIECES = HT * 128;            // uint4 pieces of one S array per chunk
+    constexpr int SPT = (2 * PIECES + NT - 1) / NT;
+    __shared__ uint4 Ssh[2][2][HT][2][2][32];   // [buf][hi/lo][hero tile][h][k-step][hero]
+    const int per = (nlogical + 7) >> 3;
+    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (logical >= nlogical) return;
+    const int b = logical / HB, i0 = (logical % HB) * (32 * HT);
+    const int nht = min(HT, (NPF - i0) >> 5);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, half = lane >> 5, l32 = lane & 31;
+    const int ncolp = NW * 32;
+    const bool wave_on = wave < NW && wave * 32 < bncol[b];
+    const int col = (wave < NW ? wave : 0) * 32 + l32;
+    const uint4* ah_p = reinterpret_cast<const uint4*>(D.Bh + ((size_t)b * ncolp + col) * NPF + 16 * half);
+    const uint4* al_p = reinterpret_cast<const uint4*>(D.Bl + ((size_t)b * ncolp + col) * NPF + 16 * half);
+    const uint4* sp0 = reinterpret_cast<const uint4*>(Sh + ((size_t)b * FNCH * NPF + i0) * 32);
+    const uint4* sp1 = reinterpret_cast<const uint4*>(Sl + ((size_t)b * FNCH * NPF + i0) * 32);
+    constexpr int CH4 = NPF * 4;                // uint4 pieces per chunk of one array (all heroes)
+    const int npieces = nht * 128;
+    f32x16_t acc[HT];
+#pragma unroll
+    for (int t = 0; t < HT; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+    uint4 sv[SPT];
+    uint4 a_cur[4], a_nxt[4];
+    flop_load_s<NT, PIECES, SPT>(sv, sp0, sp1, 0, npieces);
+    a_cur[0] = ah_p[0]; a_cur[1] = ah_p[1]; a_cur[2] = al_p[0]; a_cur[3] = al_p[1];
+    flop_store_s<NT, PIECES, SPT, HT>(sv, Ssh[0], npieces);
+    __syncthreads();
+    int buf = 0;
+    for (int c = 0; c < FNCH; ++c) {
+        const bool more = c + 1 < FNCH;
+        if (more) {
+            flop_load_s<NT, PIECES, SPT>(sv, sp0, sp1, c + 1, npieces);
+            const int o = (c + 1) * 4;
+            a_nxt[0] = ah_p[o]; a_nxt[1] = ah_p[o + 1]; a_nxt[2] = al_p[o]; a_nxt[3] = al_p[o + 1];
+        }
+        if (wave_on) {
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                bf16x8_t ah, al;
+                __builtin_memcpy(&ah, &a_cur[ks], 16);
+                __builtin_memcpy(&al, &a_cur[2 + ks], 16);
+#pragma unroll
+                for (int t = 0; t < HT; ++t) {
+                    if (t < nht) {
+                        bf16x8_t bh, bl;
+                        __builtin_memcpy(&bh, &Ssh[buf][0][t][half][ks][l32], 16);
+                        __builtin_memcpy(&bl, &Ssh[buf][1][t][half][ks][l32], 16);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bl, acc[t], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        if (more) flop_store_s<NT, PIECES, SPT, HT>(sv, Ssh[buf ^ 1], npieces);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a_cur[q] = a_nxt[q];
+        buf ^= 1;
+    }
+    if (!wave_on) return;
+    float* Cb = D.C + (size_t)b * ncolp * NPF;
+#pragma unroll
+    for (int t = 0; t < HT; ++t) {
+        if (t < nht) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int cc = wave * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+                Cb[(size_t)cc * NPF + i0 + 32 * t + l32] = acc[t][r];
+            }
+        }
+    }
+}
+
+// Next iteration's weights (w_pos, w_neg, w_strat) from the precomputed
+// table: part of the captured iteration, so replays advance by themselves.
+__global__ void k_next_weights(int* __restrict__ t, const float* __restrict__ wtab, float* __restrict__ wcur) {
+    if (threadIdx.x == 0) {
+        const int n = *t + 1;
+        *t = n;
+        wcur[0] = wtab[3 * (size_t)(n - 1)];
+        wcur[1] = wtab[3 * (size_t)(n - 1) + 1];
+        wcur[2] = wtab[3 * (size_t)(n - 1) + 2];
+    }
+}
+
+constexpr int FLOP_NT = 640;   // sweep workgroup size for flop instances: two cover the 1216 padded combos
+
+}  // namespace
+
+struct BatchImpl {
+    std::vector<mtt_hu_game> games;
+    mtt_solver_cfg cfg;
+    std::vector<HuTree> trees;
+    std::vector<int> ncol;           // showdown columns per instance
+    int n = 0, dm = 4, ht = 4;
+    int ntb = 640;                   // backward workgroup size (640, or 320 for wide, deep or LDS-heavy trees)
+    int iters_done = 0, tmax = 0;
+    hipStream_t stream = nullptr;
+    KernelTimer timer;
+    DeviceArena arena;
+    BatchedDev D;
+    float* d_ranges = nullptr;
+    __hip_bfloat16 *d_Sh = nullptr, *d_Sl = nullptr;
+    int* d_bncol = nullptr;
+    int* d_t = nullptr;
+    float *d_wtab = nullptr, *d_wcur = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = true;
+    double gemm_ms[2] = {0, 0};
+};
+
+namespace {
+
+int batch_launch_gemm(BatchImpl& B, int ht) {
+    const LevelDev& L = B.D.L[2];
+    const int hb = ht == 4 ? (NPF + 127) / 128 : (NPF + 63) / 64;
+    const int nlog = B.n * hb;
+    const dim3 g(((nlog + 7) / 8) * 8);
+    const int nw = L.ncolp / 32;
+#define MTT_FG(NW_)                                                                                                   \
+    case NW_:                                                                                                         \
+        if (ht == 4) hipLaunchKernelGGL((k_bt_gemm_flop<NW_, 4>), g, dim3((NW_ < 4 ? 4 : NW_) * 64), 0, B.stream, B.D,  \
+                                        B.d_Sh, B.d_Sl, B.d_bncol, nlog);                                            \
+        else hipLaunchKernelGGL((k_bt_gemm_flop<NW_, 2>), g, dim3((NW_ < 4 ? 4 : NW_) * 64), 0, B.stream, B.D, B.d_Sh, \
+                                B.d_Sl, B.d_bncol, nlog);                                                             \
+        break;
+    switch (nw) {
+        MTT_FG(1) MTT_FG(2) MTT_FG(3) MTT_FG(4) MTT_FG(5) MTT_FG(6) MTT_FG(7) MTT_FG(8) MTT_FG(9)
+        default: set_error("flop batch: %d showdown column tiles exceed 9", nw); return MTT_ERR_ARG;
+    }
+#undef MTT_FG
+    MTT_HIP(hipGetLastError());
+    return MTT_OK;
+}
+
+int batch_iteration(BatchImpl& B) {
+    const LevelDev& L = B.D.L[2];
+    // a third register vector for more than 64 slots; the backward runs 320-thread groups (more registers per
+    // thread) for those and for deep trees
+    const bool wide = L.maxslot > 64;
+    const int ntb = B.ntb;
+    const size_t shf = (size_t)L.maxterm * 104 * 8;
+    const size_t shb = bwd_lds(L, ntb);
+    const dim3 gf(B.n * 2), gb(B.n * ((NPF + ntb - 1) / ntb));  // backward: ceil(NPF / ntb) workgroups per instance
+    hipLaunchKernelGGL(k_next_weights, dim3(1), dim3(64), 0, B.stream, B.d_t, B.d_wtab, B.d_wcur);
+#define MTT_FWD(DM_, NRV_) \
+    hipLaunchKernelGGL((k_bt_forward<2, DM_, FLOP_NT, NRV_>), gf, dim3(FLOP_NT), shf, B.stream, B.D, B.d_ranges, 0.0f, B.d_wcur)
+#define MTT_BWD(DM_, NT_, NRV_) \
+    hipLaunchKernelGGL((k_bt_backward<2, DM_, NT_, NRV_>), gb, dim3(NT_), shb, B.stream, B.D, B.d_ranges, 0.0f, 0.0f, B.d_wcur)
+    MTT_TIMED(B, "flop_forward",
+              if (B.dm == 4) { if (wide) MTT_FWD(4, 3); else MTT_FWD(4, 2); }
+              else { if (wide) MTT_FWD(8, 3); else MTT_FWD(8, 2); });
+    int rc = MTT_OK;
+    MTT_TIMED(B, "flop_showdown_gemm", rc = batch_launch_gemm(B, B.ht));
+    if (rc) return rc;
+    MTT_TIMED(B, "flop_backward",
+              if (B.dm == 4) {
+                  if (wide) MTT_BWD(4, FLOP_NT / 2, 3);
+                  else if (ntb == FLOP_NT) MTT_BWD(4, FLOP_NT, 2);
+                  else MTT_BWD(4, FLOP_NT / 2, 2);
+              } else { if (wide) MTT_BWD(8, FLOP_NT / 2, 3); else MTT_BWD(8, FLOP_NT / 2, 2); });
+#undef MTT_FWD
+#undef MTT_BWD
+    MTT_HIP(hipGetLastError());
+    return MTT_OK;
+}
+
+// Weight table for iterations [1, tmax] on the device.
+int batch_weights(BatchImpl& B, int tmax) {
+    std::vector<float> w((size_t)3 * tmax);
+    for (int t = 1; t <= tmax; ++t) iteration_weights(B.cfg, t, w[3 * (t - 1)], w[3 * (t - 1) + 1], w[3 * (t - 1) + 2]);
+    float* d = nullptr;
+    int rc = B.arena.upload(w, &d);
+    if (rc) return rc;
+    B.d_wtab = d;
+    B.tmax = tmax;
+    return MTT_OK;
+}
+
+// Autotuner: the flop GEMM's hero-tile count (2 or 4 tiles of 32 per
+// workgroup) is timed on the real operands once per column-tile count and
+// the winner is kept in a small JSON file ($MTT_AUTOTUNE_FILE, else
+// $HOME/.cache/mtt/autotune.json); a file that cannot be read or written
+// only means tuning again.
+std::string tune_path() {
+    const char* e = getenv("MTT_AUTOTUNE_FILE");
+    if (e && *e) return e;
+    const char* h = getenv("HOME");
+    return std::string(h && *h ? h : "/tmp") + "/.cache/mtt/autotune.json";
+}
+
+int tune_lookup(const std::string& key) {
+    FILE* f = fopen(tune_path().c_str(), "r");
+    if (!f) return 0;
+    char buf[8192];
+    size_t n = fread(buf, 1, sizeof(buf) - 1, f);
+    fclose(f);
+    buf[n] = 0;
+    std::string s(buf);
+    size_t p = s.find("\"" + key + "\"");
+    if (p == std::string::npos) return 0;
+    p = s.find(':', p);
+    if (p == std::string::npos) return 0;
+    return atoi(s.c_str() + p + 1);
+}
+
+void tune_store(const std::string& key, int v) {
+    std::string path = tune_path();
+    std::string dir = path.substr(0, path.rfind('/'));
+    std::string mk = "mkdir -p '" + dir + "'";
+    if (dir.find('\'') == std::string::npos) (void)system(mk.c_str());
+    std::string s;
+    if (FILE* f = fopen(path.c_str(), "r")) {
+        char buf[8192];
+        size_t n = fread(buf, 1, sizeof(buf) - 1, f);
+        fclose(f);
+        buf[n] = 0;
+        s = buf;
+    }
+    std::string entry = "\"" + key + "\": " + std::to_string(v);
+    size_t p = s.find("\"" + key + "\"");
+    if (p != std::string::npos) {
+        size_t e = s.find_first_of(",}", p);
+        s = s.substr(0, p) + entry + s.substr(e);
+    } else if (s.find('}') != std::string::npos && s.find(':') != std::string::npos) {
+        s = s.substr(0, s.rfind('}')) + ", " + entry + "}";
+    } else {
+        s = "{" + entry + "}";
+    }
+    if (FILE* f = fopen(path.c_str(), "w")) {
+        fputs(s.c_str(), f);
+        fputs("\n", f);
+        fclose(f);
+    }
+}
+
+int batch_autotune(BatchImpl& B) {
+    const std::string key = "flop_gemm_ht/nw" + std::to_string(B.D.L[2].ncolp / 32);
+    int v = tune_lookup(key);
+    if (v == 2 || v == 4) { B.ht = v; return MTT_OK; }
+    hipEvent_t a, b;
+    MTT_HIP(hipEventCreate(&a));
+    MTT_HIP(hipEventCreate(&b));
+    const int cand[2] = {2, 4};
+    for (int k = 0; k < 2; ++k) {
+        int rc = batch_launch_gemm(B, cand[k]);  // warm
+        if (rc) return rc;
+        MTT_HIP(hipEventRecord(a, B.stream));
+        for (int r = 0; r < 3; ++r) batch_launch_gemm(B, cand[k]);
+        MTT_HIP(hipEventRecord(b, B.stream));
+        MTT_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        MTT_HIP(hipEventElapsedTime(&ms, a, b));
+        B.gemm_ms[k] = ms / 3.0;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    B.ht = B.gemm_ms[0] < B.gemm_ms[1] ? 2 : 4;
+    tune_store(key, B.ht);
+    return MTT_OK;
+}
+
+}  // namespace
+
+int batch_create(const mtt_hu_game* games, int n, const float* ranges, const mtt_solver_cfg* cfg, BatchImpl** out) {
+    *out = nullptr;
+    auto h = new (std::nothrow) BatchImpl;
+    if (!h) { set_error("mtt_batch_create: out of host memory"); return MTT_ERR_OOM; }
+    std::unique_ptr<BatchImpl> own(h);
+    BatchImpl& B = *h;
+    B.n = n;
+    B.games.assign(games, games + n);
+    if (cfg) B.cfg = *cfg;
+    else { memset(&B.cfg, 0, sizeof(B.cfg)); B.cfg.scheme = MTT_SCHEME_CFR_PLUS; }
+    memset(&B.D, 0, sizeof(B.D));
+    B.trees.resize(n);
+    LevelHost H;
+    int maxdepth = 0;
+    for (int k = 0; k < n; ++k) {
+        const mtt_hu_game& g = games[k];
+        MTT_ARG(g.n_board == 3 && !g.root_chance && g.last_street == 0,
+                "mtt_batch_create: instance %d is not a flop subgame (3-card board, flop betting, showdown over the "
+                "turn and river)", k);
+        int rc = build_hu_tree(g, B.trees[k], true);
+        if (rc) return rc;
+        MTT_ARG(B.trees[k].templates.size() == 1, "mtt_batch_create: instance %d has more than one street", k);
+        rc = add_template(B.trees[k].templates[0], H, 0);
+        if (rc) return rc;
+        B.ncol.push_back(2 * B.trees[k].templates[0].n_showdown);
+        maxdepth = std::max(maxdepth, H.maxdepth);
+    }
+    B.dm = maxdepth + 1 <= 4 ? 4 : 8;
+    MTT_ARG(maxdepth < 8, "mtt_batch_create: a tree deeper than 8 decisions");
+    // boards: alive combos of each flop in the reference order
+    std::vector<uint8_t> cc((size_t)n * NPF * 2, 0);
+    std::vector<int> nlive(n, kAliveFlop);
+    std::vector<float> rr((size_t)n * 2 * NPF, 0.0f), pot0(n);
+    std::vector<int> btmpl(n);
+    std::vector<uint8_t> flops((size_t)n * 3);
+    for (int k = 0; k < n; ++k) {
+        const mtt_hu_game& g = games[k];
+        const uint64_t fm = (1ull << g.board[0]) | (1ull << g.board[1]) | (1ull << g.board[2]);
+        int i = 0;
+        for (int a = 0; a < 52; ++a)
+            for (int c = a + 1; c < 52; ++c) {
+                if (((1ull << a) | (1ull << c)) & fm) continue;
+                cc[((size_t)k * NPF + i) * 2] = (uint8_t)a;
+                cc[((size_t)k * NPF + i) * 2 + 1] = (uint8_t)c;
+                const int gi = combo_index(a, c);
+                rr[((size_t)k * 2 + 0) * NPF + i] = ranges[((size_t)k * 2 + 0) * 1326 + gi];
+                rr[((size_t)k * 2 + 1) * NPF + i] = ranges[((size_t)k * 2 + 1) * 1326 + gi];
+                i++;
+            }
+        pot0[k] = g.pot;
+        btmpl[k] = k;
+        for (int q = 0; q < 3; ++q) flops[(size_t)k * 3 + q] = g.board[q];
+    }
+    MTT_HIP(hipStreamCreate(&B.stream));
+    DeviceArena& A = B.arena;
+    LevelDev& L = B.D.L[2];
+    // the backward must fit at the smaller workgroup size; it runs at FLOP_NT when the trees are neither wide
+    // (more than 64 slots: a third register vector) nor deep (8-level reach vectors) and their LDS still fits
+    int rc = upload_level(A, B.stream, H, L, NPF, n, cc, nlive, true, FLOP_NT / 2);
+    if (rc) return rc;
+    B.ntb = (L.maxslot > 64 || B.dm == 8 || bwd_lds(L, FLOP_NT) > 152 * 1024) ? FLOP_NT / 2 : FLOP_NT;
+    int* d_bt;
+    float* d_pot0;
+    rc = rc ? rc : A.upload(btmpl, &d_bt);
+    rc = rc ? rc : A.upload(pot0, &d_pot0);
+    rc = rc ? rc : A.upload(rr, &B.d_ranges);
+    rc = rc ? rc : A.upload(B.ncol, &B.d_bncol);
+    if (rc) return rc;
+    L.btmpl = d_bt;
+    L.bpot0 = d_pot0;
+    MTT_ARG(L.ncolp <= GCOLS, "mtt_batch_create: %d showdown columns exceed %d", L.ncol, GCOLS);
+    const size_t nbc = (size_t)n * L.ncolp * NPF;
+    rc = rc ? rc : A.alloc(nbc, &B.D.Bh);
+    rc = rc ? rc : A.alloc(nbc, &B.D.Bl);
+    rc = rc ? rc : A.alloc(nbc, &B.D.C);
+    rc = rc ? rc : A.alloc((size_t)n * 2 * NPF, &B.D.rootval0);
+    rc = rc ? rc : A.alloc((size_t)n * FNCH * NPF * 32, &B.d_Sh);
+    rc = rc ? rc : A.alloc((size_t)n * FNCH * NPF * 32, &B.d_Sl);
+    rc = rc ? rc : A.alloc(1, &B.d_t);
+    rc = rc ? rc : A.alloc(3, &B.d_wcur);
+    if (rc) return rc;
+    MTT_HIP(hipMemsetAsync(B.D.Bh, 0, nbc * 2, B.stream));
+    MTT_HIP(hipMemsetAsync(B.D.Bl, 0, nbc * 2, B.stream));
+    MTT_HIP(hipMemsetAsync(B.D.C, 0, nbc * 4, B.stream));
+    MTT_HIP(hipMemsetAsync(B.d_t, 0, sizeof(int), B.stream));
+    // W - L of every instance, from the exhaustive counts, 16 flops at a time
+    {
+        const int CH = 16;
+        DeviceArena tmp;
+        int32_t *dW, *dT;
+        rc = tmp.alloc((size_t)CH * NPF * NPF, &dW);
+        rc = rc ? rc : tmp.alloc((size_t)CH * NPF * NPF, &dT);
+        if (rc) return rc;
+        for (int k0 = 0; k0 < n; k0 += CH) {
+            const int m = std::min(CH, n - k0);
+            rc = mtt_flop_counts(flops.data() + (size_t)k0 * 3, m, dW, dT, MTT_LAYOUT_ALIVE1216, B.stream);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_flop_s_split, dim3(NPF, m), dim3(256), 0, B.stream, dW, dT, L.cc, k0, B.d_Sh, B.d_Sl);
+            MTT_HIP(hipGetLastError());
+        }
+        MTT_HIP(hipStreamSynchronize(B.stream));
+    }
+    rc = batch_weights(B, 4096);
+    if (rc) return rc;
+    rc = batch_autotune(B);
+    if (rc) return rc;
+    MTT_HIP(hipStreamSynchronize(B.stream));
+    *out = own.release();
+    return MTT_OK;
+}
+
+int batch_run(BatchImpl& B, int iters) {
+    if (B.iters_done + iters > B.tmax) {
+        int rc = batch_weights(B, std::max(2 * B.tmax, B.iters_done + iters));
+        if (rc) return rc;
+        if (B.exec) { (void)hipGraphExecDestroy(B.exec); B.exec = nullptr; }
+        if (B.graph) { (void)hipGraphDestroy(B.graph); B.graph = nullptr; }
+    }
+    if (!B.use_graph || B.timer.enabled) {
+        for (int it = 0; it < iters; ++it) {
+            int rc = batch_iteration(B);
+            if (rc) return rc;
+        }
+    } else {
+        if (!B.exec) {
+            MTT_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+            int rc = batch_iteration(B);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(B.stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+            B.graph = g;
+            MTT_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+        }
+        for (int it = 0; it < iters; ++it) MTT_HIP(hipGraphLaunch(B.exec, B.stream));
+    }
+    B.iters_done += iters;
+    MTT_HIP(hipStreamSynchronize(B.stream));
+    return MTT_OK;
+}
+
+// Raw regrets / strategy sums of one instance in its own flat layout
+// [infoset][MTT_MAX_ACTIONS][1216].
+int batch_state(BatchImpl& B, int k, float* R, float* S, size_t nf) {
+    const HuTree& tr = B.trees[k];
+    const size_t need = (size_t)tr.n_infosets * MTT_MAX_ACTIONS * NPF;
+    MTT_ARG(nf >= need, "mtt_batch_get_state: buffer of %zu floats, need %zu", nf, need);
+    const LevelDev& L = B.D.L[2];
+    const size_t per = (size_t)L.nslot * NPF;
+    std::vector<float> hR(per), hS(per);
+    MTT_HIP(hipStreamSynchronize(B.stream));
+    MTT_HIP(hipMemcpy(hR.data(), L.R + (size_t)k * per, per * 4, hipMemcpyDeviceToHost));
+    MTT_HIP(hipMemcpy(hS.data(), L.S + (size_t)k * per, per * 4, hipMemcpyDeviceToHost));
+    if (R) memset(R, 0, need * 4);
+    if (S) memset(S, 0, need * 4);
+    const Template& T = tr.templates[0];
+    for (size_t id = 0; id < tr.flat.size(); ++id) {
+        const mtt_flat_node& x = tr.flat[id];
+        if (x.type != MTT_NODE_DECISION) continue;
+        const int slot = T.nodes[tr.flat_tnode[id]].slot;
+        for (int a = 0; a < x.num_actions; ++a)
+            for (int i = 0; i < kAliveFlop; ++i) {
+                const size_t o = ((size_t)x.infoset_offset * MTT_MAX_ACTIONS + a) * NPF + i;
+                if (R) R[o] = hR[(size_t)(slot + a) * NPF + i];
+                if (S) S[o] = hS[(size_t)(slot + a) * NPF + i];
+            }
+    }
+    return MTT_OK;
+}
+
+void batch_destroy(BatchImpl* B) {
+    if (!B) return;
+    (void)hipStreamSynchronize(B->stream);
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->arena.release();
+    (void)hipStreamDestroy(B->stream);
+    delete B;
+}
+
 }  // namespace mtt
+
+struct mtt_batch {
+    mtt::BatchImpl* impl;
+};
+
+extern "C" {
+
+int mtt_batch_create(const mtt_hu_game* games, int n, const float* ranges, const mtt_solver_cfg* cfg, mtt_batch** out) {
+    using namespace mtt;
+    MTT_ARG(games && ranges && out, "mtt_batch_create: null pointer");
+    MTT_ARG(n >= 1 && n <= 65535, "mtt_batch_create: instance count %d out of 1..65535", n);
+    *out = nullptr;
+    BatchImpl* b = nullptr;
+    int rc = batch_create(games, n, ranges, cfg, &b);
+    if (rc) return rc;
+    auto h = new (std::nothrow) mtt_batch;
+    if (!h) { batch_destroy(b); set_error("mtt_batch_create: out of host memory"); return MTT_ERR_OOM; }
+    h->impl = b;
+    *out = h;
+    return MTT_OK;
+}
+
+int mtt_batch_run(mtt_batch* h, int iters) {
+    using namespace mtt;
+    MTT_ARG(h && iters >= 0, "mtt_batch_run: bad arguments");
+    return batch_run(*h->impl, iters);
+}
+
+int mtt_batch_iterations(const mtt_batch* h) { return h ? h->impl->iters_done : -1; }
+
+int mtt_batch_tree_info(mtt_batch* h, int instance, int* n_nodes, int* n_infosets) {
+    using namespace mtt;
+    MTT_ARG(h && n_nodes && n_infosets && instance >= 0 && instance < h->impl->n, "mtt_batch_tree_info: bad arguments");
+    *n_nodes = (int)h->impl->trees[instance].flat.size();
+    *n_infosets = h->impl->trees[instance].n_infosets;
+    return MTT_OK;
+}
+
+int mtt_batch_get_state(mtt_batch* h, int instance, float* R_out, float* S_out, size_t n_floats) {
+    using namespace mtt;
+    MTT_ARG(h && instance >= 0 && instance < h->impl->n, "mtt_batch_get_state: bad arguments");
+    return batch_state(*h->impl, instance, R_out, S_out, n_floats);
+}
+
+int mtt_batch_set_graph(mtt_batch* h, int on) {
+    using namespace mtt;
+    MTT_ARG(h, "mtt_batch_set_graph: null handle");
+    h->impl->use_graph = on != 0;
+    return MTT_OK;
+}
+
+int mtt_batch_tuning(mtt_batch* h, int* hero_tiles, double* gemm_ms) {
+    using namespace mtt;
+    MTT_ARG(h && hero_tiles, "mtt_batch_tuning: bad arguments");
+    *hero_tiles = h->impl->ht;
+    if (gemm_ms) { gemm_ms[0] = h->impl->gemm_ms[0]; gemm_ms[1] = h->impl->gemm_ms[1]; }
+    return MTT_OK;
+}
+
+int mtt_batch_timing(mtt_batch* h, int enable) {
+    using namespace mtt;
+    MTT_ARG(h, "mtt_batch_timing: null handle");
+    MTT_HIP(hipStreamSynchronize(h->impl->stream));
+    h->impl->timer.collect();
+    h->impl->timer.enabled = enable != 0;
+    h->impl->timer.names.clear();
+    h->impl->timer.ms.clear();
+    h->impl->timer.count.clear();
+    return MTT_OK;
+}
+
+int mtt_batch_kernel_times(mtt_batch* h, int idx, char* name, int name_cap, double* total_ms, int64_t* launches) {
+    using namespace mtt;
+    MTT_ARG(h && name && total_ms && launches && name_cap > 0, "mtt_batch_kernel_times: bad arguments");
+    MTT_HIP(hipStreamSynchronize(h->impl->stream));
+    h->impl->timer.collect();
+    if (idx < 0 || idx >= (int)h->impl->timer.names.size()) return -1;
+    snprintf(name, name_cap, "%s", h->impl->timer.names[idx].c_str());
+    *total_ms = h->impl->timer.ms[idx];
+    *launches = h->impl->timer.count[idx];
+    return MTT_OK;
+}
+
+int mtt_batch_destroy(mtt_batch* h) {
+    if (!h) return MTT_OK;
+    mtt::batch_destroy(h->impl);
+    delete h;
+    return MTT_OK;
+}
+
+}  // extern "C"

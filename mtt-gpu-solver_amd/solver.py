@@ -212,3 +212,108 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------- batched flop subgames
+for name, args in {
+    "mtt_batch_create": [_P, _I, _P, _P, _P],
+    "mtt_batch_run": [_P, _I],
+    "mtt_batch_iterations": [_P],
+    "mtt_batch_tree_info": [_P, _I, _P, _P],
+    "mtt_batch_get_state": [_P, _I, _P, _P, ctypes.c_size_t],
+    "mtt_batch_set_graph": [_P, _I],
+    "mtt_batch_tuning": [_P, _P, _P],
+    "mtt_batch_timing": [_P, _I],
+    "mtt_batch_kernel_times": [_P, _I, ctypes.c_char_p, _I, _P, _P],
+    "mtt_batch_destroy": [_P],
+}.items():
+    mtt.register(name, args)
+
+CONFIG4_SIZES = (0.25, 0.33, 0.5, 0.75, 1.0, 1.5)
+
+
+def config4(seed=0, n=512):
+    """Batched uneven heads-up flop subgames: per instance a seeded flop and
+    Uniform[0,1] ranges over its unblocked combos (the project generator,
+    seed * 100003 + k), stacks Uniform[5,60]bb, pot Uniform[2,12]bb, a menu
+    of 1-3 sizes from {25,33,50,75,100,150}% pot with one 2.5x raise size
+    and a raise cap of 1-3 (no all-in option); flop betting, then showdown
+    over the turn and river.  Menu/stack/pot draws from numpy's PCG64."""
+    rng = np.random.default_rng(seed)
+    games, ranges = [], np.zeros((n, 2, 1326), np.float32)
+    for k in range(n):
+        board, r0, r1 = seeded_inputs(seed * 100003 + k, 3, 0)
+        stack = float(rng.uniform(5.0, 60.0))
+        pot = float(rng.uniform(2.0, 12.0))
+        ns = int(rng.integers(1, 4))
+        sizes = tuple(sorted(rng.choice(CONFIG4_SIZES, ns, replace=False).tolist()))
+        cap = int(rng.integers(1, 4))
+        games.append(make_game(board, pot, stack, {0: menu(sizes, 2.5, cap, 0)}, 0, 0))
+        ranges[k, 0], ranges[k, 1] = r0, r1
+    return games, ranges
+
+
+class Batch:
+    """Many heads-up flop subgames solved together (mtt_batch_*)."""
+
+    def __init__(self, games, ranges, scheme=SCHEME_DCFR, alpha=1.5, beta=0.0, gamma=2.0):
+        self.games = (Game * len(games))(*games)
+        self.ranges = np.ascontiguousarray(ranges, np.float32)
+        cfg = SolverCfg(MODE_BF16_SPLIT, scheme, alpha, beta, gamma)
+        h = ctypes.c_void_p()
+        mtt.call("mtt_batch_create", self.games, len(games), self.ranges.ctypes.data, ctypes.byref(cfg),
+                 ctypes.byref(h))
+        self.h = h
+        self.n = len(games)
+
+    def run(self, iters):
+        mtt.call("mtt_batch_run", self.h, iters)
+
+    def set_graph(self, on):
+        mtt.call("mtt_batch_set_graph", self.h, 1 if on else 0)
+
+    def tree_info(self, k):
+        nn, ni = ctypes.c_int(), ctypes.c_int()
+        mtt.call("mtt_batch_tree_info", self.h, k, ctypes.byref(nn), ctypes.byref(ni))
+        return nn.value, ni.value
+
+    def state(self, k):
+        _, ni = self.tree_info(k)
+        n = ni * mtt.MAX_ACTIONS * 1216
+        R = np.zeros(n, np.float32)
+        S = np.zeros(n, np.float32)
+        mtt.call("mtt_batch_get_state", self.h, k, R.ctypes.data, S.ctypes.data, n)
+        return R, S
+
+    def tuning(self):
+        ht = ctypes.c_int()
+        ms = (ctypes.c_double * 2)()
+        mtt.call("mtt_batch_tuning", self.h, ctypes.byref(ht), ms)
+        return ht.value, (ms[0], ms[1])
+
+    def timing(self, on):
+        mtt.call("mtt_batch_timing", self.h, 1 if on else 0)
+
+    def kernel_times(self):
+        out = {}
+        buf = ctypes.create_string_buffer(64)
+        k = 0
+        while True:
+            ms, n = ctypes.c_double(), ctypes.c_int64()
+            rc = mtt.lib().mtt_batch_kernel_times(self.h, k, buf, 64, ctypes.byref(ms), ctypes.byref(n))
+            if rc != 0:
+                break
+            out[buf.value.decode()] = (ms.value, n.value)
+            k += 1
+        return out
+
+    def close(self):
+        if self.h:
+            mtt.lib().mtt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
