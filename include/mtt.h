@@ -278,6 +278,19 @@ int mtt_solver_kernel_times(mtt_solver* h, int idx, char* name, int name_cap, do
 /* info: [0] nodes [1] infosets [2] alive combos [3] device bytes
  * [4] decision nodes [5] showdown nodes [6] fold nodes [7] matrices */
 int mtt_solver_info(mtt_solver* h, int64_t* info);
+/* Generated sweeps (no counterpart in the reference, whose sweeps are fixed
+ * kernels): in MTT_MODE_BF16_SPLIT the turn and river sweeps are emitted as
+ * straight-line HIP per street template and compiled with hipRTC at create
+ * (environment MTT_JIT=0 selects the interpreted sweeps).  active = 1 when
+ * the handle runs generated sweeps; compile_seconds = hipRTC compile and load
+ * time (0 when the same code was already built in this process). */
+int mtt_solver_jit_info(mtt_solver* h, int* active, double* compile_seconds);
+/* The generated sweep source of a game's street templates, and a
+ * compile-only hipRTC build of it for gfx950 (neither needs a device).
+ * mtt_hu_sweep_source writes at most cap-1 chars plus a terminator and the
+ * full length to *len. */
+int mtt_hu_sweep_source(const mtt_hu_game* game, char* buf, size_t cap, size_t* len);
+int mtt_hu_sweep_compile(const mtt_hu_game* game, size_t* code_bytes, double* seconds);
 int mtt_solver_destroy(mtt_solver* h);
 
 /* ----------------------------------------------- batched flop subgames -- */

@@ -1,0 +1,58 @@
+// Per-template sweeps of the batched heads-up path, generated at run time.
+//
+// The ahead-of-time sweep kernels interpret a template's DFS event records:
+// every event is decoded on the scalar unit and the regrets are read through
+// indexed registers.  For the turn and river levels (few templates, each
+// instantiated over many boards) the walk is instead emitted as straight-line
+// HIP per template -- slots, columns, terminal offsets and pots become
+// literals, regrets become named registers -- and compiled once with hipRTC
+// for the device.  The generated kernels take the same arguments, grids and
+// LDS as the interpreted ones and compute the same recurrence.
+#pragma once
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "bt_dev.h"
+
+namespace mtt {
+
+// One street level's templates (level 0: turn, 1: river or the single street).
+struct JitLevel {
+    int level = 0;
+    int np = 0;                              // padded combos per board
+    int ntmpl = 0;
+    const std::vector<bt::DNode>* nodes = nullptr;  // DFS preorder per template, concatenated
+    const std::vector<int>* tnode0 = nullptr;       // [ntmpl+1] first node of each template
+    const std::vector<int>* tev0 = nullptr;         // [ntmpl+1] first event of each template
+    const std::vector<int>* events = nullptr;       // template-local node << 2 | EV_*
+    std::vector<int> slot0, term0, col0;            // [ntmpl+1] level-global firsts
+};
+
+struct JitSweeps {
+    bool active = false;
+    double compile_s = 0.0;                  // hipRTC compile + load (0 when cached)
+    hipFunction_t fwd[2] = {nullptr, nullptr};
+    hipFunction_t bwd[2] = {nullptr, nullptr};
+};
+
+// Threads per workgroup of the generated sweeps (as the interpreted ones).
+constexpr int JIT_NT = 576;
+
+// Generates, compiles and loads the sweeps of the given levels for the
+// current device.  Compiled modules are cached per (device, source) for the
+// life of the process.  Returns MTT_OK or an error code with
+// mtt_last_error() set.
+int bt_jit_build(const JitLevel* levels, int n, JitSweeps* out);
+
+// The generated source for the given levels (for inspection and tests).
+std::string bt_jit_source(const JitLevel* levels, int n);
+
+// hipRTC compile of a generated source for gfx950 (no device needed).
+int bt_jit_compile(const std::string& src, std::vector<char>* code);
+
+// Whether generated sweeps are enabled: MTT_JIT unset or not "0".
+bool bt_jit_enabled();
+
+}  // namespace mtt

@@ -17,6 +17,7 @@
 //     DCFR-generalised regret / strategy update in the same pass.
 // River combos are ordered by hand strength on each board.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -25,91 +26,21 @@
 
 #include <hip/hip_bf16.h>
 
+#include "bt_dev.h"
+#include "bt_jit.h"
 #include "solver.h"
 
 namespace mtt {
 
+using namespace bt;
+
 namespace {
-
-constexpr int NPR = 1088;   // river combos padded (1081 live)
-constexpr int NPT = 1152;   // turn combos padded (1128 live)
-constexpr int NPF = 1216;   // flop combos padded (1176 live), the reference's padding
-
-enum : uint8_t { DN_DEC = 0, DN_FOLD = 1, DN_SD = 2, DN_LEAF = 3 };
-enum : int { EV_ENTER = 0, EV_TERM = 1, EV_LEAVE = 2 };
-
-struct DNode {
-    uint8_t type, player, nact, depth;
-    int16_t act, slot;     // action index from parent; first slot (decision)
-    int16_t col, sidx;     // showdown column base; card-sum terminal index
-    int16_t leaf, pad;     // leaf ordinal (turn level); parent's actor
-    float pot, c0, c1;
-};
-static_assert(sizeof(DNode) == 28, "DNode layout");
 
 constexpr int MAXN = 128;      // nodes per template
 constexpr int MAXE = 256;      // DFS events per template
 constexpr int MAXSLOT = 96;    // action slots per template (three 32-register vectors)
-constexpr double FX = 281474976710656.0;  // 2^48: fixed-point scale of card sums
-
-// One DFS event of a template with its node record, 32 bytes, read by the
-// sweeps through scalar loads.
-struct EvRec {
-    DNode nd;
-    int kind;              // EV_*
-};
-static_assert(sizeof(EvRec) == 32, "EvRec layout");
-
-struct LevelDev {
-    int np, nslot, nterm, ncol, ncolp, ntmpl, nboards;
-    int maxterm;               // largest terminal count of one template
-    int maxcolt;               // largest showdown column count of one template
-    int maxslot;               // largest action-slot count of one template
-    const EvRec* ev;           // DFS events of every template, concatenated
-    const int* tev0;           // [ntmpl+1]
-    const EvRec* fev;          // the same without LEAVE events (preorder nodes)
-    const int* tfev0;          // [ntmpl+1]
-    const int* tslot0;         // [ntmpl+1] first slot of each template
-    const int* tterm0;         // [ntmpl+1] first terminal of each template
-    const int* tcol0;          // [ntmpl+1] first showdown column of each template
-    float* R;                  // [board][nslot][np]
-    float* S;
-    unsigned long long* part;  // [board][nterm][half][2][52] card sums of reach, 2^48 fixed point
-    const uint8_t* cc;         // [board][np][2] cards of combo
-    const int* nlive;          // [board]
-    const int* btmpl;          // flop instances: board -> its private template (else null)
-    const float* bpot0;        // flop instances: starting pot of each board
-};
-
-struct BatchedDev {
-    LevelDev L[3];             // L[0] turn (two-level games), L[1] river, L[2] flop instances
-    int two_level;
-    int nleaf;                 // turn template leaves (= river templates)
-    const int* leaf_tmpl;      // turn leaf -> river template
-    const int* tmpl_leaf;      // river template -> turn leaf
-    float* leafreach;          // [49][nleaf][2][NPT]
-    float* leafval;            // [49][nleaf][2][NPT] turn-leaf values (mean over rivers)
-    float* rootval;            // [boards1][ntmpl1][2][NPR]
-    float* rootval0;           // [49][2][NPT]
-    const uint16_t* rmap;      // [boards1][NPR] river combo -> turn combo
-    const uint16_t* imap;      // [boards1][NPT] turn combo -> river combo (0xFFFF dead)
-    __hip_bfloat16* Bh;        // [boards1][ncolp][NPR]
-    __hip_bfloat16* Bl;
-    float* C;                  // [boards1][ncolp][NPR]
-    const uint16_t* grank;     // [boards1][NPR] strength group, ascending
-    const uint32_t* cx;        // [boards1][NPR] c1|c2<<8|c1<<16|c2<<24
-    const uint32_t* cy;        // [boards1][NPR] c1|c1<<8|c2<<16|c2<<24
-    float* B0;                 // [49][ncol0][NPT] turn showdown columns
-    float* C0;
-    const float* Eturn;        // [49][NPT][NPT] villain-major
-    float pot0;
-};
 
 typedef float f32x32_t __attribute__((ext_vector_type(32)));
-typedef const __attribute__((address_space(4))) int* cint_p;
-
-// Wave-uniform reads of small per-level tables through the scalar data cache.
-__device__ __forceinline__ int ldu(const int* p) { return *(cint_p)p; }
 
 // Event records are staged in LDS per workgroup; the walk keeps the next
 // record in vector registers (issued one event ahead) and makes the current
@@ -152,32 +83,6 @@ __device__ __forceinline__ float rselv(const f32x32_t& a, const f32x32_t& b, con
     }
 }
 
-// 1/n for an action count n in 1..4 (wave-uniform; exact constants).
-__device__ __forceinline__ float uniform_share(int n) {
-    return n == 2 ? 0.5f : (n == 3 ? 1.0f / 3.0f : (n == 4 ? 0.25f : 1.0f));
-}
-
-
-// Non-blocked reach mass of the other player for a combo holding cards
-// a, b: total - sum[a] - sum[b] + own (the pair sharing both cards is the
-// combo itself and was subtracted twice), exact in 2^48 fixed point.  The
-// first difference can be negative (by at most own), so it is taken as a
-// two's-complement signed value.
-__device__ __forceinline__ float mass_u(const unsigned long long* s, int a, int b, float self) {
-    const long long m = (long long)(s[52] - s[a] - s[b]);
-    return (float)((double)m * (1.0 / FX) + (double)self);
-}
-
-__device__ __forceinline__ float root_reach(const BatchedDev& D, const float* __restrict__ ranges, int level, int b,
-                                            int tm, int np, int k, int p) {
-    if (level == 1 && D.two_level) {
-        int t = b / 48, leaf = D.tmpl_leaf[tm];
-        int ti = D.rmap[(size_t)b * NPR + k];
-        return D.leafreach[(((size_t)t * D.nleaf + leaf) * 2 + p) * NPT + ti];
-    }
-    return ranges[((size_t)b * 2 + p) * np + k];
-}
-
 constexpr int SWEEP_NT = 576;   // threads per sweep workgroup; two workgroups cover one board's combos
 
 // ------------------------------------------------------------ forward ----
@@ -212,7 +117,7 @@ __global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __
     const int t0 = ldu(L.tterm0 + tm), nt = ldu(L.tterm0 + tm + 1) - t0;
     // flop instances keep per-board state for their own template only
     const int sb = LEVEL == 2 ? 0 : s0, tb = LEVEL == 2 ? 0 : t0, cb0 = LEVEL == 2 ? ldu(L.tcol0 + tm) : 0;
-    for (int e = tid; e < nt * 104; e += NT) bins[e] = 0ull;
+    fwd_bins_clear<NT>(bins, nt);
     ev_stage(sev, L.fev + e0, e1 - e0);
     int c1 = 0, c2 = 0;
     float root0 = 0.0f, root1 = 0.0f;
@@ -285,11 +190,8 @@ __global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __
             if (LEVEL >= 1) {
                 constexpr int NX = LEVEL == 2 ? NPF : NPR;
                 const size_t cb = ((size_t)b * L.ncolp + (nd.col - cb0)) * NX + k;
-                const __hip_bfloat16 h0 = __float2bfloat16(o0), h1 = __float2bfloat16(o1);
-                D.Bh[cb] = h0;
-                D.Bl[cb] = __float2bfloat16(o0 - __bfloat162float(h0));
-                D.Bh[cb + NX] = h1;
-                D.Bl[cb + NX] = __float2bfloat16(o1 - __bfloat162float(h1));
+                store_split(D.Bh, D.Bl, cb, o0);
+                store_split(D.Bh, D.Bl, cb + NX, o1);
             } else {
                 const size_t cb = ((size_t)b * L.ncolp + nd.col) * NPT + k;
                 D.B0[cb] = o0;
@@ -310,9 +212,7 @@ __global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __
             }
         }
     }
-    __syncthreads();
-    unsigned long long* out = L.part + ((size_t)b * L.nterm + tb) * 208 + half * 104;
-    for (int e = tid; e < nt * 104; e += NT) out[(size_t)(e / 104) * 208 + e % 104] = bins[e];
+    fwd_bins_flush<NT>(L, bins, b, tb, half, nt);
 }
 
 // ----------------------------------------------------------- backward ----
@@ -353,16 +253,7 @@ __global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* 
     // flop instances keep per-board state for their own template only
     const int sb = LEVEL == 2 ? 0 : s0, tb = LEVEL == 2 ? 0 : t0, cbase = LEVEL == 2 ? 0 : col0;
     float* Cs = reinterpret_cast<float*>(ssum + (size_t)L.maxterm * 106) + tid;
-    {
-        const unsigned long long* P = L.part + ((size_t)b * L.nterm + tb) * 208;
-        for (int e = tid; e < nt * 104; e += NTB) {
-            const int tl = e / 104, r = e % 104;
-            ssum[tl * 106 + (r / 52) * 53 + r % 52] = P[(size_t)tl * 208 + r] + P[(size_t)tl * 208 + 104 + r];
-        }
-        constexpr int cstride = LEVEL == 2 ? NPF : (LEVEL == 1 ? NPR : NPT);
-        const float* Cb = (LEVEL >= 1 ? D.C : D.C0) + ((size_t)b * L.ncolp + cbase) * cstride + k;
-        for (int x = 0; x < ncolt; ++x) Cs[x * NTB] = live ? Cb[(size_t)x * cstride] : 0.0f;
-    }
+    bwd_stage<LEVEL, NTB>(D, L, ssum, Cs, b, k, live, nt, tb, cbase, ncolt);
     ev_stage(sev, L.ev + e0, e1 - e0);
     int c1 = 0, c2 = 0;
     float root0 = 0.0f, root1 = 0.0f;
@@ -380,14 +271,7 @@ __global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* 
         root0 = root_reach(D, ranges, LEVEL, b, tm, np, k, 0);
         root1 = root_reach(D, ranges, LEVEL, b, tm, np, k, 1);
     }
-    __syncthreads();
-    for (int e = tid; e < nt * 2; e += NTB) {
-        unsigned long long t = 0;
-        const unsigned long long* sp = ssum + (size_t)(e >> 1) * 106 + (e & 1) * 53;
-        for (int c = 0; c < 52; ++c) t += sp[c];
-        ssum[(size_t)(e >> 1) * 106 + (e & 1) * 53 + 52] = t >> 1;  // each combo counted on two cards
-    }
-    __syncthreads();
+    bwd_totals<NTB>(ssum, nt);
     vD rs0, rs1, op;
     vS sg, cv;
     float out0 = 0.0f, out1 = 0.0f;
@@ -494,25 +378,7 @@ __global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* 
             op[d - 1] += pp == 0 ? v1 : v0;
         }
     }
-    if (k >= np) return;
-    if (LEVEL == 2) {
-        if (D.rootval0) {
-            D.rootval0[((size_t)b * 2 + 0) * NPF + k] = live ? out0 : 0.0f;
-            D.rootval0[((size_t)b * 2 + 1) * NPF + k] = live ? out1 : 0.0f;
-        }
-    } else if (LEVEL == 1) {
-        if (D.two_level) {
-            float* rv = D.rootval + (((size_t)b * L.ntmpl + tm) * 2) * NPR + k;
-            rv[0] = live ? out0 : 0.0f;
-            rv[NPR] = live ? out1 : 0.0f;
-        } else {
-            D.rootval0[k] = live ? out0 : 0.0f;
-            D.rootval0[np + k] = live ? out1 : 0.0f;
-        }
-    } else {
-        D.rootval0[((size_t)b * 2 + 0) * NPT + k] = live ? out0 : 0.0f;
-        D.rootval0[((size_t)b * 2 + 1) * NPT + k] = live ? out1 : 0.0f;
-    }
+    bwd_root_out<LEVEL>(D, L, b, tm, k, np, live, out0, out1);
 }
 
 // Turn-leaf values: mean over the 44 rivers that leave the combo alive of
@@ -785,6 +651,7 @@ struct BatchedState {
     float* d_ranges = nullptr;  // root reach of the first street [boards][2][np]
     uint8_t* d_S8 = nullptr;    // [boards1][NPR][NPR] sign-blocker bytes
     int dm[2] = {8, 8};         // register-stack depth per level (4, 6 or 8)
+    JitSweeps jit;              // generated per-template sweeps (when enabled)
 };
 
 void batched_destroy(BatchedState* b) { delete b; }
@@ -940,14 +807,62 @@ int upload_level(DeviceArena& A, hipStream_t st, LevelHost& H, LevelDev& L, int 
     return MTT_OK;
 }
 
+// Street levels of a heads-up game for the batched path: river games (one
+// level, the single street) and turn+river games from a flop (the turn
+// template, then every river line as its own template).
+int hu_levels(const mtt_hu_game& g, const HuTree& tr, bool& two, LevelHost& H0, LevelHost& H1) {
+    two = g.n_board == 3 && g.root_chance == 1 && g.last_street == 2;
+    const bool one = g.n_board == 5;
+    MTT_ARG(two || one, "batched mode supports river games (5-card board) and turn+river games from a flop");
+    int rc = MTT_OK;
+    if (two) {
+        const Template& T0 = tr.templates[0];
+        rc = add_template(T0, H0, 0);
+        if (rc) return rc;
+        for (int l = 0; l < T0.n_leaf; ++l) {
+            rc = add_template(tr.templates[T0.leaf_next[l]], H1, 0);
+            if (rc) return rc;
+        }
+    } else {
+        rc = add_template(tr.templates[0], H1, 0);
+    }
+    return rc;
+}
+
+// The generated-sweep description of the levels (pointers into H0 / H1).
+std::vector<JitLevel> jit_levels(const LevelHost& H0, const LevelHost& H1, bool two) {
+    auto desc = [](const LevelHost& H, int level, int np) {
+        JitLevel J;
+        J.level = level;
+        J.np = np;
+        J.ntmpl = (int)H.tnode0.size() - 1;
+        J.nodes = &H.nodes;
+        J.tnode0 = &H.tnode0;
+        J.tev0 = &H.tev0;
+        J.events = &H.events;
+        J.slot0 = H.slot0;
+        J.slot0.push_back(H.nslot);
+        J.term0 = H.term0;
+        J.term0.push_back(H.nterm);
+        J.col0 = H.col0;
+        J.col0.push_back(H.ncol);
+        return J;
+    };
+    std::vector<JitLevel> lv;
+    if (two) lv.push_back(desc(H0, 0, NPT));
+    lv.push_back(desc(H1, 1, NPR));
+    return lv;
+}
+
 }  // namespace
 
 int batched_create(SolverImpl& s) {
     const mtt_hu_game& g = s.game;
     const HuTree& tr = s.tree;
-    bool two = g.n_board == 3 && g.root_chance == 1 && g.last_street == 2;
-    bool one = g.n_board == 5;
-    MTT_ARG(two || one, "batched mode supports river games (5-card board) and turn+river games from a flop");
+    LevelHost H0, H1;
+    bool two = false;
+    int rc = hu_levels(g, tr, two, H0, H1);
+    if (rc) return rc;
     auto st = new (std::nothrow) BatchedState;
     if (!st) { set_error("out of host memory"); return MTT_ERR_OOM; }
     s.batched = st;
@@ -957,24 +872,13 @@ int batched_create(SolverImpl& s) {
     B.D.two_level = two;
     B.D.pot0 = g.pot;
     DeviceArena& A = B.arena;
-    // ---------------------------------------------------------- levels --
-    LevelHost H0, H1;
-    int rc = MTT_OK;
+    // river template l is turn leaf l
     std::vector<int> leaf_tmpl, tmpl_leaf;
-    if (two) {
-        const Template& T0 = tr.templates[0];
-        rc = add_template(T0, H0, 0);
-        if (rc) return rc;
-        for (int l = 0; l < T0.n_leaf; ++l) {
-            rc = add_template(tr.templates[T0.leaf_next[l]], H1, 0);
-            if (rc) return rc;
+    if (two)
+        for (int l = 0; l < tr.templates[0].n_leaf; ++l) {
             leaf_tmpl.push_back(l);
             tmpl_leaf.push_back(l);
         }
-    } else {
-        rc = add_template(tr.templates[0], H1, 0);
-        if (rc) return rc;
-    }
     // ---------------------------------------------------------- boards --
     const int nb1 = two ? 49 * 48 : 1;
     B.nb1 = nb1;
@@ -1177,14 +1081,27 @@ int batched_create(SolverImpl& s) {
     }
     rc = A.upload(B.export_recs, &B.d_export);
     if (rc) return rc;
+    // ------------------------------------------------ generated sweeps --
+    if (bt_jit_enabled()) {
+        const std::vector<JitLevel> lv = jit_levels(H0, H1, two);
+        rc = bt_jit_build(lv.data(), (int)lv.size(), &B.jit);
+        if (rc) return rc;
+    }
     MTT_HIP(hipStreamSynchronize(s.stream));
+    return MTT_OK;
+}
+
+// One generated sweep over the grid (JIT_NT threads per workgroup).
+int jit_launch(hipFunction_t f, dim3 grid, size_t shmem, hipStream_t st, void** args) {
+    MTT_HIP(hipModuleLaunchKernel(f, grid.x, 1, 1, JIT_NT, 1, 1, (unsigned)shmem, st, args, nullptr));
     return MTT_OK;
 }
 
 int batched_run(SolverImpl& s, int iters) {
     BatchedState& B = *s.batched;
     const LevelDev& L1 = B.D.L[1];
-    for (int it = 0; it < iters; ++it) {
+    int rc = MTT_OK;
+    for (int it = 0; it < iters && rc == MTT_OK; ++it) {
         int t = s.iters_done + 1;
         float wp, wn, ws;
         iteration_weights(s.cfg, t, wp, wn, ws);
@@ -1193,15 +1110,19 @@ int batched_run(SolverImpl& s, int iters) {
             const LevelDev& L0 = B.D.L[0];
             const size_t sh = (size_t)L0.maxterm * 104 * 8;
             const dim3 g(49 * L0.ntmpl * 2);
+            void* args[] = {&B.D, &ranges, &ws};
             MTT_TIMED(s, "turn_forward",
-                      if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_forward<0, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
+                      if (B.jit.active) rc = jit_launch(B.jit.fwd[0], g, sh, s.stream, args);
+                      else if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_forward<0, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
                       else hipLaunchKernelGGL((k_bt_forward<0, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
         {
             const size_t sh = (size_t)L1.maxterm * 104 * 8;
             const dim3 g(B.nb1 * L1.ntmpl * 2);
+            void* args[] = {&B.D, &ranges, &ws};
             MTT_TIMED(s, "river_forward",
-                      if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_forward<1, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
+                      if (B.jit.active) rc = jit_launch(B.jit.fwd[1], g, sh, s.stream, args);
+                      else if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_forward<1, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
                       else hipLaunchKernelGGL((k_bt_forward<1, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
         ::mtt::KernelTimer::Mark gm;
@@ -1225,8 +1146,10 @@ int batched_run(SolverImpl& s, int iters) {
         {
             const size_t sh = bwd_lds(L1);
             const dim3 g(B.nb1 * L1.ntmpl * BWD_NH);
+            void* args[] = {&B.D, &ranges, &wp, &wn};
             MTT_TIMED(s, "river_backward",
-                      if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_backward<1, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
+                      if (B.jit.active) rc = jit_launch(B.jit.bwd[1], g, sh, s.stream, args);
+                      else if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_backward<1, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
                       else hipLaunchKernelGGL((k_bt_backward<1, 8, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr));
         }
         if (B.two_level) {
@@ -1238,14 +1161,17 @@ int batched_run(SolverImpl& s, int iters) {
                                          L0.ncol));
             const size_t sh = bwd_lds(L0);
             const dim3 g(49 * L0.ntmpl * BWD_NH);
+            void* args[] = {&B.D, &ranges, &wp, &wn};
             MTT_TIMED(s, "turn_backward",
-                      if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_backward<0, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
+                      if (B.jit.active) rc = jit_launch(B.jit.bwd[0], g, sh, s.stream, args);
+                      else if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_backward<0, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
                       else hipLaunchKernelGGL((k_bt_backward<0, 8, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr));
         }
+        if (rc) return rc;
         MTT_HIP(hipGetLastError());
         s.iters_done = t;
     }
-    return MTT_OK;
+    return rc;
 }
 
 int batched_export(SolverImpl& s, float* d_out, int which) {
@@ -1261,6 +1187,27 @@ int batched_export(SolverImpl& s, float* d_out, int which) {
 }
 
 int batched_strategy_sums(SolverImpl& s, float* d_S_ref) { return batched_export(s, d_S_ref, 0); }
+
+int batched_jit_info(const SolverImpl& s, int* active, double* compile_s) {
+    const bool on = s.batched && s.batched->jit.active;
+    if (active) *active = on ? 1 : 0;
+    if (compile_s) *compile_s = on ? s.batched->jit.compile_s : 0.0;
+    return MTT_OK;
+}
+
+// Generated sweep source of a game's street templates (no device needed).
+int hu_sweep_source(const mtt_hu_game& g, std::string* src) {
+    HuTree tr;
+    int rc = build_hu_tree(g, tr, false);
+    if (rc) return rc;
+    LevelHost H0, H1;
+    bool two = false;
+    rc = hu_levels(g, tr, two, H0, H1);
+    if (rc) return rc;
+    const std::vector<JitLevel> lv = jit_levels(H0, H1, two);
+    *src = bt_jit_source(lv.data(), (int)lv.size());
+    return MTT_OK;
+}
 
 
 // ===================================================== flop instances ===
@@ -1791,6 +1738,36 @@ struct mtt_batch {
 };
 
 extern "C" {
+
+int mtt_hu_sweep_source(const mtt_hu_game* game, char* buf, size_t cap, size_t* len) {
+    using namespace mtt;
+    MTT_ARG(game, "mtt_hu_sweep_source: null game");
+    std::string src;
+    int rc = hu_sweep_source(*game, &src);
+    if (rc) return rc;
+    if (len) *len = src.size();
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, src.size());
+        memcpy(buf, src.data(), n);
+        buf[n] = 0;
+    }
+    return MTT_OK;
+}
+
+int mtt_hu_sweep_compile(const mtt_hu_game* game, size_t* code_bytes, double* seconds) {
+    using namespace mtt;
+    MTT_ARG(game, "mtt_hu_sweep_compile: null game");
+    std::string src;
+    int rc = hu_sweep_source(*game, &src);
+    if (rc) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<char> code;
+    rc = bt_jit_compile(src, &code);
+    if (rc) return rc;
+    if (code_bytes) *code_bytes = code.size();
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return MTT_OK;
+}
 
 int mtt_batch_create(const mtt_hu_game* games, int n, const float* ranges, const mtt_solver_cfg* cfg, mtt_batch** out) {
     using namespace mtt;

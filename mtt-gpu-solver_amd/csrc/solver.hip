@@ -470,6 +470,11 @@ int mtt_solver_info(mtt_solver* h, int64_t* info) {
     return MTT_OK;
 }
 
+int mtt_solver_jit_info(mtt_solver* h, int* active, double* compile_seconds) {
+    MTT_ARG(h, "mtt_solver_jit_info: null handle");
+    return mtt::batched_jit_info(h->impl, active, compile_seconds);
+}
+
 int mtt_solver_destroy(mtt_solver* h) {
     delete h;
     return MTT_OK;

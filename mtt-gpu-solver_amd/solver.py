@@ -53,6 +53,9 @@ for name, args in {
     "mtt_solver_get_strategy": [_P, _P, ctypes.c_size_t],
     "mtt_solver_exploitability_fp64": [_P, _P],
     "mtt_solver_info": [_P, _P],
+    "mtt_solver_jit_info": [_P, _P, _P],
+    "mtt_hu_sweep_source": [_P, _P, ctypes.c_size_t, _P],
+    "mtt_hu_sweep_compile": [_P, _P, _P],
     "mtt_solver_get_state": [_P, _P, _P, ctypes.c_size_t],
     "mtt_solver_timing": [_P, _I],
     "mtt_solver_kernel_times": [_P, _I, ctypes.c_char_p, _I, _P, _P],
@@ -137,6 +140,22 @@ def tree(game):
     return arr, ni.value, nm.value
 
 
+def sweep_source(game):
+    """Generated sweep source of a game's street templates (batched path)."""
+    n = ctypes.c_size_t()
+    mtt.call("mtt_hu_sweep_source", ctypes.byref(game), None, 0, ctypes.byref(n))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    mtt.call("mtt_hu_sweep_source", ctypes.byref(game), buf, n.value + 1, ctypes.byref(n))
+    return buf.value.decode()
+
+
+def sweep_compile(game):
+    """hipRTC compile-only build of the generated sweeps: (code bytes, seconds)."""
+    n, t = ctypes.c_size_t(), ctypes.c_double()
+    mtt.call("mtt_hu_sweep_compile", ctypes.byref(game), ctypes.byref(n), ctypes.byref(t))
+    return n.value, t.value
+
+
 def alive_index(board):
     """Root-alive combo list (lexicographic) and its 1326 global indices."""
     bs = set(int(c) for c in board)
@@ -161,6 +180,12 @@ class Solver:
 
     def run(self, iters):
         mtt.call("mtt_solver_run", self.h, iters)
+
+    def jit_info(self):
+        """(generated sweeps active, hipRTC compile + load seconds)."""
+        a, t = ctypes.c_int(), ctypes.c_double()
+        mtt.call("mtt_solver_jit_info", self.h, ctypes.byref(a), ctypes.byref(t))
+        return bool(a.value), t.value
 
     @property
     def iterations(self):

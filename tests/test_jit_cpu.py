@@ -1,0 +1,43 @@
+"""Generated per-template sweeps of the batched path: the source emitted for a
+game's street templates, and a hipRTC compile of it for gfx950 (no device
+needed)."""
+import re
+
+import pytest
+
+from conftest import PKG, _load
+import os
+
+
+@pytest.fixture(scope="module")
+def S(mtt):
+    return _load("mtt_solver", os.path.join(PKG, "solver.py"))
+
+
+def test_sweep_source_has_one_case_per_template(S):
+    g, _, _ = S.config3(0)
+    src = S.sweep_source(g)
+    nodes, ni, _ = S.tree(g)
+    # the turn template, then one template per river line (one per turn leaf)
+    n_river = len(re.findall(r"^case \d+: \{", src.split("mtt_jit_fwd1")[1].split("mtt_jit_bwd1")[0], re.M))
+    n_turn = len(re.findall(r"^case \d+: \{", src.split("mtt_jit_fwd0")[1].split("mtt_jit_bwd0")[0], re.M))
+    assert n_turn == 1
+    assert n_river >= 2
+    for k in ("mtt_jit_fwd0", "mtt_jit_bwd0", "mtt_jit_fwd1", "mtt_jit_bwd1"):
+        assert f"void {k}(" in src
+    # every regret slot of the river level is loaded exactly once per template in each sweep
+    fwd1 = src.split("void mtt_jit_fwd1(")[1].split("void mtt_jit_bwd1(")[0]
+    assert fwd1.count("unsafeAtomicAdd(") > 0
+    assert "#pragma once" not in src
+
+
+def test_sweep_source_deterministic(S):
+    g, _, _ = S.config1(0)
+    assert S.sweep_source(g) == S.sweep_source(g)
+
+
+@pytest.mark.parametrize("which", ["config1", "config3"])
+def test_sweeps_compile_for_gfx950(S, which):
+    g, _, _ = getattr(S, which)(0)
+    nbytes, secs = S.sweep_compile(g)
+    assert nbytes > 1000
