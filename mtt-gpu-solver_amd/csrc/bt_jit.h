@@ -30,24 +30,39 @@ struct JitLevel {
     std::vector<int> slot0, term0, col0;            // [ntmpl+1] level-global firsts
 };
 
+// Launch shapes of the generated sweeps.  The forward always runs two
+// workgroups of 576 per board (as the interpreted kernels); the backward runs
+// ceil(1152 / bwd_nt) workgroups of bwd_nt threads (576 or 384).  *_waves > 0
+// asks the compiler to fit that many waves per SIMD (amdgpu_waves_per_eu).
+struct JitConfig {
+    int bwd_nt = 576;
+    int bwd_waves = 0;
+    int fwd_waves = 0;
+};
+
 struct JitSweeps {
     bool active = false;
     double compile_s = 0.0;                  // hipRTC compile + load (0 when cached)
+    JitConfig cfg;
     hipFunction_t fwd[2] = {nullptr, nullptr};
     hipFunction_t bwd[2] = {nullptr, nullptr};
 };
 
-// Threads per workgroup of the generated sweeps (as the interpreted ones).
+// Threads per workgroup of the generated forward sweeps.
 constexpr int JIT_NT = 576;
 
 // Generates, compiles and loads the sweeps of the given levels for the
 // current device.  Compiled modules are cached per (device, source) for the
 // life of the process.  Returns MTT_OK or an error code with
 // mtt_last_error() set.
-int bt_jit_build(const JitLevel* levels, int n, JitSweeps* out);
+int bt_jit_build(const JitLevel* levels, int n, const JitConfig& cfg, JitSweeps* out);
 
 // The generated source for the given levels (for inspection and tests).
-std::string bt_jit_source(const JitLevel* levels, int n);
+std::string bt_jit_source(const JitLevel* levels, int n, const JitConfig& cfg);
+
+// The default launch shapes, with MTT_JIT_BWD_NT / MTT_JIT_BWD_WAVES /
+// MTT_JIT_FWD_WAVES overrides from the environment (invalid values ignored).
+JitConfig bt_jit_config();
 
 // hipRTC compile of a generated source for gfx950 (no device needed).
 int bt_jit_compile(const std::string& src, std::vector<char>* code);

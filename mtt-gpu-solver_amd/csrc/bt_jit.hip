@@ -275,11 +275,15 @@ void gen_backward_case(Src& o, const JitLevel& J, int t) {
     o("}");
 }
 
-void gen_level(Src& o, const JitLevel& J) {
+std::string waves_attr(int w) {
+    return w > 0 ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(w) + ")))" : std::string();
+}
+
+void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
     const int L = J.level;
     // ---- forward
-    o("extern \"C\" __global__ __launch_bounds__(%d) void mtt_jit_fwd%d(mtt::bt::BatchedDev D, "
-      "const float* __restrict__ ranges, float ws) {", JIT_NT, L);
+    o("extern \"C\" __global__ __launch_bounds__(%d)%s void mtt_jit_fwd%d(mtt::bt::BatchedDev D, "
+      "const float* __restrict__ ranges, float ws) {", JIT_NT, waves_attr(cfg.fwd_waves).c_str(), L);
     o("using namespace mtt::bt;");
     o("constexpr int LEVEL = %d, NT = %d, NP = %d;", L, JIT_NT, J.np);
     o("const LevelDev& L = D.L[LEVEL];");
@@ -304,10 +308,10 @@ void gen_level(Src& o, const JitLevel& J) {
     o("}");
     o("}");
     // ---- backward
-    o("extern \"C\" __global__ __launch_bounds__(%d) void mtt_jit_bwd%d(mtt::bt::BatchedDev D, "
-      "const float* __restrict__ ranges, float wp, float wn) {", JIT_NT, L);
+    o("extern \"C\" __global__ __launch_bounds__(%d)%s void mtt_jit_bwd%d(mtt::bt::BatchedDev D, "
+      "const float* __restrict__ ranges, float wp, float wn) {", cfg.bwd_nt, waves_attr(cfg.bwd_waves).c_str(), L);
     o("using namespace mtt::bt;");
-    o("constexpr int LEVEL = %d, NTB = %d, NP = %d;", L, JIT_NT, J.np);
+    o("constexpr int LEVEL = %d, NTB = %d, NP = %d;", L, cfg.bwd_nt, J.np);
     o("constexpr int NH = (NPT + NTB - 1) / NTB;");
     o("const LevelDev& L = D.L[LEVEL];");
     o("extern __shared__ unsigned long long ssum[];");
@@ -351,14 +355,30 @@ bool bt_jit_enabled() {
     return !(e && e[0] == '0' && e[1] == 0);
 }
 
-std::string bt_jit_source(const JitLevel* levels, int n) {
+JitConfig bt_jit_config() {
+    JitConfig c;
+    auto env = [](const char* k, int lo, int hi, int def) {
+        const char* e = getenv(k);
+        if (!e || !*e) return def;
+        char* end = nullptr;
+        const long v = strtol(e, &end, 10);
+        return (end && *end == 0 && v >= lo && v <= hi) ? (int)v : def;
+    };
+    c.bwd_nt = env("MTT_JIT_BWD_NT", 384, 576, c.bwd_nt);
+    if (c.bwd_nt != 384 && c.bwd_nt != 576) c.bwd_nt = 576;
+    c.bwd_waves = env("MTT_JIT_BWD_WAVES", 0, 8, c.bwd_waves);
+    c.fwd_waves = env("MTT_JIT_FWD_WAVES", 0, 8, c.fwd_waves);
+    return c;
+}
+
+std::string bt_jit_source(const JitLevel* levels, int n, const JitConfig& cfg) {
     Src o;
     o("// generated per-template sweeps");
     o("%s", "");
     std::string s = kBtDevSrc;
     const size_t po = s.find("#pragma once");  // the header is the main file here
     if (po != std::string::npos) s.erase(po, 12);
-    for (int i = 0; i < n; ++i) gen_level(o, levels[i]);
+    for (int i = 0; i < n; ++i) gen_level(o, levels[i], cfg);
     return s + "\n" + o.s;
 }
 
@@ -406,11 +426,13 @@ int bt_jit_compile(const std::string& src, std::vector<char>* code) {
     return MTT_OK;
 }
 
-int bt_jit_build(const JitLevel* levels, int n, JitSweeps* out) {
+int bt_jit_build(const JitLevel* levels, int n, const JitConfig& cfg, JitSweeps* out) {
     *out = JitSweeps();
     int rc = check_levels(levels, n);
     if (rc) return rc;
-    const std::string src = bt_jit_source(levels, n);
+    MTT_ARG(cfg.bwd_nt == 384 || cfg.bwd_nt == 576, "generated sweeps: backward workgroup %d not 384 or 576",
+            cfg.bwd_nt);
+    const std::string src = bt_jit_source(levels, n, cfg);
     int dev = 0;
     MTT_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_mu);
@@ -438,6 +460,7 @@ int bt_jit_build(const JitLevel* levels, int n, JitSweeps* out) {
         out->fwd[L] = it->second.fwd[L];
         out->bwd[L] = it->second.bwd[L];
     }
+    out->cfg = cfg;
     out->active = true;
     return MTT_OK;
 }

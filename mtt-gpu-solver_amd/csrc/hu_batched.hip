@@ -1084,16 +1084,16 @@ int batched_create(SolverImpl& s) {
     // ------------------------------------------------ generated sweeps --
     if (bt_jit_enabled()) {
         const std::vector<JitLevel> lv = jit_levels(H0, H1, two);
-        rc = bt_jit_build(lv.data(), (int)lv.size(), &B.jit);
+        rc = bt_jit_build(lv.data(), (int)lv.size(), bt_jit_config(), &B.jit);
         if (rc) return rc;
     }
     MTT_HIP(hipStreamSynchronize(s.stream));
     return MTT_OK;
 }
 
-// One generated sweep over the grid (JIT_NT threads per workgroup).
-int jit_launch(hipFunction_t f, dim3 grid, size_t shmem, hipStream_t st, void** args) {
-    MTT_HIP(hipModuleLaunchKernel(f, grid.x, 1, 1, JIT_NT, 1, 1, (unsigned)shmem, st, args, nullptr));
+// One generated sweep over the grid.
+int jit_launch(hipFunction_t f, dim3 grid, int nt, size_t shmem, hipStream_t st, void** args) {
+    MTT_HIP(hipModuleLaunchKernel(f, grid.x, 1, 1, nt, 1, 1, (unsigned)shmem, st, args, nullptr));
     return MTT_OK;
 }
 
@@ -1112,7 +1112,7 @@ int batched_run(SolverImpl& s, int iters) {
             const dim3 g(49 * L0.ntmpl * 2);
             void* args[] = {&B.D, &ranges, &ws};
             MTT_TIMED(s, "turn_forward",
-                      if (B.jit.active) rc = jit_launch(B.jit.fwd[0], g, sh, s.stream, args);
+                      if (B.jit.active) rc = jit_launch(B.jit.fwd[0], g, JIT_NT, sh, s.stream, args);
                       else if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_forward<0, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
                       else hipLaunchKernelGGL((k_bt_forward<0, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
@@ -1121,7 +1121,7 @@ int batched_run(SolverImpl& s, int iters) {
             const dim3 g(B.nb1 * L1.ntmpl * 2);
             void* args[] = {&B.D, &ranges, &ws};
             MTT_TIMED(s, "river_forward",
-                      if (B.jit.active) rc = jit_launch(B.jit.fwd[1], g, sh, s.stream, args);
+                      if (B.jit.active) rc = jit_launch(B.jit.fwd[1], g, JIT_NT, sh, s.stream, args);
                       else if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_forward<1, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
                       else hipLaunchKernelGGL((k_bt_forward<1, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
@@ -1144,11 +1144,12 @@ int batched_run(SolverImpl& s, int iters) {
         }
         if (s.timer.enabled) s.timer.end(gm, s.stream);
         {
-            const size_t sh = bwd_lds(L1);
-            const dim3 g(B.nb1 * L1.ntmpl * BWD_NH);
+            const int jnt = B.jit.cfg.bwd_nt, jnh = (NPT + jnt - 1) / jnt;
+            const size_t sh = B.jit.active ? bwd_lds(L1, jnt) : bwd_lds(L1);
+            const dim3 g(B.nb1 * L1.ntmpl * (B.jit.active ? jnh : BWD_NH));
             void* args[] = {&B.D, &ranges, &wp, &wn};
             MTT_TIMED(s, "river_backward",
-                      if (B.jit.active) rc = jit_launch(B.jit.bwd[1], g, sh, s.stream, args);
+                      if (B.jit.active) rc = jit_launch(B.jit.bwd[1], g, jnt, sh, s.stream, args);
                       else if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_backward<1, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
                       else hipLaunchKernelGGL((k_bt_backward<1, 8, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr));
         }
@@ -1159,11 +1160,12 @@ int batched_run(SolverImpl& s, int iters) {
             MTT_TIMED(s, "turn_showdown",
                       hipLaunchKernelGGL(k_bt_turn_sd<16>, dim3(NPT / 256 + 1, 49), dim3(256), 0, s.stream, B.D,
                                          L0.ncol));
-            const size_t sh = bwd_lds(L0);
-            const dim3 g(49 * L0.ntmpl * BWD_NH);
+            const int jnt = B.jit.cfg.bwd_nt, jnh = (NPT + jnt - 1) / jnt;
+            const size_t sh = B.jit.active ? bwd_lds(L0, jnt) : bwd_lds(L0);
+            const dim3 g(49 * L0.ntmpl * (B.jit.active ? jnh : BWD_NH));
             void* args[] = {&B.D, &ranges, &wp, &wn};
             MTT_TIMED(s, "turn_backward",
-                      if (B.jit.active) rc = jit_launch(B.jit.bwd[0], g, sh, s.stream, args);
+                      if (B.jit.active) rc = jit_launch(B.jit.bwd[0], g, jnt, sh, s.stream, args);
                       else if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_backward<0, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
                       else hipLaunchKernelGGL((k_bt_backward<0, 8, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr));
         }
@@ -1205,7 +1207,7 @@ int hu_sweep_source(const mtt_hu_game& g, std::string* src) {
     rc = hu_levels(g, tr, two, H0, H1);
     if (rc) return rc;
     const std::vector<JitLevel> lv = jit_levels(H0, H1, two);
-    *src = bt_jit_source(lv.data(), (int)lv.size());
+    *src = bt_jit_source(lv.data(), (int)lv.size(), bt_jit_config());
     return MTT_OK;
 }
 

@@ -205,3 +205,75 @@ def test_batched_multistreet_matches_oracle(solver_api, oracle, torch_dev):
     the all-in-only turn+river game."""
     print("config3-small batched bf16-split, seed 1", flush=True)
     _multistreet_check(solver_api, oracle, solver_api.MODE_BF16_SPLIT, STATE_REL_BF16)
+
+
+@pytest.mark.parametrize("which", ["config1", "config3_small", "config3"])
+def test_generated_sweeps_match_interpreted(solver_api, torch_dev, which, monkeypatch):
+    """The per-template sweeps generated at solver creation and the
+    interpreted sweeps give the same state after 5 iterations."""
+    S = solver_api
+    g, r0, r1 = getattr(S, which)(0 if which != "config3_small" else 1)
+    print(f"{which}: generated vs interpreted sweeps", flush=True)
+    a = S.Solver(g, r0, r1, S.MODE_BF16_SPLIT)
+    assert a.jit_info()[0]
+    monkeypatch.setenv("MTT_JIT", "0")
+    b = S.Solver(g, r0, r1, S.MODE_BF16_SPLIT)
+    monkeypatch.delenv("MTT_JIT")
+    assert not b.jit_info()[0]
+    a.run(5)
+    b.run(5)
+    Ra, Sa = a.state()
+    Rb, Sb = b.state()
+    dr = np.abs(Ra - Rb).max() / max(np.abs(Rb).max(), 1e-30)
+    ds = np.abs(Sa - Sb).max() / max(np.abs(Sb).max(), 1e-30)
+    print(f"{which}: max rel diff R {dr:.3g} S {ds:.3g}", flush=True)
+    assert dr <= 1e-5 and ds <= 1e-5
+    a.close()
+    b.close()
+
+
+def test_batched_config3_matches_oracle(solver_api, oracle, torch_dev):
+    """The benchmark's turn+river game (33%/75%/2.5x/all-in per street, nine
+    river lines) on the batched path with generated sweeps, against the FP32
+    oracle.  Iteration 1: every regret and strategy sum within tolerance.
+    Iteration 2: regret matching plays uniform when the positive regrets sum
+    to at most 1e-7, and entries whose iteration-1 sum is rounding noise
+    around that threshold (value ties) can land on opposite sides in two
+    implementations that sum in different orders; such a flip changes that
+    combo's regrets by O(pot) there and at its ancestors (and its strategy
+    sum there).  So at most 1e-7 of the entries may differ, and each must
+    belong to a combo with an iteration-1 positive-regret sum in (0, 1e-6] in
+    either implementation."""
+    g, r0, r1 = solver_api.config3(0)
+    print("config3 batched bf16-split vs oracle, 2 iterations", flush=True)
+    s = solver_api.Solver(g, r0, r1, solver_api.MODE_BF16_SPLIT)
+    assert s.jit_info()[0]
+    h32, nodes, ni = _setup(solver_api, oracle, g, r0, r1, 1)
+    s.run(1)
+    h32.iterate(1)
+    Rg, Sg = s.state()
+    Rc, Sc = h32.state()
+    assert np.abs(Rg - Rc).max() <= STATE_REL_BF16 * np.abs(Rc).max()
+    assert np.abs(Sg - Sc).max() <= STATE_REL_BF16 * np.abs(Sc).max()
+    # combos with a positive-regret sum near the uniform threshold somewhere
+    amb = np.zeros(1216, bool)
+    for R in (Rg, Rc):
+        pos = np.maximum(R.reshape(ni, 4, 1216), 0).sum(axis=1)
+        amb |= ((pos > 0) & (pos <= 1e-6)).any(axis=0)
+    del Rg, Sg, Rc, Sc
+    s.run(1)
+    h32.iterate(1)
+    Rg, Sg = s.state()
+    Rc, Sc = h32.state()
+    tol = STATE_REL_BF16 * np.abs(Rc).max()
+    bad = np.argwhere(np.abs(Rg - Rc).reshape(ni, 4, 1216) > tol)
+    print(f"iteration 2: {len(bad)} of {Rc.size} regrets beyond tolerance; "
+          f"{int(amb.sum())} combos with near-threshold sums", flush=True)
+    assert len(bad) <= 1e-7 * Rc.size
+    assert amb[bad[:, 2]].all()
+    del Rg, Rc
+    bad = np.argwhere(np.abs(Sg - Sc).reshape(ni, 4, 1216) > STATE_REL_BF16 * np.abs(Sc).max())
+    print(f"iteration 2: {len(bad)} strategy sums beyond tolerance", flush=True)
+    assert len(bad) <= 1e-7 * Sc.size
+    assert amb[bad[:, 2]].all()
+    s.close()
