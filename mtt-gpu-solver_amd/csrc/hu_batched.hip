@@ -441,57 +441,74 @@ __global__ void k_bt_s8(BatchedDev D, uint8_t* __restrict__ S8) {
     }
 }
 
-constexpr int GHB = (NPR + 127) / 128;   // hero blocks of 128 per board
 constexpr int GNCH = NPR / 32;           // K chunks of 32 villains
 
 // River showdown GEMM: C[b][col][i] = 2 * sum_j (S/2)[i][j] * (Bh + Bl)[b][col][j].
-// One workgroup = 128 heroes x all columns of one board; wave w owns the
-// 32-column tile w (NW = column tiles) and all four 32-hero tiles.  S chunks
-// (128 heroes x 32 villains) are widened to bf16 once per workgroup into LDS
-// (double-buffered); each wave streams its own reach columns (hi and lo) from
-// HBM straight into MFMA A-fragments, one chunk ahead.  Within a chunk, lane
-// half h of a fragment covers villains 16h..16h+15 (8 per MFMA k-step), a
-// permutation of K shared by both operands.  Workgroups are numbered so that
-// the hero blocks of one board run on one XCD (dispatch is round-robin over
-// the 8 XCDs), where they share that board's reach columns in L2.
-template <int NW>
-__global__ __launch_bounds__(NW * 64) void k_bt_gemm3(BatchedDev D, const uint8_t* __restrict__ S8, int nlogical) {
+// One workgroup = 4 32-hero tiles x NW 32-column tiles of one board (the
+// board's column tiles are split into ncg groups of NW); wave w owns column
+// tile cg * NW + w and all four hero tiles (waves past the board's tiles only
+// help stage S).  The K loop runs over chunks of 32 villains: S chunks
+// (128 heroes x 32 villains, bytes of bf16(S/2)) are widened to bf16 once per
+// workgroup into LDS (double-buffered, one barrier per chunk); each wave
+// streams its own reach columns (hi and lo) straight into MFMA A-fragments.
+// Both streams are fetched PD chunks ahead into register rings, so the HBM
+// latency of a chunk is covered by PD chunks of MFMA work.  Within a chunk,
+// lane half h of a fragment covers villains 16h..16h+15 (8 per MFMA k-step),
+// a permutation of K shared by both operands.  Workgroups are numbered so
+// that all workgroups of one board run on one XCD (dispatch is round-robin
+// over the 8 XCDs), where they share that board's reach and S bytes in L2.
+constexpr int GHT = 4;                                       // hero tiles per workgroup
+constexpr int GHB = (NPR + 32 * GHT - 1) / (32 * GHT);       // hero blocks per board
+
+template <int NW, int PD>
+__global__ __launch_bounds__(NW * 64) void k_bt_gemm5(BatchedDev D, const uint8_t* __restrict__ S8, int nlogical,
+                                                      int ncg) {
     constexpr int NT = NW * 64;
-    constexpr int SPT = (256 + NT - 1) / NT;   // 16-byte S pieces staged per thread
-    __shared__ uint4 Ssh[2][4][2][2][32];      // [buf][hero tile][h][k-step][hero in tile]
+    constexpr int PIECES = GHT * 64;                 // 16-byte S pieces of one chunk
+    constexpr int SPT = (PIECES + NT - 1) / NT;      // pieces staged per thread
+    __shared__ uint4 Ssh[2][GHT][2][2][32];          // [buf][hero tile][h][k-step][hero in tile]
     const int per = (nlogical + 7) >> 3;
     const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (logical >= nlogical) return;
-    const int b = logical / GHB, i0 = (logical % GHB) * 128;
-    const int nht = min(4, (NPR - i0) >> 5);
+    const int b = logical / (GHB * ncg), rem = logical % (GHB * ncg);
+    const int hb = rem / ncg, cg = rem % ncg;
+    const int i0 = hb * 32 * GHT;
+    const int nht = min(GHT, (NPR - i0) >> 5);
     const int npieces = nht * 64;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, half = lane >> 5, l32 = lane & 31;
-    const int ncolp = NW * 32;
-    const int col = wave * 32 + l32;
+    const int ncolp = D.L[1].ncolp;
+    const int tile = cg * NW + wave;
+    const bool on = tile * 32 < ncolp;
+    const int col = (on ? tile : 0) * 32 + l32;
     const uint4* ah_p = reinterpret_cast<const uint4*>(D.Bh + ((size_t)b * ncolp + col) * NPR + 16 * half);
     const uint4* al_p = reinterpret_cast<const uint4*>(D.Bl + ((size_t)b * ncolp + col) * NPR + 16 * half);
     const uint4* s_p = reinterpret_cast<const uint4*>(S8 + (size_t)b * NPR * NPR + (size_t)i0 * 32);
     constexpr int CH4 = NPR * 2;               // uint4 pieces per S chunk (all heroes)
-    f32x16_t acc[4];
+    f32x16_t acc[GHT];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < GHT; ++t)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
-    uint4 sv[SPT];
-    auto load_s = [&](int c) {
+    uint4 sv[PD][SPT], av[PD][4];   // rings: S pieces; reach hi s0, hi s1, lo s0, lo s1
+    auto load = [&](int c, uint4 (&sr)[SPT], uint4 (&ar)[4]) {
 #pragma unroll
         for (int q = 0; q < SPT; ++q) {
             const int e = threadIdx.x + NT * q;
-            if (e < npieces) sv[q] = s_p[(size_t)c * CH4 + e];
+            if (e < npieces) sr[q] = s_p[(size_t)c * CH4 + e];
         }
+        const int o = c * 4;        // 32 villains = 4 uint4 of bf16 per row; this half's 16 at +0, +1
+        ar[0] = ah_p[o];
+        ar[1] = ah_p[o + 1];
+        ar[2] = al_p[o];
+        ar[3] = al_p[o + 1];
     };
-    auto store_s = [&](int buf) {
+    auto store_s = [&](int buf, const uint4 (&sr)[SPT]) {
 #pragma unroll
         for (int q = 0; q < SPT; ++q) {
             const int e = threadIdx.x + NT * q;
             if (e < npieces) {
                 const int hl = e >> 1, h = e & 1;
-                const uint4 w = sv[q];
+                const uint4 w = sr[q];
                 Ssh[buf][hl >> 5][h][0][hl & 31] =
                     make_uint4(__builtin_amdgcn_perm(0u, w.x, 0x010C000Cu), __builtin_amdgcn_perm(0u, w.x, 0x030C020Cu),
                                __builtin_amdgcn_perm(0u, w.y, 0x010C000Cu), __builtin_amdgcn_perm(0u, w.y, 0x030C020Cu));
@@ -501,48 +518,208 @@ __global__ __launch_bounds__(NW * 64) void k_bt_gemm3(BatchedDev D, const uint8_
             }
         }
     };
-    uint4 a_cur[4], a_nxt[4];   // hi s0, hi s1, lo s0, lo s1
-    load_s(0);
-    a_cur[0] = ah_p[0]; a_cur[1] = ah_p[1]; a_cur[2] = al_p[0]; a_cur[3] = al_p[1];
-    store_s(0);
+    // ring slot x % PD holds chunk x; at step c the slots hold chunks c .. c + PD - 1, so chunk c + 1 is
+    // already there when step c stages it (PD >= 2)
+    static_assert(PD >= 2 && GNCH >= PD, "prefetch depth");
+#pragma unroll
+    for (int p = 0; p < PD; ++p) load(p, sv[p], av[p]);
+    store_s(0, sv[0]);
     __syncthreads();
-    int buf = 0;
-    for (int c = 0; c < GNCH; ++c) {
-        const bool more = c + 1 < GNCH;
-        if (more) {
-            load_s(c + 1);
-            const int o = (c + 1) * 4;   // 32 villains = 4 uint4 of bf16 per row; this half's 16 at +0, +1
-            a_nxt[0] = ah_p[o]; a_nxt[1] = ah_p[o + 1]; a_nxt[2] = al_p[o]; a_nxt[3] = al_p[o + 1];
-        }
+    for (int c0 = 0; c0 < GNCH; c0 += PD) {
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            bf16x8_t ah, al;
-            __builtin_memcpy(&ah, &a_cur[ks], 16);
-            __builtin_memcpy(&al, &a_cur[2 + ks], 16);
+        for (int p = 0; p < PD; ++p) {
+            const int c = c0 + p;
+            if (c < GNCH) {
+                // chunk c + 1 into the other LDS buffer (its readers passed the last barrier)
+                if (c + 1 < GNCH) store_s((c + 1) & 1, sv[(p + 1) % PD]);
+                if (on) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                if (t < nht) {
-                    bf16x8_t bop;
-                    __builtin_memcpy(&bop, &Ssh[buf][t][half][ks][l32], 16);
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bop, acc[t], 0, 0, 0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bop, acc[t], 0, 0, 0);
+                    for (int ks = 0; ks < 2; ++ks) {
+                        bf16x8_t ah, al;
+                        __builtin_memcpy(&ah, &av[p][ks], 16);
+                        __builtin_memcpy(&al, &av[p][2 + ks], 16);
+#pragma unroll
+                        for (int t = 0; t < GHT; ++t) {
+                            if (t < nht) {
+                                bf16x8_t bop;
+                                __builtin_memcpy(&bop, &Ssh[c & 1][t][half][ks][l32], 16);
+                                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bop, acc[t], 0, 0, 0);
+                                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bop, acc[t], 0, 0, 0);
+                            }
+                        }
+                    }
                 }
+                // ring slot p is free again: fetch chunk c + PD into it
+                if (c + PD < GNCH) load(c + PD, sv[p], av[p]);
+                __syncthreads();
             }
         }
-        if (more) store_s(buf ^ 1);
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) a_cur[q] = a_nxt[q];
-        buf ^= 1;
     }
+    if (!on) return;
     float* Cb = D.C + (size_t)b * ncolp * NPR;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < GHT; ++t) {
         if (t < nht) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int cc = wave * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+                const int cc = tile * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
                 Cb[(size_t)cc * NPR + i0 + 32 * t + l32] = 2.0f * acc[t][r];
+            }
+        }
+    }
+}
+
+// River showdown GEMM, hero-major form: one workgroup of 8 waves = 256
+// heroes (8 tiles) x all of the board's column tiles.  Wave w owns hero
+// tiles 2(w%4), 2(w%4)+1 and the first (w < 4) or second half of the column
+// tiles; waves w and w+4 share a SIMD, so every SIMD does the same work.
+// Per K chunk of 32 villains, the reach hi/lo of all columns is staged once
+// into LDS in MFMA A-fragment order ([tile][hi/lo][k-step][lane]; each wave
+// writes whole 1 KB fragment blocks, free of bank conflicts), and every wave
+// reads its column tiles' fragments from there; each wave loads its own S
+// rows (bytes of bf16(S/2), one contiguous 1 KB per hero tile and chunk)
+// straight into registers and widens them into B-fragments.  Each reach
+// element is fetched by 5 workgroups per board (instead of one per 128
+// heroes).  Chunk c + 1 is fetched while chunk c is multiplied.
+// Bytes of bf16(S/2) (16 villains) widened into the two k-step B-fragments.
+__device__ __forceinline__ void gemm_widen(const uint4& w, bf16x8_t& k0, bf16x8_t& k1) {
+    const uint4 a = make_uint4(__builtin_amdgcn_perm(0u, w.x, 0x010C000Cu), __builtin_amdgcn_perm(0u, w.x, 0x030C020Cu),
+                               __builtin_amdgcn_perm(0u, w.y, 0x010C000Cu), __builtin_amdgcn_perm(0u, w.y, 0x030C020Cu));
+    const uint4 c = make_uint4(__builtin_amdgcn_perm(0u, w.z, 0x010C000Cu), __builtin_amdgcn_perm(0u, w.z, 0x030C020Cu),
+                               __builtin_amdgcn_perm(0u, w.w, 0x010C000Cu), __builtin_amdgcn_perm(0u, w.w, 0x030C020Cu));
+    __builtin_memcpy(&k0, &a, 16);
+    __builtin_memcpy(&k1, &c, 16);
+}
+
+constexpr int G6_HT = 8;                                      // hero tiles per workgroup
+constexpr int G6_HB = (NPR / 32 + G6_HT - 1) / G6_HT;         // hero blocks per board
+
+template <int NTL>
+__global__ __launch_bounds__(512) void k_bt_gemm6(BatchedDev D, const uint8_t* __restrict__ S8, int nlogical) {
+    constexpr int NTILE_H = NPR / 32;                         // hero tiles of a board
+    constexpr int APT = (9 * 256 + 511) / 512;                // staged reach pieces per thread (at most)
+    __shared__ uint4 As[2][9][2][2][64];                      // [buf][column tile][hi/lo][k-step][lane]
+    const int per = (nlogical + 7) >> 3;
+    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (logical >= nlogical) return;
+    const int b = logical / G6_HB, hb = logical % G6_HB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, half = lane >> 5, l32 = lane & 31;
+    const int ncolp = D.L[1].ncolp, ntiles = ncolp >> 5;
+    const int nfirst = (ntiles + 1) >> 1;
+    const int ct0 = wave < 4 ? 0 : nfirst;
+    const int nct = wave < 4 ? nfirst : ntiles - nfirst;      // <= NTL
+    const int ht0 = hb * G6_HT + 2 * (wave & 3);
+    const bool on0 = ht0 < NTILE_H, on1 = ht0 + 1 < NTILE_H;
+    // S rows of this lane: hero (tile * 32 + l32), villains 16 * half .. + 15 of each chunk
+    const uint4* sb = reinterpret_cast<const uint4*>(S8 + (size_t)b * NPR * NPR);
+    const size_t so0 = (size_t)((on0 ? ht0 : 0) * 32 + l32) * 2 + half;
+    const size_t so1 = (size_t)((on1 ? ht0 + 1 : 0) * 32 + l32) * 2 + half;
+    constexpr int CH4 = NPR * 2;                              // uint4 per S chunk (all heroes)
+    // reach rows of the board: NPR bf16 = NPR / 8 uint4 per column
+    const uint4* bh = reinterpret_cast<const uint4*>(D.Bh + (size_t)b * ncolp * NPR);
+    const uint4* bl = reinterpret_cast<const uint4*>(D.Bl + (size_t)b * ncolp * NPR);
+    const int npieces = ntiles * 256;                         // 32 columns x 2 planes x 4 uint4 per tile
+    f32x16_t acc[NTL][2];
+#pragma unroll
+    for (int i = 0; i < NTL; ++i)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][t][r] = 0.0f;
+    uint4 ap0, ap1, ap2, ap3, ap4;   // staged reach pieces (APT == 5 named registers)
+    static_assert(APT == 5, "reach staging registers");
+    uint4 s0, s1;
+#define G6_LOAD_A(C_)                                                                            \
+    do {                                                                                         \
+        const int c_ = (C_);                                                                     \
+        G6_PIECE_LOAD(0, ap0, c_) G6_PIECE_LOAD(1, ap1, c_) G6_PIECE_LOAD(2, ap2, c_)           \
+        G6_PIECE_LOAD(3, ap3, c_) G6_PIECE_LOAD(4, ap4, c_)                                      \
+    } while (0)
+#define G6_PIECE_LOAD(Q_, R_, C_)                                                                \
+    {                                                                                            \
+        const int e = threadIdx.x + 512 * (Q_);                                                  \
+        if (e < npieces) {                                                                       \
+            const int blk = e >> 6, ln = e & 63;  /* fragment block (tile, hi/lo, k-step), lane */ \
+            const int col = (blk >> 2) * 32 + (ln & 31), j = 2 * (ln >> 5) + (blk & 1);         \
+            R_ = ((blk >> 1) & 1 ? bl : bh)[(size_t)col * (NPR / 8) + (C_) * 4 + j];             \
+        }                                                                                        \
+    }
+#define G6_STORE_A(BUF_)                                                                         \
+    do {                                                                                         \
+        G6_PIECE_STORE(0, ap0, BUF_) G6_PIECE_STORE(1, ap1, BUF_) G6_PIECE_STORE(2, ap2, BUF_)  \
+        G6_PIECE_STORE(3, ap3, BUF_) G6_PIECE_STORE(4, ap4, BUF_)                                \
+    } while (0)
+#define G6_PIECE_STORE(Q_, R_, BUF_)                                                             \
+    {                                                                                            \
+        const int e = threadIdx.x + 512 * (Q_);                                                  \
+        if (e < npieces) (&As[BUF_][0][0][0][0])[e] = R_;  /* whole 1 KB blocks per wave */      \
+    }
+    auto widen = [](const uint4& w, bf16x8_t& k0, bf16x8_t& k1) {
+        const uint4 a = make_uint4(__builtin_amdgcn_perm(0u, w.x, 0x010C000Cu), __builtin_amdgcn_perm(0u, w.x, 0x030C020Cu),
+                                   __builtin_amdgcn_perm(0u, w.y, 0x010C000Cu), __builtin_amdgcn_perm(0u, w.y, 0x030C020Cu));
+        const uint4 c = make_uint4(__builtin_amdgcn_perm(0u, w.z, 0x010C000Cu), __builtin_amdgcn_perm(0u, w.z, 0x030C020Cu),
+                                   __builtin_amdgcn_perm(0u, w.w, 0x010C000Cu), __builtin_amdgcn_perm(0u, w.w, 0x030C020Cu));
+        __builtin_memcpy(&k0, &a, 16);
+        __builtin_memcpy(&k1, &c, 16);
+    };
+    ap0 = ap1 = ap2 = ap3 = ap4 = make_uint4(0u, 0u, 0u, 0u);
+    G6_LOAD_A(0);
+    s0 = sb[so0];
+    s1 = sb[so1];
+    G6_STORE_A(0);
+    __syncthreads();
+    for (int c = 0; c < GNCH; ++c) {
+        const bool more = c + 1 < GNCH;
+        uint4 n0 = s0, n1 = s1;
+        if (more) {
+            G6_LOAD_A(c + 1);
+            n0 = sb[(size_t)(c + 1) * CH4 + so0];
+            n1 = sb[(size_t)(c + 1) * CH4 + so1];
+        }
+        bf16x8_t b00, b01, b10, b11;   // [hero tile][k-step]
+        widen(s0, b00, b01);
+        widen(s1, b10, b11);
+        const int buf = c & 1;
+#pragma unroll
+        for (int i = 0; i < NTL; ++i) {
+            if (on0 && i < nct) {
+                const int ct = ct0 + i;
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    bf16x8_t ah, al;
+                    __builtin_memcpy(&ah, &As[buf][ct][0][ks][lane], 16);
+                    __builtin_memcpy(&al, &As[buf][ct][1][ks][lane], 16);
+                    const bf16x8_t& bq0 = ks ? b01 : b00;
+                    const bf16x8_t& bq1 = ks ? b11 : b10;
+                    acc[i][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bq0, acc[i][0], 0, 0, 0);
+                    acc[i][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bq1, acc[i][1], 0, 0, 0);
+                    acc[i][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bq0, acc[i][0], 0, 0, 0);
+                    acc[i][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bq1, acc[i][1], 0, 0, 0);
+                }
+            }
+        }
+        if (more) G6_STORE_A(buf ^ 1);
+        __syncthreads();
+        s0 = n0;
+        s1 = n1;
+    }
+#undef G6_LOAD_A
+#undef G6_PIECE_LOAD
+#undef G6_STORE_A
+#undef G6_PIECE_STORE
+    float* Cb = D.C + (size_t)b * ncolp * NPR;
+#pragma unroll
+    for (int i = 0; i < NTL; ++i) {
+        if (i < nct) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                if (t == 0 ? on0 : on1) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int cc = (ct0 + i) * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+                        Cb[(size_t)cc * NPR + (ht0 + t) * 32 + l32] = 2.0f * acc[i][t][r];
+                    }
+                }
             }
         }
     }
@@ -652,6 +829,7 @@ struct BatchedState {
     uint8_t* d_S8 = nullptr;    // [boards1][NPR][NPR] sign-blocker bytes
     int dm[2] = {8, 8};         // register-stack depth per level (4, 6 or 8)
     JitSweeps jit;              // generated per-template sweeps (when enabled)
+    int gemm_shape = 100;       // river GEMM form and shape (autotuned; see river_gemm)
 };
 
 void batched_destroy(BatchedState* b) { delete b; }
@@ -807,6 +985,61 @@ int upload_level(DeviceArena& A, hipStream_t st, LevelHost& H, LevelDev& L, int 
     return MTT_OK;
 }
 
+// Tuning file: small JSON of "key": integer entries, $MTT_AUTOTUNE_FILE,
+// else $HOME/.cache/mtt/autotune.json.  A file that cannot be read or
+// written only means tuning again.
+std::string tune_path() {
+    const char* e = getenv("MTT_AUTOTUNE_FILE");
+    if (e && *e) return e;
+    const char* h = getenv("HOME");
+    return std::string(h && *h ? h : "/tmp") + "/.cache/mtt/autotune.json";
+}
+
+int tune_lookup(const std::string& key) {
+    FILE* f = fopen(tune_path().c_str(), "r");
+    if (!f) return 0;
+    char buf[8192];
+    size_t n = fread(buf, 1, sizeof(buf) - 1, f);
+    fclose(f);
+    buf[n] = 0;
+    std::string s(buf);
+    size_t p = s.find("\"" + key + "\"");
+    if (p == std::string::npos) return 0;
+    p = s.find(':', p);
+    if (p == std::string::npos) return 0;
+    return atoi(s.c_str() + p + 1);
+}
+
+void tune_store(const std::string& key, int v) {
+    std::string path = tune_path();
+    std::string dir = path.substr(0, path.rfind('/'));
+    std::string mk = "mkdir -p '" + dir + "'";
+    if (dir.find('\'') == std::string::npos) (void)system(mk.c_str());
+    std::string s;
+    if (FILE* f = fopen(path.c_str(), "r")) {
+        char buf[8192];
+        size_t n = fread(buf, 1, sizeof(buf) - 1, f);
+        fclose(f);
+        buf[n] = 0;
+        s = buf;
+    }
+    std::string entry = "\"" + key + "\": " + std::to_string(v);
+    size_t p = s.find("\"" + key + "\"");
+    if (p != std::string::npos) {
+        size_t e = s.find_first_of(",}", p);
+        s = s.substr(0, p) + entry + s.substr(e);
+    } else if (s.find('}') != std::string::npos && s.find(':') != std::string::npos) {
+        s = s.substr(0, s.rfind('}')) + ", " + entry + "}";
+    } else {
+        s = "{" + entry + "}";
+    }
+    if (FILE* f = fopen(path.c_str(), "w")) {
+        fputs(s.c_str(), f);
+        fputs("\n", f);
+        fclose(f);
+    }
+}
+
 // Street levels of a heads-up game for the batched path: river games (one
 // level, the single street) and turn+river games from a flop (the turn
 // template, then every river line as its own template).
@@ -855,6 +1088,8 @@ std::vector<JitLevel> jit_levels(const LevelHost& H0, const LevelHost& H1, bool 
 }
 
 }  // namespace
+
+int river_gemm_autotune(BatchedState& B, hipStream_t st);
 
 int batched_create(SolverImpl& s) {
     const mtt_hu_game& g = s.game;
@@ -1087,7 +1322,90 @@ int batched_create(SolverImpl& s) {
         rc = bt_jit_build(lv.data(), (int)lv.size(), bt_jit_config(), &B.jit);
         if (rc) return rc;
     }
+    rc = river_gemm_autotune(B, s.stream);
+    if (rc) return rc;
     MTT_HIP(hipStreamSynchronize(s.stream));
+    return MTT_OK;
+}
+
+// River showdown GEMM in one of its forms, by shape code: nw * 10 + pd for
+// the column-major form (k_bt_gemm5: NW column tiles per workgroup, PD K
+// chunks prefetched) or 100 for the hero-major form (k_bt_gemm6).
+int river_gemm(const BatchedState& B, int shape, hipStream_t st) {
+    const int ntiles = B.D.L[1].ncolp / 32;
+    if (shape == 100) {
+        const int nlog = B.nb1 * G6_HB;
+        const dim3 g(((nlog + 7) / 8) * 8);
+        switch ((ntiles + 1) / 2) {
+        case 1: hipLaunchKernelGGL(k_bt_gemm6<1>, g, dim3(512), 0, st, B.D, B.d_S8, nlog); break;
+        case 2: hipLaunchKernelGGL(k_bt_gemm6<2>, g, dim3(512), 0, st, B.D, B.d_S8, nlog); break;
+        case 3: hipLaunchKernelGGL(k_bt_gemm6<3>, g, dim3(512), 0, st, B.D, B.d_S8, nlog); break;
+        case 4: hipLaunchKernelGGL(k_bt_gemm6<4>, g, dim3(512), 0, st, B.D, B.d_S8, nlog); break;
+        default: hipLaunchKernelGGL(k_bt_gemm6<5>, g, dim3(512), 0, st, B.D, B.d_S8, nlog); break;
+        }
+        MTT_HIP(hipGetLastError());
+        return MTT_OK;
+    }
+    const int nw = shape / 10, pd = shape % 10;
+    MTT_ARG(nw >= 1 && nw <= 9 && pd >= 2 && pd <= 3, "river GEMM shape %d not available", shape);
+    const int ncg = (ntiles + nw - 1) / nw;
+    const int nlog = B.nb1 * GHB * ncg;
+    const dim3 g(((nlog + 7) / 8) * 8);
+#define MTT_RG(NW_)                                                                                               \
+    case NW_:                                                                                                     \
+        if (pd == 2) hipLaunchKernelGGL((k_bt_gemm5<NW_, 2>), g, dim3(NW_ * 64), 0, st, B.D, B.d_S8, nlog, ncg);  \
+        else hipLaunchKernelGGL((k_bt_gemm5<NW_, 3>), g, dim3(NW_ * 64), 0, st, B.D, B.d_S8, nlog, ncg);          \
+        break;
+    switch (nw) { MTT_RG(1) MTT_RG(2) MTT_RG(3) MTT_RG(4) MTT_RG(5) MTT_RG(6) MTT_RG(7) MTT_RG(8) MTT_RG(9) }
+#undef MTT_RG
+    MTT_HIP(hipGetLastError());
+    return MTT_OK;
+}
+
+// Autotuner of the river GEMM: the hero-major form and the column-major
+// form with all of the board's column tiles, or a half or a third of them,
+// per workgroup, prefetching 2 or 3 K chunks; each timed on the real
+// operands, the winner kept in the tuning file per column-tile count.
+// MTT_RIVER_GEMM=<shape code> forces one.
+int river_gemm_autotune(BatchedState& B, hipStream_t st) {
+    const int ntiles = B.D.L[1].ncolp / 32;
+    auto valid = [&](int v) {
+        return v == 100 || (v / 10 >= 1 && v / 10 <= std::min(ntiles, 9) && v % 10 >= 2 && v % 10 <= 3);
+    };
+    if (const char* e = getenv("MTT_RIVER_GEMM")) {
+        const int v = atoi(e);
+        if (valid(v)) { B.gemm_shape = v; return MTT_OK; }
+    }
+    const std::string key = "river_gemm_shape/tiles" + std::to_string(ntiles);
+    const int v = tune_lookup(key);
+    if (valid(v)) { B.gemm_shape = v; return MTT_OK; }
+    std::vector<int> cand{100};
+    for (int d = 1; d <= 3; ++d) {
+        const int nw = (ntiles + d - 1) / d;
+        for (int pd = 2; pd <= 3; ++pd)
+            if (std::find(cand.begin(), cand.end(), nw * 10 + pd) == cand.end()) cand.push_back(nw * 10 + pd);
+    }
+    hipEvent_t a, b;
+    MTT_HIP(hipEventCreate(&a));
+    MTT_HIP(hipEventCreate(&b));
+    float best = 1e30f;
+    int rc = MTT_OK;
+    for (int shape : cand) {
+        rc = river_gemm(B, shape, st);  // warm
+        if (rc) break;
+        (void)hipEventRecord(a, st);
+        for (int r = 0; r < 2 && !rc; ++r) rc = river_gemm(B, shape, st);
+        if (rc) break;
+        (void)hipEventRecord(b, st);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (ms < best) { best = ms; B.gemm_shape = shape; }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    tune_store(key, B.gemm_shape);
     return MTT_OK;
 }
 
@@ -1125,29 +1443,12 @@ int batched_run(SolverImpl& s, int iters) {
                       else if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_forward<1, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
                       else hipLaunchKernelGGL((k_bt_forward<1, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
-        ::mtt::KernelTimer::Mark gm;
-        if (s.timer.enabled) s.timer.begin("river_showdown_gemm", s.stream, gm);
-        {
-            const int nlog = B.nb1 * GHB;
-            const dim3 g(((nlog + 7) / 8) * 8);
-            switch (L1.ncolp / 32) {
-            case 1: hipLaunchKernelGGL(k_bt_gemm3<1>, g, dim3(64), 0, s.stream, B.D, B.d_S8, nlog); break;
-            case 2: hipLaunchKernelGGL(k_bt_gemm3<2>, g, dim3(128), 0, s.stream, B.D, B.d_S8, nlog); break;
-            case 3: hipLaunchKernelGGL(k_bt_gemm3<3>, g, dim3(192), 0, s.stream, B.D, B.d_S8, nlog); break;
-            case 4: hipLaunchKernelGGL(k_bt_gemm3<4>, g, dim3(256), 0, s.stream, B.D, B.d_S8, nlog); break;
-            case 5: hipLaunchKernelGGL(k_bt_gemm3<5>, g, dim3(320), 0, s.stream, B.D, B.d_S8, nlog); break;
-            case 6: hipLaunchKernelGGL(k_bt_gemm3<6>, g, dim3(384), 0, s.stream, B.D, B.d_S8, nlog); break;
-            case 7: hipLaunchKernelGGL(k_bt_gemm3<7>, g, dim3(448), 0, s.stream, B.D, B.d_S8, nlog); break;
-            case 8: hipLaunchKernelGGL(k_bt_gemm3<8>, g, dim3(512), 0, s.stream, B.D, B.d_S8, nlog); break;
-            default: hipLaunchKernelGGL(k_bt_gemm3<9>, g, dim3(576), 0, s.stream, B.D, B.d_S8, nlog); break;
-            }
-        }
-        if (s.timer.enabled) s.timer.end(gm, s.stream);
+        MTT_TIMED(s, "river_showdown_gemm", rc = river_gemm(B, B.gemm_shape, s.stream));
         {
             const int jnt = B.jit.cfg.bwd_nt, jnh = (NPT + jnt - 1) / jnt;
             const size_t sh = B.jit.active ? bwd_lds(L1, jnt) : bwd_lds(L1);
             const dim3 g(B.nb1 * L1.ntmpl * (B.jit.active ? jnh : BWD_NH));
-            void* args[] = {&B.D, &ranges, &wp, &wn};
+            void* args[] = {&B.D, &ranges, &wp, &wn, &ws};
             MTT_TIMED(s, "river_backward",
                       if (B.jit.active) rc = jit_launch(B.jit.bwd[1], g, jnt, sh, s.stream, args);
                       else if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_backward<1, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
@@ -1163,7 +1464,7 @@ int batched_run(SolverImpl& s, int iters) {
             const int jnt = B.jit.cfg.bwd_nt, jnh = (NPT + jnt - 1) / jnt;
             const size_t sh = B.jit.active ? bwd_lds(L0, jnt) : bwd_lds(L0);
             const dim3 g(49 * L0.ntmpl * (B.jit.active ? jnh : BWD_NH));
-            void* args[] = {&B.D, &ranges, &wp, &wn};
+            void* args[] = {&B.D, &ranges, &wp, &wn, &ws};
             MTT_TIMED(s, "turn_backward",
                       if (B.jit.active) rc = jit_launch(B.jit.bwd[0], g, jnt, sh, s.stream, args);
                       else if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_backward<0, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
@@ -1470,61 +1771,7 @@ int batch_weights(BatchImpl& B, int tmax) {
 
 // Autotuner: the flop GEMM's hero-tile count (2 or 4 tiles of 32 per
 // workgroup) is timed on the real operands once per column-tile count and
-// the winner is kept in a small JSON file ($MTT_AUTOTUNE_FILE, else
-// $HOME/.cache/mtt/autotune.json); a file that cannot be read or written
-// only means tuning again.
-std::string tune_path() {
-    const char* e = getenv("MTT_AUTOTUNE_FILE");
-    if (e && *e) return e;
-    const char* h = getenv("HOME");
-    return std::string(h && *h ? h : "/tmp") + "/.cache/mtt/autotune.json";
-}
-
-int tune_lookup(const std::string& key) {
-    FILE* f = fopen(tune_path().c_str(), "r");
-    if (!f) return 0;
-    char buf[8192];
-    size_t n = fread(buf, 1, sizeof(buf) - 1, f);
-    fclose(f);
-    buf[n] = 0;
-    std::string s(buf);
-    size_t p = s.find("\"" + key + "\"");
-    if (p == std::string::npos) return 0;
-    p = s.find(':', p);
-    if (p == std::string::npos) return 0;
-    return atoi(s.c_str() + p + 1);
-}
-
-void tune_store(const std::string& key, int v) {
-    std::string path = tune_path();
-    std::string dir = path.substr(0, path.rfind('/'));
-    std::string mk = "mkdir -p '" + dir + "'";
-    if (dir.find('\'') == std::string::npos) (void)system(mk.c_str());
-    std::string s;
-    if (FILE* f = fopen(path.c_str(), "r")) {
-        char buf[8192];
-        size_t n = fread(buf, 1, sizeof(buf) - 1, f);
-        fclose(f);
-        buf[n] = 0;
-        s = buf;
-    }
-    std::string entry = "\"" + key + "\": " + std::to_string(v);
-    size_t p = s.find("\"" + key + "\"");
-    if (p != std::string::npos) {
-        size_t e = s.find_first_of(",}", p);
-        s = s.substr(0, p) + entry + s.substr(e);
-    } else if (s.find('}') != std::string::npos && s.find(':') != std::string::npos) {
-        s = s.substr(0, s.rfind('}')) + ", " + entry + "}";
-    } else {
-        s = "{" + entry + "}";
-    }
-    if (FILE* f = fopen(path.c_str(), "w")) {
-        fputs(s.c_str(), f);
-        fputs("\n", f);
-        fclose(f);
-    }
-}
-
+// the winner is kept in the tuning file (see tune_path).
 int batch_autotune(BatchImpl& B) {
     const std::string key = "flop_gemm_ht/nw" + std::to_string(B.D.L[2].ncolp / 32);
     int v = tune_lookup(key);
