@@ -1448,7 +1448,7 @@ int batched_run(SolverImpl& s, int iters) {
             const int jnt = B.jit.cfg.bwd_nt, jnh = (NPT + jnt - 1) / jnt;
             const size_t sh = B.jit.active ? bwd_lds(L1, jnt) : bwd_lds(L1);
             const dim3 g(B.nb1 * L1.ntmpl * (B.jit.active ? jnh : BWD_NH));
-            void* args[] = {&B.D, &ranges, &wp, &wn, &ws};
+            void* args[] = {&B.D, &ranges, &wp, &wn};
             MTT_TIMED(s, "river_backward",
                       if (B.jit.active) rc = jit_launch(B.jit.bwd[1], g, jnt, sh, s.stream, args);
                       else if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_backward<1, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
@@ -1464,7 +1464,7 @@ int batched_run(SolverImpl& s, int iters) {
             const int jnt = B.jit.cfg.bwd_nt, jnh = (NPT + jnt - 1) / jnt;
             const size_t sh = B.jit.active ? bwd_lds(L0, jnt) : bwd_lds(L0);
             const dim3 g(49 * L0.ntmpl * (B.jit.active ? jnh : BWD_NH));
-            void* args[] = {&B.D, &ranges, &wp, &wn, &ws};
+            void* args[] = {&B.D, &ranges, &wp, &wn};
             MTT_TIMED(s, "turn_backward",
                       if (B.jit.active) rc = jit_launch(B.jit.bwd[0], g, jnt, sh, s.stream, args);
                       else if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_backward<0, 4, BWD_NT>), g, dim3(BWD_NT), sh, s.stream, B.D, ranges, wp, wn, nullptr);
