@@ -1524,11 +1524,11 @@ namespace {
 constexpr int FNCH = NPF / 32;  // K chunks of 32 villains
 
 // W - L of every (hero, villain) pair from exhaustive counts (alive-1216
-// layout), split into two BF16-exact parts: hi = 16 * round(S / 16)
-// (|hi| <= 992, six significant bits) and lo = S - hi (|lo| <= 8).
-// Layout [board][chunk][hero][32] like the sign bytes of the river path.
-__global__ void k_flop_s_split(const int32_t* __restrict__ W, const int32_t* __restrict__ T, const uint8_t* __restrict__ cc,
-                               int b0, __hip_bfloat16* __restrict__ Sh, __hip_bfloat16* __restrict__ Sl) {
+// layout) as int16 (|W - L| <= 990; blocked and padded pairs 0), layout
+// [board][chunk][hero][32] like the sign bytes of the river path.  The GEMM
+// splits each value exactly into BF16 parts when it stages S.
+__global__ void k_flop_s16(const int32_t* __restrict__ W, const int32_t* __restrict__ T, const uint8_t* __restrict__ cc,
+                           int b0, int16_t* __restrict__ S16) {
     const int i = blockIdx.x, bl = blockIdx.y, b = b0 + bl;
     const uint8_t a1 = cc[((size_t)b * NPF + i) * 2], a2 = cc[((size_t)b * NPF + i) * 2 + 1];
     const int32_t* w = W + ((size_t)bl * NPF + i) * NPF;
@@ -1539,54 +1539,66 @@ __global__ void k_flop_s_split(const int32_t* __restrict__ W, const int32_t* __r
             const uint8_t b1 = cc[((size_t)b * NPF + j) * 2], b2 = cc[((size_t)b * NPF + j) * 2 + 1];
             if (a1 != b1 && a1 != b2 && a2 != b1 && a2 != b2) v = 2 * w[j] + t[j] - 990;
         }
-        const int hi = 16 * ((v + (v >= 0 ? 8 : -8)) / 16);
-        const size_t o = (((size_t)b * FNCH + (j >> 5)) * NPF + i) * 32 + (j & 31);
-        Sh[o] = __float2bfloat16((float)hi);
-        Sl[o] = __float2bfloat16((float)(v - hi));
+        S16[(((size_t)b * FNCH + (j >> 5)) * NPF + i) * 32 + (j & 31)] = (int16_t)v;
     }
 }
 
+// Eight int16 W - L values (one uint4) split exactly into BF16 parts:
+// v = 16 * (v >> 4) + (v & 15), |16 * (v >> 4)| <= 992 (six significant
+// bits) and v & 15 in [0, 15].  Element order is kept.
+__device__ __forceinline__ unsigned s16_hi2(unsigned u) {  // 16 * (v >> 4) of both halves, as bf16 pair
+    const float h0 = (float)(((int)(u << 16) >> 20) * 16), h1 = (float)(((int)u >> 20) * 16);
+    return __builtin_amdgcn_perm(__float_as_uint(h1), __float_as_uint(h0), 0x07060302u);
+}
+__device__ __forceinline__ unsigned s16_lo2(unsigned u) {  // v & 15 of both halves, as bf16 pair
+    const float l0 = (float)(u & 15u), l1 = (float)((u >> 16) & 15u);
+    return __builtin_amdgcn_perm(__float_as_uint(l1), __float_as_uint(l0), 0x07060302u);
+}
+__device__ __forceinline__ void s16_split(const uint4& w, uint4& hi, uint4& lo) {
+    const unsigned a = w.x, b = w.y, c = w.z, d = w.w;
+    hi = make_uint4(s16_hi2(a), s16_hi2(b), s16_hi2(c), s16_hi2(d));
+    lo = make_uint4(s16_lo2(a), s16_lo2(b), s16_lo2(c), s16_lo2(d));
+}
+
 // Flop showdown GEMM: C[b][col][i] = sum_j (Sh + Sl)[i][j] * (Bh + Bl)[b][col][j]
-// in four BF16 products per K step.  HT 32-hero tiles per workgroup, one
-// 32-column tile per wave (waves past the board's columns skip the math);
-// S chunks staged in LDS (double-buffered), reach fragments streamed one
-// chunk ahead; XCD-grouped numbering as in the river GEMM.
-// S staging of the flop GEMM: piece e < 2*PIECES of a chunk is uint4 x of
-// the hi (e < PIECES) or lo array; x = hero * 4 + (villains 8q .. 8q+7).
+// in four BF16 products per K step, S = W - L read as int16 and split into
+// Sh + Sl while it is staged.  HT 32-hero tiles per workgroup, one 32-column
+// tile per wave (waves past the board's columns skip the math); S chunks
+// staged in LDS (double-buffered), reach fragments streamed one chunk ahead;
+// XCD-grouped numbering as in the river GEMM.
+// S staging: piece x < npieces of a chunk is uint4 x of the int16 array,
+// x = hero * 4 + (villains 8q .. 8q+7).
 template <int NT, int PIECES, int SPT>
-__device__ __forceinline__ void flop_load_s(uint4 (&sv)[SPT], const uint4* __restrict__ s0, const uint4* __restrict__ s1,
-                                            int c, int npieces) {
+__device__ __forceinline__ void flop_load_s(uint4 (&sv)[SPT], const uint4* __restrict__ s16, int c, int npieces) {
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
-        const int e = threadIdx.x + NT * q;
-        const bool lo = e >= PIECES;
-        const int x = lo ? e - PIECES : e;
-        const bool ok = e < 2 * PIECES && x < npieces;
-        sv[q] = (lo ? s1 : s0)[(size_t)c * (NPF * 4) + (ok ? x : 0)];  // every path writes sv (keeps it in registers)
+        const int x = threadIdx.x + NT * q;
+        const bool ok = x < PIECES && x < npieces;
+        sv[q] = s16[(size_t)c * (NPF * 4) + (ok ? x : 0)];  // every path writes sv (keeps it in registers)
     }
 }
 template <int NT, int PIECES, int SPT, int HT>
 __device__ __forceinline__ void flop_store_s(const uint4 (&sv)[SPT], uint4 (&dst)[2][HT][2][2][32], int npieces) {
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
-        const int e = threadIdx.x + NT * q;
-        const bool lo = e >= PIECES;
-        const int x = lo ? e - PIECES : e;
-        if (e < 2 * PIECES && x < npieces) {
+        const int x = threadIdx.x + NT * q;
+        if (x < PIECES && x < npieces) {
             const int hero = x >> 2, qq = x & 3;
-            dst[lo ? 1 : 0][hero >> 5][qq >> 1][qq & 1][hero & 31] = sv[q];
+            uint4 hi, lo;
+            s16_split(sv[q], hi, lo);
+            dst[0][hero >> 5][qq >> 1][qq & 1][hero & 31] = hi;
+            dst[1][hero >> 5][qq >> 1][qq & 1][hero & 31] = lo;
         }
     }
 }
 
 template <int NW, int HT>
-__global__ __launch_bounds__((NW < 4 ? 4 : NW) * 64) void k_bt_gemm_flop(BatchedDev D, const __hip_bfloat16* __restrict__ Sh,
-                                                                         const __hip_bfloat16* __restrict__ Sl,
+__global__ __launch_bounds__((NW < 4 ? 4 : NW) * 64) void k_bt_gemm_flop(BatchedDev D, const int16_t* __restrict__ S16,
                                                                          const int* __restrict__ bncol, int nlogical) {
     constexpr int NT = (NW < 4 ? 4 : NW) * 64;  // at least four waves share the S staging
     constexpr int HB = (NPF + 32 * HT - 1) / (32 * HT);
-    constexpr int PIECES = HT * 128;            // uint4 pieces of one S array per chunk
-    constexpr int SPT = (2 * PIECES + NT - 1) / NT;
+    constexpr int PIECES = HT * 128;            // uint4 pieces of the int16 S array per chunk
+    constexpr int SPT = (PIECES + NT - 1) / NT;
     __shared__ uint4 Ssh[2][2][HT][2][2][32];   // [buf][hi/lo][hero tile][h][k-step][hero]
     const int per = (nlogical + 7) >> 3;
     const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
@@ -1599,9 +1611,7 @@ __global__ __launch_bounds__((NW < 4 ? 4 : NW) * 64) void k_bt_gemm_flop(Batched
     const int col = (wave < NW ? wave : 0) * 32 + l32;
     const uint4* ah_p = reinterpret_cast<const uint4*>(D.Bh + ((size_t)b * ncolp + col) * NPF + 16 * half);
     const uint4* al_p = reinterpret_cast<const uint4*>(D.Bl + ((size_t)b * ncolp + col) * NPF + 16 * half);
-    const uint4* sp0 = reinterpret_cast<const uint4*>(Sh + ((size_t)b * FNCH * NPF + i0) * 32);
-    const uint4* sp1 = reinterpret_cast<const uint4*>(Sl + ((size_t)b * FNCH * NPF + i0) * 32);
-    constexpr int CH4 = NPF * 4;                // uint4 pieces per chunk of one array (all heroes)
+    const uint4* sp = reinterpret_cast<const uint4*>(S16 + ((size_t)b * FNCH * NPF + i0) * 32);
     const int npieces = nht * 128;
     f32x16_t acc[HT];
 #pragma unroll
@@ -1610,7 +1620,7 @@ __global__ __launch_bounds__((NW < 4 ? 4 : NW) * 64) void k_bt_gemm_flop(Batched
         for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
     uint4 sv[SPT];
     uint4 a_cur[4], a_nxt[4];
-    flop_load_s<NT, PIECES, SPT>(sv, sp0, sp1, 0, npieces);
+    flop_load_s<NT, PIECES, SPT>(sv, sp, 0, npieces);
     a_cur[0] = ah_p[0]; a_cur[1] = ah_p[1]; a_cur[2] = al_p[0]; a_cur[3] = al_p[1];
     flop_store_s<NT, PIECES, SPT, HT>(sv, Ssh[0], npieces);
     __syncthreads();
@@ -1618,7 +1628,7 @@ __global__ __launch_bounds__((NW < 4 ? 4 : NW) * 64) void k_bt_gemm_flop(Batched
     for (int c = 0; c < FNCH; ++c) {
         const bool more = c + 1 < FNCH;
         if (more) {
-            flop_load_s<NT, PIECES, SPT>(sv, sp0, sp1, c + 1, npieces);
+            flop_load_s<NT, PIECES, SPT>(sv, sp, c + 1, npieces);
             const int o = (c + 1) * 4;
             a_nxt[0] = ah_p[o]; a_nxt[1] = ah_p[o + 1]; a_nxt[2] = al_p[o]; a_nxt[3] = al_p[o + 1];
         }
@@ -1691,7 +1701,7 @@ struct BatchImpl {
     DeviceArena arena;
     BatchedDev D;
     float* d_ranges = nullptr;
-    __hip_bfloat16 *d_Sh = nullptr, *d_Sl = nullptr;
+    int16_t* d_S16 = nullptr;        // W - L, [instance][chunk][hero][32]
     int* d_bncol = nullptr;
     int* d_t = nullptr;
     float *d_wtab = nullptr, *d_wcur = nullptr;
@@ -1712,9 +1722,9 @@ int batch_launch_gemm(BatchImpl& B, int ht) {
 #define MTT_FG(NW_)                                                                                                   \
     case NW_:                                                                                                         \
         if (ht == 4) hipLaunchKernelGGL((k_bt_gemm_flop<NW_, 4>), g, dim3((NW_ < 4 ? 4 : NW_) * 64), 0, B.stream, B.D,  \
-                                        B.d_Sh, B.d_Sl, B.d_bncol, nlog);                                            \
-        else hipLaunchKernelGGL((k_bt_gemm_flop<NW_, 2>), g, dim3((NW_ < 4 ? 4 : NW_) * 64), 0, B.stream, B.D, B.d_Sh, \
-                                B.d_Sl, B.d_bncol, nlog);                                                             \
+                                        B.d_S16, B.d_bncol, nlog);                                                   \
+        else hipLaunchKernelGGL((k_bt_gemm_flop<NW_, 2>), g, dim3((NW_ < 4 ? 4 : NW_) * 64), 0, B.stream, B.D,       \
+                                B.d_S16, B.d_bncol, nlog);                                                           \
         break;
     switch (nw) {
         MTT_FG(1) MTT_FG(2) MTT_FG(3) MTT_FG(4) MTT_FG(5) MTT_FG(6) MTT_FG(7) MTT_FG(8) MTT_FG(9)
@@ -1771,9 +1781,13 @@ int batch_weights(BatchImpl& B, int tmax) {
 
 // Autotuner: the flop GEMM's hero-tile count (2 or 4 tiles of 32 per
 // workgroup) is timed on the real operands once per column-tile count and
-// the winner is kept in the tuning file (see tune_path).
+// batch-size bucket, and the winner is kept in the tuning file (see
+// tune_path).
 int batch_autotune(BatchImpl& B) {
-    const std::string key = "flop_gemm_ht/nw" + std::to_string(B.D.L[2].ncolp / 32);
+    // the best tile count depends on how many workgroups fill the GPU: tuned per power-of-two batch size
+    int bucket = 1;
+    while (bucket < B.n) bucket <<= 1;
+    const std::string key = "flop_gemm_ht/nw" + std::to_string(B.D.L[2].ncolp / 32) + "/n" + std::to_string(bucket);
     int v = tune_lookup(key);
     if (v == 2 || v == 4) { B.ht = v; return MTT_OK; }
     hipEvent_t a, b;
@@ -1876,8 +1890,7 @@ int batch_create(const mtt_hu_game* games, int n, const float* ranges, const mtt
     rc = rc ? rc : A.alloc(nbc, &B.D.Bl);
     rc = rc ? rc : A.alloc(nbc, &B.D.C);
     rc = rc ? rc : A.alloc((size_t)n * 2 * NPF, &B.D.rootval0);
-    rc = rc ? rc : A.alloc((size_t)n * FNCH * NPF * 32, &B.d_Sh);
-    rc = rc ? rc : A.alloc((size_t)n * FNCH * NPF * 32, &B.d_Sl);
+    rc = rc ? rc : A.alloc((size_t)n * FNCH * NPF * 32, &B.d_S16);
     rc = rc ? rc : A.alloc(1, &B.d_t);
     rc = rc ? rc : A.alloc(3, &B.d_wcur);
     if (rc) return rc;
@@ -1897,7 +1910,7 @@ int batch_create(const mtt_hu_game* games, int n, const float* ranges, const mtt
             const int m = std::min(CH, n - k0);
             rc = mtt_flop_counts(flops.data() + (size_t)k0 * 3, m, dW, dT, MTT_LAYOUT_ALIVE1216, B.stream);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_flop_s_split, dim3(NPF, m), dim3(256), 0, B.stream, dW, dT, L.cc, k0, B.d_Sh, B.d_Sl);
+            hipLaunchKernelGGL(k_flop_s16, dim3(NPF, m), dim3(256), 0, B.stream, dW, dT, L.cc, k0, B.d_S16);
             MTT_HIP(hipGetLastError());
         }
         MTT_HIP(hipStreamSynchronize(B.stream));
