@@ -443,6 +443,45 @@ __global__ void k_bt_s8(BatchedDev D, uint8_t* __restrict__ S8) {
 
 constexpr int GNCH = NPR / 32;           // K chunks of 32 villains
 
+// The same signs at two bits per pair: word (b, chunk c, hero i, half h)
+// holds villains 32c + 16h + e (e = 0..15) at bits 2e, 2e+1; code 1 = +1/2
+// (hero ranks higher), 2 = -1/2, 0 = tie, blocked or dead.  One thread per
+// word.
+__global__ void k_bt_s2(BatchedDev D, uint32_t* __restrict__ S2) {
+    const int b = blockIdx.y, i = blockIdx.x, q = threadIdx.x;  // q = c * 2 + h
+    if (q >= GNCH * 2) return;
+    const int nl = D.L[1].nlive[b];
+    const uint16_t gi = D.grank[(size_t)b * NPR + i];
+    const uint32_t xi = D.cx[(size_t)b * NPR + i];
+    const int c = q >> 1, h = q & 1;
+    uint32_t w = 0;
+    for (int e = 0; e < 16; ++e) {
+        const int j = c * 32 + 16 * h + e;
+        if (i < nl && j < nl && !conflict(xi, D.cy[(size_t)b * NPR + j])) {
+            const uint16_t gj = D.grank[(size_t)b * NPR + j];
+            w |= (gi > gj ? 1u : (gi < gj ? 2u : 0u)) << (2 * e);
+        }
+    }
+    S2[(((size_t)b * GNCH + c) * NPR + i) * 2 + h] = w;
+}
+
+// Eight 2-bit sign codes (bits 0..15 of x, element e at bits 2e, 2e+1) to
+// the bf16 B-fragment of S/2: one byte lookup per pair of elements
+// (table bytes 0x00, 0x3F, 0xBF: the high byte of bf16 0, +1/2, -1/2).
+__device__ __forceinline__ bf16x8_t s2_frag(uint32_t x) {
+    constexpr uint32_t T = 0x00BF3F00u;
+    uint32_t o[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const uint32_t cp = (x >> (4 * m)) & 15u;
+        const uint32_t sel = 0x000C000Cu | ((cp & 3u) << 8) | ((cp >> 2) << 24);
+        o[m] = __builtin_amdgcn_perm(0u, T, sel);
+    }
+    bf16x8_t r;
+    __builtin_memcpy(&r, o, 16);
+    return r;
+}
+
 // River showdown GEMM: C[b][col][i] = 2 * sum_j (S/2)[i][j] * (Bh + Bl)[b][col][j].
 // One workgroup = 4 32-hero tiles x NW 32-column tiles of one board (the
 // board's column tiles are split into ncg groups of NW); wave w owns column
@@ -725,6 +764,137 @@ __global__ __launch_bounds__(512) void k_bt_gemm6(BatchedDev D, const uint8_t* _
     }
 }
 
+// The hero-major river GEMM with S read at two bits per pair (k_bt_s2): a
+// lane's 16 villains of one hero row are one dword per chunk, so the S
+// stream is a quarter of the byte form and is fetched three chunks ahead in
+// a register shift ring; the reach stream and everything else are as in
+// k_bt_gemm6.
+template <int NTL>
+__global__ __launch_bounds__(512) void k_bt_gemm6s(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical) {
+    constexpr int NTILE_H = NPR / 32;
+    constexpr int APT = (9 * 256 + 511) / 512;
+    __shared__ uint4 As[2][9][2][2][64];
+    const int per = (nlogical + 7) >> 3;
+    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (logical >= nlogical) return;
+    const int b = logical / G6_HB, hb = logical % G6_HB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, half = lane >> 5, l32 = lane & 31;
+    const int ncolp = D.L[1].ncolp, ntiles = ncolp >> 5;
+    const int nfirst = (ntiles + 1) >> 1;
+    const int ct0 = wave < 4 ? 0 : nfirst;
+    const int nct = wave < 4 ? nfirst : ntiles - nfirst;
+    const int ht0 = hb * G6_HT + 2 * (wave & 3);
+    const bool on0 = ht0 < NTILE_H, on1 = ht0 + 1 < NTILE_H;
+    // S words of this lane: [b][chunk][hero][half]
+    const uint32_t* sw = S2 + (size_t)b * GNCH * NPR * 2;
+    const size_t so0 = (size_t)((on0 ? ht0 : 0) * 32 + l32) * 2 + half;
+    const size_t so1 = (size_t)((on1 ? ht0 + 1 : 0) * 32 + l32) * 2 + half;
+    constexpr int CHW = NPR * 2;                               // S words per chunk (all heroes)
+    const uint4* bh = reinterpret_cast<const uint4*>(D.Bh + (size_t)b * ncolp * NPR);
+    const uint4* bl = reinterpret_cast<const uint4*>(D.Bl + (size_t)b * ncolp * NPR);
+    const int npieces = ntiles * 256;
+    f32x16_t acc[NTL][2];
+#pragma unroll
+    for (int i = 0; i < NTL; ++i)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][t][r] = 0.0f;
+    uint4 ap0, ap1, ap2, ap3, ap4;
+    static_assert(APT == 5, "reach staging registers");
+#define G6S_PIECE_LOAD(Q_, R_, C_)                                                                \
+    {                                                                                             \
+        const int e = threadIdx.x + 512 * (Q_);                                                   \
+        if (e < npieces) {                                                                        \
+            const int blk = e >> 6, ln = e & 63;                                                  \
+            const int col = (blk >> 2) * 32 + (ln & 31), j = 2 * (ln >> 5) + (blk & 1);          \
+            R_ = ((blk >> 1) & 1 ? bl : bh)[(size_t)col * (NPR / 8) + (C_) * 4 + j];              \
+        }                                                                                         \
+    }
+#define G6S_PIECE_STORE(Q_, R_, BUF_)                                                             \
+    {                                                                                             \
+        const int e = threadIdx.x + 512 * (Q_);                                                   \
+        if (e < npieces) (&As[BUF_][0][0][0][0])[e] = R_;                                         \
+    }
+#define G6S_LOAD_A(C_)                                                                            \
+    do {                                                                                          \
+        const int c_ = (C_);                                                                      \
+        G6S_PIECE_LOAD(0, ap0, c_) G6S_PIECE_LOAD(1, ap1, c_) G6S_PIECE_LOAD(2, ap2, c_)         \
+        G6S_PIECE_LOAD(3, ap3, c_) G6S_PIECE_LOAD(4, ap4, c_)                                     \
+    } while (0)
+#define G6S_STORE_A(BUF_)                                                                         \
+    do {                                                                                          \
+        G6S_PIECE_STORE(0, ap0, BUF_) G6S_PIECE_STORE(1, ap1, BUF_) G6S_PIECE_STORE(2, ap2, BUF_) \
+        G6S_PIECE_STORE(3, ap3, BUF_) G6S_PIECE_STORE(4, ap4, BUF_)                               \
+    } while (0)
+    ap0 = ap1 = ap2 = ap3 = ap4 = make_uint4(0u, 0u, 0u, 0u);
+    static_assert(GNCH >= 3, "S ring depth");
+    // S shift ring: (a0, a1) chunk c, (b0, b1) chunk c + 1, (c0, c1) chunk c + 2
+    uint32_t sa0 = sw[so0], sa1 = sw[so1];
+    uint32_t sb0 = sw[CHW + so0], sb1 = sw[CHW + so1];
+    uint32_t sc0 = sw[2 * CHW + so0], sc1 = sw[2 * CHW + so1];
+    G6S_LOAD_A(0);
+    G6S_STORE_A(0);
+    __syncthreads();
+    for (int c = 0; c < GNCH; ++c) {
+        const bool more = c + 1 < GNCH;
+        if (more) G6S_LOAD_A(c + 1);
+        uint32_t sd0 = 0u, sd1 = 0u;   // chunk c + 3
+        if (c + 3 < GNCH) {
+            sd0 = sw[(size_t)(c + 3) * CHW + so0];
+            sd1 = sw[(size_t)(c + 3) * CHW + so1];
+        }
+        const int buf = c & 1;
+        if (on0) {
+            const bf16x8_t b00 = s2_frag(sa0), b01 = s2_frag(sa0 >> 16);
+            const bf16x8_t b10 = s2_frag(sa1), b11 = s2_frag(sa1 >> 16);
+#pragma unroll
+            for (int i = 0; i < NTL; ++i) {
+                if (i < nct) {
+                    const int ct = ct0 + i;
+#pragma unroll
+                    for (int ks = 0; ks < 2; ++ks) {
+                        bf16x8_t ah, al;
+                        __builtin_memcpy(&ah, &As[buf][ct][0][ks][lane], 16);
+                        __builtin_memcpy(&al, &As[buf][ct][1][ks][lane], 16);
+                        const bf16x8_t& bq0 = ks ? b01 : b00;
+                        const bf16x8_t& bq1 = ks ? b11 : b10;
+                        acc[i][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bq0, acc[i][0], 0, 0, 0);
+                        acc[i][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bq1, acc[i][1], 0, 0, 0);
+                        acc[i][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bq0, acc[i][0], 0, 0, 0);
+                        acc[i][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bq1, acc[i][1], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        if (more) G6S_STORE_A(buf ^ 1);
+        __syncthreads();
+        sa0 = sb0; sa1 = sb1;
+        sb0 = sc0; sb1 = sc1;
+        sc0 = sd0; sc1 = sd1;
+    }
+#undef G6S_LOAD_A
+#undef G6S_PIECE_LOAD
+#undef G6S_STORE_A
+#undef G6S_PIECE_STORE
+    float* Cb = D.C + (size_t)b * ncolp * NPR;
+#pragma unroll
+    for (int i = 0; i < NTL; ++i) {
+        if (i < nct) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                if (t == 0 ? on0 : on1) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int cc = (ct0 + i) * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+                        Cb[(size_t)cc * NPR + (ht0 + t) * 32 + l32] = 2.0f * acc[i][t][r];
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Turn all-in showdowns: C0[t][col][i] = sum_j E[t][j][i] * B0[t][col][j].
 template <int MAXC>
 __global__ __launch_bounds__(256) void k_bt_turn_sd(BatchedDev D, int ncol) {
@@ -827,6 +997,7 @@ struct BatchedState {
     uint16_t *d_inv0 = nullptr, *d_inv1 = nullptr;
     float* d_ranges = nullptr;  // root reach of the first street [boards][2][np]
     uint8_t* d_S8 = nullptr;    // [boards1][NPR][NPR] sign-blocker bytes
+    uint32_t* d_S2 = nullptr;   // the same signs at two bits per pair ([boards1][chunk][NPR][2] words)
     int dm[2] = {8, 8};         // register-stack depth per level (4, 6 or 8)
     JitSweeps jit;              // generated per-template sweeps (when enabled)
     int gemm_shape = 100;       // river GEMM form and shape (autotuned; see river_gemm)
@@ -1217,6 +1388,9 @@ int batched_create(SolverImpl& s) {
         rc = A.alloc((size_t)nb1 * NPR * NPR, &B.d_S8);
         if (rc) return rc;
         hipLaunchKernelGGL(k_bt_s8, dim3(NPR, nb1), dim3(256), 0, s.stream, B.D, B.d_S8);
+        rc = A.alloc((size_t)nb1 * GNCH * NPR * 2, &B.d_S2);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_bt_s2, dim3(NPR, nb1), dim3(GNCH * 2), 0, s.stream, B.D, B.d_S2);
         MTT_HIP(hipGetLastError());
         MTT_HIP(hipMemsetAsync(B.D.Bh, 0, nbc * 2, s.stream));
         MTT_HIP(hipMemsetAsync(B.D.Bl, 0, nbc * 2, s.stream));
@@ -1330,9 +1504,23 @@ int batched_create(SolverImpl& s) {
 
 // River showdown GEMM in one of its forms, by shape code: nw * 10 + pd for
 // the column-major form (k_bt_gemm5: NW column tiles per workgroup, PD K
-// chunks prefetched) or 100 for the hero-major form (k_bt_gemm6).
+// chunks prefetched), 100 for the hero-major form (k_bt_gemm6) or 101 for
+// the hero-major form reading S at two bits per pair (k_bt_gemm6s).
 int river_gemm(const BatchedState& B, int shape, hipStream_t st) {
     const int ntiles = B.D.L[1].ncolp / 32;
+    if (shape == 101) {
+        const int nlog = B.nb1 * G6_HB;
+        const dim3 g(((nlog + 7) / 8) * 8);
+        switch ((ntiles + 1) / 2) {
+        case 1: hipLaunchKernelGGL(k_bt_gemm6s<1>, g, dim3(512), 0, st, B.D, B.d_S2, nlog); break;
+        case 2: hipLaunchKernelGGL(k_bt_gemm6s<2>, g, dim3(512), 0, st, B.D, B.d_S2, nlog); break;
+        case 3: hipLaunchKernelGGL(k_bt_gemm6s<3>, g, dim3(512), 0, st, B.D, B.d_S2, nlog); break;
+        case 4: hipLaunchKernelGGL(k_bt_gemm6s<4>, g, dim3(512), 0, st, B.D, B.d_S2, nlog); break;
+        default: hipLaunchKernelGGL(k_bt_gemm6s<5>, g, dim3(512), 0, st, B.D, B.d_S2, nlog); break;
+        }
+        MTT_HIP(hipGetLastError());
+        return MTT_OK;
+    }
     if (shape == 100) {
         const int nlog = B.nb1 * G6_HB;
         const dim3 g(((nlog + 7) / 8) * 8);
@@ -1370,7 +1558,7 @@ int river_gemm(const BatchedState& B, int shape, hipStream_t st) {
 int river_gemm_autotune(BatchedState& B, hipStream_t st) {
     const int ntiles = B.D.L[1].ncolp / 32;
     auto valid = [&](int v) {
-        return v == 100 || (v / 10 >= 1 && v / 10 <= std::min(ntiles, 9) && v % 10 >= 2 && v % 10 <= 3);
+        return v == 100 || v == 101 || (v / 10 >= 1 && v / 10 <= std::min(ntiles, 9) && v % 10 >= 2 && v % 10 <= 3);
     };
     if (const char* e = getenv("MTT_RIVER_GEMM")) {
         const int v = atoi(e);
@@ -1379,7 +1567,7 @@ int river_gemm_autotune(BatchedState& B, hipStream_t st) {
     const std::string key = "river_gemm_shape/tiles" + std::to_string(ntiles);
     const int v = tune_lookup(key);
     if (valid(v)) { B.gemm_shape = v; return MTT_OK; }
-    std::vector<int> cand{100};
+    std::vector<int> cand{101, 100};
     for (int d = 1; d <= 3; ++d) {
         const int nw = (ntiles + d - 1) / d;
         for (int pd = 2; pd <= 3; ++pd)
