@@ -163,20 +163,34 @@ __device__ __forceinline__ void fwd_bins_flush(const LevelDev& L, const unsigned
 
 // Backward sweep, first part of the prologue: the card sums of the
 // template's nt terminals (both forward halves added) into ssum
-// ([nt][2][53], slot 52 left for the total) and this workgroup's showdown
-// values of its ncolt columns into Cs ([ncolt][NTB], Cs already offset by the
-// thread).  No barrier: the caller issues its own loads before bwd_totals.
-template <int LEVEL, int NTB>
-__device__ __forceinline__ void bwd_stage(const BatchedDev& D, const LevelDev& L, unsigned long long* ssum, float* Cs,
-                                          int b, int k, bool live, int nt, int tb, int cbase, int ncolt) {
+// ([nt][2][53], slot 52 left for the total).  No barrier.
+template <int NTB>
+__device__ __forceinline__ void bwd_sums(const LevelDev& L, unsigned long long* ssum, int b, int nt, int tb) {
     const unsigned long long* P = L.part + ((size_t)b * L.nterm + tb) * 208;
     for (int e = threadIdx.x; e < nt * 104; e += NTB) {
         const int tl = e / 104, r = e % 104;
         ssum[tl * 106 + (r / 52) * 53 + r % 52] = P[(size_t)tl * 208 + r] + P[(size_t)tl * 208 + 104 + r];
     }
+}
+
+// This thread's showdown values of the template's ncolt columns into Cs
+// ([ncolt][NTB], Cs already offset by the thread; each thread reads back only
+// its own entries, so no barrier is needed).
+template <int LEVEL, int NTB>
+__device__ __forceinline__ void bwd_cs(const BatchedDev& D, const LevelDev& L, float* Cs, int b, int k, bool live,
+                                       int cbase, int ncolt) {
     constexpr int cstride = LEVEL == 2 ? NPF : (LEVEL == 1 ? NPR : NPT);
     const float* Cb = (LEVEL >= 1 ? D.C : D.C0) + ((size_t)b * L.ncolp + cbase) * cstride + k;
     for (int x = 0; x < ncolt; ++x) Cs[x * NTB] = live ? Cb[(size_t)x * cstride] : 0.0f;
+}
+
+// Both parts (the interpreted kernels' prologue).  No barrier: the caller
+// issues its own loads before bwd_totals.
+template <int LEVEL, int NTB>
+__device__ __forceinline__ void bwd_stage(const BatchedDev& D, const LevelDev& L, unsigned long long* ssum, float* Cs,
+                                          int b, int k, bool live, int nt, int tb, int cbase, int ncolt) {
+    bwd_sums<NTB>(L, ssum, b, nt, tb);
+    bwd_cs<LEVEL, NTB>(D, L, Cs, b, k, live, cbase, ncolt);
 }
 
 // Backward sweep, second part: barrier, then every (terminal, player) total

@@ -31,14 +31,29 @@ struct JitLevel {
 };
 
 // Launch shapes of the generated sweeps.  The forward always runs two
-// workgroups of 576 per board (as the interpreted kernels); the backward runs
-// ceil(1152 / bwd_nt) workgroups of bwd_nt threads (576 or 384).  *_waves > 0
-// asks the compiler to fit that many waves per SIMD (amdgpu_waves_per_eu).
+// workgroups of 576 per board (as the interpreted kernels).  The turn
+// backward runs ceil(1152 / bwd_nt) workgroups of bwd_nt threads (576 or
+// 384) per (board, template); the river backward does too when
+// bwd_passes = 1, and with bwd_passes = 2 (the default) runs one workgroup of
+// 576 per (board, template) that walks the two halves of the combos in turn,
+// so the card sums of the template's terminals are staged and totalled once
+// instead of twice.  *_waves > 0 asks the compiler to fit that many waves per
+// SIMD (amdgpu_waves_per_eu).
 struct JitConfig {
     int bwd_nt = 576;
     int bwd_waves = 0;
     int fwd_waves = 0;
+    int bwd_passes = 2;
 };
+
+// Passes of the backward of a level (0: turn, 1: river) under cfg.
+inline int bwd_passes(int level, const JitConfig& cfg) { return level == 1 ? cfg.bwd_passes : 1; }
+
+// Workgroups per (board, template) of the backward of a level, of bwd_nt
+// threads each.
+inline int bwd_groups(int level, const JitConfig& cfg) {
+    return bwd_passes(level, cfg) == 2 ? 1 : (1152 + cfg.bwd_nt - 1) / cfg.bwd_nt;
+}
 
 struct JitSweeps {
     bool active = false;
@@ -61,7 +76,8 @@ int bt_jit_build(const JitLevel* levels, int n, const JitConfig& cfg, JitSweeps*
 std::string bt_jit_source(const JitLevel* levels, int n, const JitConfig& cfg);
 
 // The default launch shapes, with MTT_JIT_BWD_NT / MTT_JIT_BWD_WAVES /
-// MTT_JIT_FWD_WAVES overrides from the environment (invalid values ignored).
+// MTT_JIT_FWD_WAVES / MTT_JIT_BWD_PASSES overrides from the environment
+// (invalid values ignored).
 JitConfig bt_jit_config();
 
 // hipRTC compile of a generated source for gfx950 (no device needed).

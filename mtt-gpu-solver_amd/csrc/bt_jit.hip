@@ -195,13 +195,42 @@ void emit_up(Src& o, const Tmpl& T, int n, int p) {
     o("op%d += v%d_%d;", p, 1 - pp, n);
 }
 
-void gen_backward_case(Src& o, const JitLevel& J, int t) {
+void gen_backward_case(Src& o, const JitLevel& J, int passes, int t) {
     Tmpl T(J, t);
     o("case %d: {", t);
-    o("bwd_stage<LEVEL, NTB>(D, L, ssum, Cs, b, k, live, %d, %d, %d, %d);", T.nt, T.t0, T.c0, T.ncolt);
-    o("float* __restrict__ Rg = L.R + ((size_t)b * L.nslot + %d) * NP + k;", T.s0);
-    for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rg[%d] : 0.0f;", j, j * J.np);
-    o("bwd_totals<NTB>(ssum, %d);", T.nt);
+    if (passes == 2) {
+        // card sums and totals once; then each half of the combos in turn (the showdown values a thread stages
+        // in LDS are its own, so the passes need no barrier)
+        o("bwd_sums<NTB>(L, ssum, b, %d, %d);", T.nt, T.t0);
+        o("bwd_totals<NTB>(ssum, %d);", T.nt);
+        o("#pragma unroll 1");  // one walk's registers at a time
+        o("for (int pass = 0; pass < 2; ++pass) {");
+        // an offset the compiler cannot see is zero: the card-sum reads are not hoisted out of the pass loop
+        // (where they would all stay live across the walk)
+        o("int soff = 0;");
+        o("asm volatile(\"\" : \"+v\"(soff));");
+        o("const unsigned long long* ssv = ssum + soff;");
+        o("const int k = pass * NTB + tid;");
+        o("const bool live = k < nl;");
+        o("int c1 = 0, c2 = 0;");
+        o("float root0 = 0.0f, root1 = 0.0f;");
+        o("if (live) {");
+        o("c1 = L.cc[((size_t)b * NP + k) * 2];");
+        o("c2 = L.cc[((size_t)b * NP + k) * 2 + 1];");
+        o("root0 = root_reach(D, ranges, LEVEL, b, tm, NP, k, 0);");
+        o("root1 = root_reach(D, ranges, LEVEL, b, tm, NP, k, 1);");
+        o("}");
+        o("bwd_cs<LEVEL, NTB>(D, L, Cs, b, k, live, %d, %d);", T.c0, T.ncolt);
+        o("float* __restrict__ Rg = L.R + ((size_t)b * L.nslot + %d) * NP + k;", T.s0);
+        for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rg[%d] : 0.0f;", j, j * J.np);
+        o("float out0 = 0.0f, out1 = 0.0f;");
+    } else {
+        o("bwd_stage<LEVEL, NTB>(D, L, ssum, Cs, b, k, live, %d, %d, %d, %d);", T.nt, T.t0, T.c0, T.ncolt);
+        o("float* __restrict__ Rg = L.R + ((size_t)b * L.nslot + %d) * NP + k;", T.s0);
+        for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rg[%d] : 0.0f;", j, j * J.np);
+        o("bwd_totals<NTB>(ssum, %d);", T.nt);
+        o("const unsigned long long* ssv = ssum;");
+    }
     int stack[16];
     for (int e = T.e0; e < T.e1; ++e) {
         const int ev = (*J.events)[e], n = ev >> 2, kind = ev & 3;
@@ -226,15 +255,15 @@ void gen_backward_case(Src& o, const JitLevel& J, int t) {
                 const int w = nd.player;
                 const int off = (nd.sidx - T.t0) * 106;
                 const float cw = w == 0 ? nd.c0 : nd.c1, cl = w == 0 ? nd.c1 : nd.c0;
-                o("const float vw = %s * mass_u(ssum + %d, c1, c2, x%d_%d);", flit(nd.pot - cw).c_str(),
+                o("const float vw = %s * mass_u(ssv + %d, c1, c2, x%d_%d);", flit(nd.pot - cw).c_str(),
                   off + (1 - w) * 53, 1 - w, n);
-                o("const float vl = %s * mass_u(ssum + %d, c1, c2, x%d_%d);", flit(-cl).c_str(), off + w * 53, w, n);
+                o("const float vl = %s * mass_u(ssv + %d, c1, c2, x%d_%d);", flit(-cl).c_str(), off + w * 53, w, n);
                 o("v0_%d = live ? %s : 0.0f;", n, w == 0 ? "vw" : "vl");
                 o("v1_%d = live ? %s : 0.0f;", n, w == 0 ? "vl" : "vw");
             } else if (nd.type == bt::DN_SD) {
                 const int off = (nd.sidx - T.t0) * 106;
                 const int lc = nd.col - T.c0;
-                o("const float m0 = mass_u(ssum + %d, c1, c2, x1_%d), m1 = mass_u(ssum + %d, c1, c2, x0_%d);",
+                o("const float m0 = mass_u(ssv + %d, c1, c2, x1_%d), m1 = mass_u(ssv + %d, c1, c2, x0_%d);",
                   off + 53, n, off, n);
                 o("const float q0 = Cs[%d * NTB], q1 = Cs[%d * NTB];", lc, lc + 1);
                 if (J.level == 1) {
@@ -270,6 +299,10 @@ void gen_backward_case(Src& o, const JitLevel& J, int t) {
         o("v%d_%d = op%d;", 1 - pl, n, n);
         o("}");
         emit_up(o, T, n, p);
+    }
+    if (passes == 2) {
+        o("bwd_root_out<LEVEL>(D, L, b, tm, k, NP, live, out0, out1);");
+        o("}");
     }
     o("break;");
     o("}");
@@ -312,9 +345,25 @@ void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
       "const float* __restrict__ ranges, float wp, float wn) {", cfg.bwd_nt, waves_attr(cfg.bwd_waves).c_str(), L);
     o("using namespace mtt::bt;");
     o("constexpr int LEVEL = %d, NTB = %d, NP = %d;", L, cfg.bwd_nt, J.np);
-    o("constexpr int NH = (NPT + NTB - 1) / NTB;");
     o("const LevelDev& L = D.L[LEVEL];");
     o("extern __shared__ unsigned long long ssum[];");
+    const int passes = bwd_passes(J.level, cfg);
+    if (passes == 2) {
+        o("const int inst = blockIdx.x;");
+        o("const int b = inst / %d, tm = inst %% %d;", J.ntmpl, J.ntmpl);
+        o("const int tid = threadIdx.x;");
+        o("const int nl = ldu(L.nlive + b);");
+        o("float* Cs = reinterpret_cast<float*>(ssum + (size_t)L.maxterm * 106) + tid;");
+        o("const float hp0 = 0.5f * D.pot0;");
+        o("(void)hp0;");
+        o("switch (tm) {");
+        for (int t = 0; t < J.ntmpl; ++t) gen_backward_case(o, J, passes, t);
+        o("default: break;");
+        o("}");
+        o("}");
+        return;
+    }
+    o("constexpr int NH = (NPT + NTB - 1) / NTB;");
     o("const int inst = blockIdx.x / NH, half = blockIdx.x %% NH;");
     o("const int b = inst / %d, tm = inst %% %d;", J.ntmpl, J.ntmpl);
     o("const int tid = threadIdx.x;");
@@ -333,7 +382,7 @@ void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
     o("(void)hp0;");
     o("float out0 = 0.0f, out1 = 0.0f;");
     o("switch (tm) {");
-    for (int t = 0; t < J.ntmpl; ++t) gen_backward_case(o, J, t);
+    for (int t = 0; t < J.ntmpl; ++t) gen_backward_case(o, J, passes, t);
     o("default: break;");
     o("}");
     o("bwd_root_out<LEVEL>(D, L, b, tm, k, NP, live, out0, out1);");
@@ -368,6 +417,9 @@ JitConfig bt_jit_config() {
     if (c.bwd_nt != 384 && c.bwd_nt != 576) c.bwd_nt = 576;
     c.bwd_waves = env("MTT_JIT_BWD_WAVES", 0, 8, c.bwd_waves);
     c.fwd_waves = env("MTT_JIT_FWD_WAVES", 0, 8, c.fwd_waves);
+    if (c.bwd_nt != 576) c.bwd_passes = 1;  // two passes need 576 threads
+    c.bwd_passes = env("MTT_JIT_BWD_PASSES", 1, 2, c.bwd_passes);
+    if (c.bwd_passes == 2) c.bwd_nt = 576;
     return c;
 }
 
@@ -432,6 +484,8 @@ int bt_jit_build(const JitLevel* levels, int n, const JitConfig& cfg, JitSweeps*
     if (rc) return rc;
     MTT_ARG(cfg.bwd_nt == 384 || cfg.bwd_nt == 576, "generated sweeps: backward workgroup %d not 384 or 576",
             cfg.bwd_nt);
+    MTT_ARG(cfg.bwd_passes == 1 || (cfg.bwd_passes == 2 && cfg.bwd_nt == 576),
+            "generated sweeps: two-pass backward needs 576-thread workgroups");
     const std::string src = bt_jit_source(levels, n, cfg);
     int dev = 0;
     MTT_HIP(hipGetDevice(&dev));

@@ -1633,7 +1633,7 @@ int batched_run(SolverImpl& s, int iters) {
         }
         MTT_TIMED(s, "river_showdown_gemm", rc = river_gemm(B, B.gemm_shape, s.stream));
         {
-            const int jnt = B.jit.cfg.bwd_nt, jnh = (NPT + jnt - 1) / jnt;
+            const int jnt = B.jit.cfg.bwd_nt, jnh = bwd_groups(1, B.jit.cfg);
             const size_t sh = B.jit.active ? bwd_lds(L1, jnt) : bwd_lds(L1);
             const dim3 g(B.nb1 * L1.ntmpl * (B.jit.active ? jnh : BWD_NH));
             void* args[] = {&B.D, &ranges, &wp, &wn};
@@ -1649,7 +1649,7 @@ int batched_run(SolverImpl& s, int iters) {
             MTT_TIMED(s, "turn_showdown",
                       hipLaunchKernelGGL(k_bt_turn_sd<16>, dim3(NPT / 256 + 1, 49), dim3(256), 0, s.stream, B.D,
                                          L0.ncol));
-            const int jnt = B.jit.cfg.bwd_nt, jnh = (NPT + jnt - 1) / jnt;
+            const int jnt = B.jit.cfg.bwd_nt, jnh = bwd_groups(0, B.jit.cfg);
             const size_t sh = B.jit.active ? bwd_lds(L0, jnt) : bwd_lds(L0);
             const dim3 g(49 * L0.ntmpl * (B.jit.active ? jnh : BWD_NH));
             void* args[] = {&B.D, &ranges, &wp, &wn};
