@@ -42,18 +42,27 @@ def load(name, path):
     return mod
 
 
+def river_sweep_mode():
+    """The library's river showdown mode (environment, as the library reads it)."""
+    return not os.environ.get("MTT_RIVER_GEMM") and os.environ.get("MTT_RIVER_SD") == "sweep"
+
+
 def algorithmic_work(S, game):
     """Per-iteration algorithmic work of the river kernels of the batched path
     (n = 1081 live combos per river board, 49x48 boards).
 
-    River showdown GEMM: 2 flops per (hero, villain, column) over the live
-    combos of each board: 2 * n^2 * (2 * showdowns); the hi+lo split's
-    second product is not counted.
+    River showdown GEMM (default): 2 flops per (hero, villain, column) over
+    the live combos of each board: 2 * n^2 * (2 * showdowns); the hi+lo
+    split's second product is not counted.
+    River showdown, exact strength-order sweep (MTT_RIVER_SD=sweep): each
+    column's reach read and its values written (4 + 4 B per live combo) and
+    the column's 52 card totals read from both forward halves (832 B).
     River backward: regrets read and written (8 B per action slot and
     combo), showdown values read (4 B per column entry), card sums of each
     terminal read (2 players x 53 x 8 B).
     River forward: regrets read (4 B), strategy sums read-modify-write (8 B),
-    reach columns out as BF16 hi + lo (4 B per column entry), card sums out."""
+    reach columns out (4 B per column entry: FP32, or BF16 hi + lo for the
+    GEMM), card sums out."""
     nodes, ni, nm = S.tree(game)
     river = nodes["river_idx"] >= 64
     river_dec = int(((nodes["type"] == 0) & river).sum())
@@ -65,10 +74,12 @@ def algorithmic_work(S, game):
     n = 1081
     sums = river_terms * 2 * 53 * 8
     gemm_flops = 2.0 * n * n * (2 * river_sd)
+    showdown_bytes = 2 * river_sd * (n * 8 + 2 * 52 * 8)
     backward_bytes = river_actions * n * 8 + 2 * river_sd * n * 4 + sums
     forward_bytes = river_actions * n * (4 + 8) + 2 * river_sd * n * 4 + sums
     return dict(boards=boards, river_decisions=river_dec, river_showdowns=river_sd, river_terminals=river_terms,
-                river_actions=river_actions, gemm_flops=gemm_flops, backward_bytes=backward_bytes,
+                river_actions=river_actions, gemm_flops=gemm_flops, showdown_bytes=showdown_bytes,
+                backward_bytes=backward_bytes,
                 forward_bytes=forward_bytes)
 
 
@@ -181,24 +192,28 @@ def main():
     # ------------------------------------------------------- roofline ---
     work = algorithmic_work(S, game)
     per_iter = {k: v[0] / max(v[1], 1) for k, v in ktimes.items()}
-    gemm_ms = per_iter.get("river_showdown_gemm", float("nan"))
-    bwd_ms = per_iter.get("river_backward", float("nan"))
+    sd_ms = per_iter.get("river_showdown", float("nan"))
+    sweep = river_sweep_mode()
     dominant = max(per_iter, key=per_iter.get) if per_iter else None
-    if dominant == "river_showdown_gemm":
-        ach = work["gemm_flops"] / (gemm_ms * 1e-3) / 1e12
-        roof = {"kernel": dominant, "bound": "mfma", "achieved": ach, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / BF16_PEAK_TFLOPS, "traffic": hbm_traffic(dominant), "avg_ms": gemm_ms}
-    else:
-        ms = per_iter.get(dominant, float("nan"))
-        byt = work["backward_bytes"] if dominant == "river_backward" else work["forward_bytes"]
+
+    def hbm_roof(kernel, byt, ms):
         ach = byt / (ms * 1e-3) / 1e9
-        roof = {"kernel": dominant, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": hbm_traffic(dominant), "avg_ms": ms,
-                "algorithmic_bytes": byt}
-    gemm_roof = {"kernel": "river_showdown_gemm", "bound": "mfma",
-                 "achieved": work["gemm_flops"] / (gemm_ms * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS,
-                 "unit": "TFLOP/s", "avg_ms": gemm_ms}
-    gemm_roof["frac"] = gemm_roof["achieved"] / BF16_PEAK_TFLOPS
+        return {"kernel": kernel, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": hbm_traffic(kernel), "avg_ms": ms, "algorithmic_bytes": byt}
+
+    def mfma_roof(kernel, ms):
+        ach = work["gemm_flops"] / (ms * 1e-3) / 1e12
+        return {"kernel": kernel, "bound": "mfma", "achieved": ach, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / BF16_PEAK_TFLOPS, "traffic": hbm_traffic(kernel), "avg_ms": ms}
+
+    byts = {"river_backward": work["backward_bytes"], "river_forward": work["forward_bytes"],
+            "river_showdown": work["showdown_bytes"]}
+    if dominant == "river_showdown" and not sweep:
+        roof = mfma_roof(dominant, sd_ms)
+    else:
+        roof = hbm_roof(dominant, byts.get(dominant, float("nan")), per_iter.get(dominant, float("nan")))
+    sd_roof = hbm_roof("river_showdown", work["showdown_bytes"], sd_ms) if sweep else mfma_roof("river_showdown", sd_ms)
+    sd_roof["form"] = "strength-order sweep" if sweep else "sign x blocker GEMM"
 
     cpu = None
     if not args.no_cpu:
@@ -219,7 +234,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16",
+        "dtype": "fp32" if sweep else "bf16",
         "data": "synthetic (seeded board and Dirichlet(0.5) ranges)",
         "config": {"workload": "hu_turn_river_cfr_plus", "seed": args.seed, "board": [int(c) for c in game.board[:3]],
                    "pot_bb": 6, "stack_bb": 40, "bets": "33%/75% + 2.5x raise + all-in per street",
@@ -231,7 +246,7 @@ def main():
         "plain_fp32": plain,
         "speedup_vs_plain_fp32": (ips_b / plain["iter_per_s"]) if plain else None,
         "roofline": roof,
-        "roofline_gemm": gemm_roof,
+        "roofline_showdown": sd_roof,
         "work_per_iter": work,
         "cpu_baseline": cpu,
     }

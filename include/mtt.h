@@ -291,11 +291,29 @@ int mtt_solver_jit_info(mtt_solver* h, int* active, double* compile_seconds);
  * full length to *len. */
 int mtt_hu_sweep_source(const mtt_hu_game* game, char* buf, size_t cap, size_t* len);
 int mtt_hu_sweep_compile(const mtt_hu_game* game, size_t* code_bytes, double* seconds);
+/* River showdown values of the batched mode (no counterpart in the
+ * reference: it multiplies by equity matrices,
+ * honest_multistreet_solver.cu:128-196; these two are for inspection and
+ * tests).  The batched mode computes them by a sign x blocker GEMM, or, with
+ * MTT_RIVER_SD=sweep in the environment at create, by an exact sweep in
+ * strength order driven by a per-board program.  mtt_river_sweep_program writes the sweep program of one river
+ * board -- n live combos in strength order, cards[2 * n], non-decreasing
+ * group ranks grank[n] -- as 16-word records: at most cap words to out (may
+ * be null), the full length to *len.  mtt_solver_river_showdown copies, for
+ * river board `board` of a batched handle, the last iteration's showdown
+ * columns (opponent reach, reach[ncol][1088]) and values (values[ncol][1088],
+ * weaker minus stronger non-blocked reach) and the board's combo order
+ * (cards[1088][2], grank[1088]); *nlive and *ncol receive the live combo and
+ * column counts.  Null arrays query the counts only. */
+int mtt_river_sweep_program(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* out, size_t cap,
+                            size_t* len);
+int mtt_solver_river_showdown(mtt_solver* h, int board, float* reach, float* values, uint8_t* cards,
+                              uint16_t* grank, int* nlive, int* ncol);
 int mtt_solver_destroy(mtt_solver* h);
 
 /* ----------------------------------------------- batched flop subgames -- */
 /* Many independent heads-up flop subgames solved together (new: the
- * reference has no batch driver; SURVEY's mtt_batch_solve).  Every instance
+ * reference has no batch driver).  Every instance
  * is a game with n_board = 3, root_chance = 0, last_street = 0 (flop
  * betting, showdown over the 990 turn/river runouts) and its own board,
  * pot, stack and menu[0].  ranges: [n][2][1326] (global combo order).

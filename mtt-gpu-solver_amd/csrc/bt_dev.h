@@ -91,6 +91,7 @@ struct BatchedDev {
     bf16_t* Bh;                // [boards1][ncolp][NPR]
     bf16_t* Bl;
     float* C;                  // [boards1][ncolp][NPR]
+    float* Bf;                 // [boards1][ncolp][NPR] showdown columns in FP32 (strength-order sweep), else null
     const uint16_t* grank;     // [boards1][NPR] strength group, ascending
     const uint32_t* cx;        // [boards1][NPR] c1|c2<<8|c1<<16|c2<<24
     const uint32_t* cy;        // [boards1][NPR] c1|c1<<8|c2<<16|c2<<24
@@ -99,7 +100,7 @@ struct BatchedDev {
     const float* Eturn;        // [49][NPT][NPT] villain-major
     float pot0;
 };
-static_assert(sizeof(BatchedDev) == 608, "BatchedDev layout");
+static_assert(sizeof(BatchedDev) == 616, "BatchedDev layout");
 
 typedef const __attribute__((address_space(4))) int* cint_p;
 

@@ -44,6 +44,8 @@ struct JitConfig {
     int bwd_waves = 0;
     int fwd_waves = 0;
     int bwd_passes = 2;
+    // river showdown columns: 0 = BF16 hi + lo (Bh, Bl) for the GEMM, 1 = FP32 (Bf) for the strength-order sweep
+    int sd_fp32 = 0;
 };
 
 // Passes of the backward of a level (0: turn, 1: river) under cfg.

@@ -125,7 +125,7 @@ void emit_bins(Src& o, const Tmpl& T, int n) {
     o("}");
 }
 
-void gen_forward_case(Src& o, const JitLevel& J, int t) {
+void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int t) {
     Tmpl T(J, t);
     o("case %d: {", t);
     o("fwd_bins_clear<NT>(bins, %d);", T.nt);
@@ -165,8 +165,13 @@ void gen_forward_case(Src& o, const JitLevel& J, int t) {
             o("const float o0 = live ? x1_%d : 0.0f, o1 = live ? x0_%d : 0.0f;", n, n);
             if (J.level == 1) {
                 o("const size_t cb = ((size_t)b * L.ncolp + %d) * NPR + k;", nd.col);
-                o("store_split(D.Bh, D.Bl, cb, o0);");
-                o("store_split(D.Bh, D.Bl, cb + NPR, o1);");
+                if (sd_fp32) {
+                    o("D.Bf[cb] = o0;");
+                    o("D.Bf[cb + NPR] = o1;");
+                } else {
+                    o("store_split(D.Bh, D.Bl, cb, o0);");
+                    o("store_split(D.Bh, D.Bl, cb + NPR, o1);");
+                }
             } else {
                 o("const size_t cb = ((size_t)b * L.ncolp + %d) * NPT + k;", nd.col);
                 o("D.B0[cb] = o0;");
@@ -336,7 +341,7 @@ void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
     o("}");
     o("(void)inb;");
     o("switch (tm) {");
-    for (int t = 0; t < J.ntmpl; ++t) gen_forward_case(o, J, t);
+    for (int t = 0; t < J.ntmpl; ++t) gen_forward_case(o, J, cfg.sd_fp32 != 0, t);
     o("default: break;");
     o("}");
     o("}");

@@ -1,0 +1,57 @@
+// River showdown values of the batched heads-up path by an exact sweep in
+// strength order (the alternative to the sign x blocker GEMM).
+//
+// For a river board whose live combos are sorted by strength, a showdown
+// column r (the opponent's reach over the combos) gives hero i (cards a, b,
+// tie group [gs, ge)) the value
+//     C[i] = sum of r_j over non-blocked j in weaker groups
+//            - sum of r_j over non-blocked j in stronger groups,
+// the same quantity as S r with S = sign x blocker (the GEMM's product).
+// With r in 2^48 fixed point (R_j), prefix sums P(x) = sum_{j<x} R_j and
+// per-card prefix sums cs_c(x) = sum_{j<x, c in j} R_j:
+//     C[i] = P(gs) + P(ge) - Ptot - sum_{c in {a,b}} (cs_c(gs) + cs_c(ge) - csT_c)
+// exactly (integers), where Ptot, csT_c are the column's totals -- the card
+// sums the forward sweep already accumulates per terminal.
+//
+// One wave sweeps 64 columns of one board (lane = column), so the card
+// indices of each step are wave-uniform and the 52 running card sums of a
+// lane sit in LDS at [card][lane].  A per-board program (built on the host
+// from the board's strength order) drives the sweep: singleton groups read
+// their two card sums and add their reach; a tie group saves the card sums
+// it touches, adds half of every member's reach (then every member reads
+// exactly cs(gs) + cs(ge)), and restores the sums to old + 2 x half.
+// Members are packed into batches whose cards are pairwise distinct, so a
+// batch costs one LDS round trip.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "bt_dev.h"
+
+namespace mtt {
+namespace bt {
+
+// Appends the sweep program of one board (16-word records, padded to whole
+// chunks of SD_CHUNK records): n live combos in strength order, cards
+// c[2 * j], c[2 * j + 1] and group ranks g[j] (non-decreasing).  Returns
+// false if n exceeds NPR, a card is invalid or the ranks are not sorted.
+bool sd_program(const uint8_t* cards, const uint16_t* grank, int n, std::vector<uint32_t>* prog);
+
+// Words per program record; records per chunk the kernel stages in LDS.
+constexpr int SD_REC = 16;
+constexpr int SD_CHUNK = 16;
+// Records the kernel may read past the end of the last program.
+constexpr int SD_PAD_RECORDS = 2 * SD_CHUNK;
+
+// Launches the sweep for every board of level 1: columns from D.Bf, card
+// totals from the forward's card sums (L.part), values to D.C.
+// prog / poff: the concatenated programs (followed by SD_PAD_RECORDS zero
+// records) and each board's first record;
+// colinfo[col] = terminal | player << 31 of each showdown column.
+void sd_launch(const BatchedDev& D, const uint32_t* prog, const uint32_t* poff, const int* colinfo, int nboards,
+               int ncol, hipStream_t st);
+
+}  // namespace bt
+}  // namespace mtt

@@ -28,6 +28,7 @@
 
 #include "bt_dev.h"
 #include "bt_jit.h"
+#include "bt_showdown.h"
 #include "solver.h"
 
 namespace mtt {
@@ -190,8 +191,13 @@ __global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __
             if (LEVEL >= 1) {
                 constexpr int NX = LEVEL == 2 ? NPF : NPR;
                 const size_t cb = ((size_t)b * L.ncolp + (nd.col - cb0)) * NX + k;
-                store_split(D.Bh, D.Bl, cb, o0);
-                store_split(D.Bh, D.Bl, cb + NX, o1);
+                if (LEVEL == 1 && D.Bf) {
+                    D.Bf[cb] = o0;
+                    D.Bf[cb + NX] = o1;
+                } else {
+                    store_split(D.Bh, D.Bl, cb, o0);
+                    store_split(D.Bh, D.Bl, cb + NX, o1);
+                }
             } else {
                 const size_t cb = ((size_t)b * L.ncolp + nd.col) * NPT + k;
                 D.B0[cb] = o0;
@@ -988,7 +994,7 @@ static size_t bwd_lds(const LevelDev& L, int nt = BWD_NT) {
 }
 
 struct BatchedState {
-    BatchedDev D;
+    BatchedDev D{};            // value-initialised: pointers a mode does not use stay null
     DeviceArena arena;
     int two_level = 0;
     int nb1 = 0;          // river boards
@@ -1001,6 +1007,10 @@ struct BatchedState {
     int dm[2] = {8, 8};         // register-stack depth per level (4, 6 or 8)
     JitSweeps jit;              // generated per-template sweeps (when enabled)
     int gemm_shape = 100;       // river GEMM form and shape (autotuned; see river_gemm)
+    bool sd_sweep = false;      // river showdown by the strength-order sweep (else the GEMM)
+    uint32_t* d_sdprog = nullptr;   // sweep programs of the river boards (SD_REC-word records)
+    uint32_t* d_sdoff = nullptr;    // [boards1] first record of each board's program
+    int* d_colinfo = nullptr;       // [ncol] terminal | player << 31 of each river showdown column
 };
 
 void batched_destroy(BatchedState* b) { delete b; }
@@ -1262,6 +1272,16 @@ std::vector<JitLevel> jit_levels(const LevelHost& H0, const LevelHost& H1, bool 
 
 int river_gemm_autotune(BatchedState& B, hipStream_t st);
 
+// River showdown by the sign x blocker GEMM (autotuned form, or
+// MTT_RIVER_GEMM=<shape code>), or by the exact strength-order sweep when
+// MTT_RIVER_SD=sweep (and no GEMM form is forced).
+bool river_sweep_mode() {
+    const char* g = getenv("MTT_RIVER_GEMM");
+    if (g && *g) return false;
+    const char* m = getenv("MTT_RIVER_SD");
+    return m && strcmp(m, "sweep") == 0;
+}
+
 int batched_create(SolverImpl& s) {
     const mtt_hu_game& g = s.game;
     const HuTree& tr = s.tree;
@@ -1381,19 +1401,45 @@ int batched_create(SolverImpl& s) {
         if (rc) return rc;
         B.D.grank = d_rank; B.D.cx = d_cx; B.D.cy = d_cy; B.D.rmap = d_rmap;
         size_t nbc = (size_t)nb1 * L1.ncolp * NPR;
-        rc = rc ? rc : A.alloc(nbc, &B.D.Bh);
-        rc = rc ? rc : A.alloc(nbc, &B.D.Bl);
-        rc = rc ? rc : A.alloc(nbc, &B.D.C);
+        B.sd_sweep = river_sweep_mode();
+        rc = A.alloc(nbc, &B.D.C);
         if (rc) return rc;
-        rc = A.alloc((size_t)nb1 * NPR * NPR, &B.d_S8);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_bt_s8, dim3(NPR, nb1), dim3(256), 0, s.stream, B.D, B.d_S8);
-        rc = A.alloc((size_t)nb1 * GNCH * NPR * 2, &B.d_S2);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_bt_s2, dim3(NPR, nb1), dim3(GNCH * 2), 0, s.stream, B.D, B.d_S2);
-        MTT_HIP(hipGetLastError());
-        MTT_HIP(hipMemsetAsync(B.D.Bh, 0, nbc * 2, s.stream));
-        MTT_HIP(hipMemsetAsync(B.D.Bl, 0, nbc * 2, s.stream));
+        if (B.sd_sweep) {
+            // FP32 showdown columns, the per-board sweep programs and the column -> (terminal, player) map
+            rc = A.alloc(nbc, &B.D.Bf);
+            if (rc) return rc;
+            MTT_HIP(hipMemsetAsync(B.D.Bf, 0, nbc * 4, s.stream));
+            std::vector<uint32_t> prog, poff(nb1);
+            for (int bi = 0; bi < nb1; ++bi) {
+                poff[bi] = (uint32_t)(prog.size() / SD_REC);
+                MTT_ARG(sd_program(&cc1[(size_t)bi * NPR * 2], &rank1[(size_t)bi * NPR], nlive1[bi], &prog),
+                        "river board %d: combos not in strength order", bi);
+            }
+            prog.resize(prog.size() + (size_t)SD_PAD_RECORDS * SD_REC, 0u);
+            std::vector<int> ci(std::max(L1.ncol, 1), 0);
+            for (const DNode& d : H1.nodes)
+                if (d.type == DN_SD) {
+                    ci[d.col] = d.sidx;
+                    ci[d.col + 1] = (int)((uint32_t)d.sidx | 0x80000000u);
+                }
+            rc = A.upload(prog, &B.d_sdprog);
+            rc = rc ? rc : A.upload(poff, &B.d_sdoff);
+            rc = rc ? rc : A.upload(ci, &B.d_colinfo);
+            if (rc) return rc;
+        } else {
+            rc = rc ? rc : A.alloc(nbc, &B.D.Bh);
+            rc = rc ? rc : A.alloc(nbc, &B.D.Bl);
+            if (rc) return rc;
+            rc = A.alloc((size_t)nb1 * NPR * NPR, &B.d_S8);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_bt_s8, dim3(NPR, nb1), dim3(256), 0, s.stream, B.D, B.d_S8);
+            rc = A.alloc((size_t)nb1 * GNCH * NPR * 2, &B.d_S2);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_bt_s2, dim3(NPR, nb1), dim3(GNCH * 2), 0, s.stream, B.D, B.d_S2);
+            MTT_HIP(hipGetLastError());
+            MTT_HIP(hipMemsetAsync(B.D.Bh, 0, nbc * 2, s.stream));
+            MTT_HIP(hipMemsetAsync(B.D.Bl, 0, nbc * 2, s.stream));
+        }
         MTT_HIP(hipMemsetAsync(B.D.C, 0, nbc * 4, s.stream));
         rc = A.upload(inv1, &B.d_inv1);
         if (rc) return rc;
@@ -1493,11 +1539,15 @@ int batched_create(SolverImpl& s) {
     // ------------------------------------------------ generated sweeps --
     if (bt_jit_enabled()) {
         const std::vector<JitLevel> lv = jit_levels(H0, H1, two);
-        rc = bt_jit_build(lv.data(), (int)lv.size(), bt_jit_config(), &B.jit);
+        JitConfig jc = bt_jit_config();
+        jc.sd_fp32 = B.sd_sweep ? 1 : 0;
+        rc = bt_jit_build(lv.data(), (int)lv.size(), jc, &B.jit);
         if (rc) return rc;
     }
-    rc = river_gemm_autotune(B, s.stream);
-    if (rc) return rc;
+    if (!B.sd_sweep) {
+        rc = river_gemm_autotune(B, s.stream);
+        if (rc) return rc;
+    }
     MTT_HIP(hipStreamSynchronize(s.stream));
     return MTT_OK;
 }
@@ -1631,7 +1681,12 @@ int batched_run(SolverImpl& s, int iters) {
                       else if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_forward<1, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
                       else hipLaunchKernelGGL((k_bt_forward<1, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
-        MTT_TIMED(s, "river_showdown_gemm", rc = river_gemm(B, B.gemm_shape, s.stream));
+        if (B.sd_sweep) {
+            MTT_TIMED(s, "river_showdown",
+                      sd_launch(B.D, B.d_sdprog, B.d_sdoff, B.d_colinfo, B.nb1, B.D.L[1].ncol, s.stream));
+        } else {
+            MTT_TIMED(s, "river_showdown", rc = river_gemm(B, B.gemm_shape, s.stream));
+        }
         {
             const int jnt = B.jit.cfg.bwd_nt, jnh = bwd_groups(1, B.jit.cfg);
             const size_t sh = B.jit.active ? bwd_lds(L1, jnt) : bwd_lds(L1);
@@ -1686,6 +1741,42 @@ int batched_jit_info(const SolverImpl& s, int* active, double* compile_s) {
     return MTT_OK;
 }
 
+// The last iteration's river showdown columns and values of one board, with
+// the board's combo order (for inspection and tests).
+int batched_river_showdown(SolverImpl& s, int board, float* reach, float* values, uint8_t* cards, uint16_t* grank,
+                           int* nlive, int* ncol) {
+    MTT_ARG(s.batched, "river showdown: not a batched handle");
+    BatchedState& B = *s.batched;
+    const LevelDev& L1 = B.D.L[1];
+    MTT_ARG(board >= 0 && board < B.nb1, "river showdown: board %d outside [0, %d)", board, B.nb1);
+    MTT_HIP(hipStreamSynchronize(s.stream));
+    int nl = 0;
+    MTT_HIP(hipMemcpy(&nl, L1.nlive + board, sizeof(int), hipMemcpyDeviceToHost));
+    if (nlive) *nlive = nl;
+    if (ncol) *ncol = L1.ncol;
+    const size_t n = (size_t)L1.ncol * NPR, o = (size_t)board * L1.ncolp * NPR;
+    if (values) MTT_HIP(hipMemcpy(values, B.D.C + o, n * 4, hipMemcpyDeviceToHost));
+    if (reach) {
+        if (B.D.Bf) {
+            MTT_HIP(hipMemcpy(reach, B.D.Bf + o, n * 4, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<uint16_t> hi(n), lo(n);
+            MTT_HIP(hipMemcpy(hi.data(), B.D.Bh + o, n * 2, hipMemcpyDeviceToHost));
+            MTT_HIP(hipMemcpy(lo.data(), B.D.Bl + o, n * 2, hipMemcpyDeviceToHost));
+            for (size_t x = 0; x < n; ++x) {
+                uint32_t uh = (uint32_t)hi[x] << 16, ul = (uint32_t)lo[x] << 16;
+                float fh, fl;
+                memcpy(&fh, &uh, 4);
+                memcpy(&fl, &ul, 4);
+                reach[x] = fh + fl;
+            }
+        }
+    }
+    if (cards) MTT_HIP(hipMemcpy(cards, L1.cc + (size_t)board * NPR * 2, NPR * 2, hipMemcpyDeviceToHost));
+    if (grank) MTT_HIP(hipMemcpy(grank, B.D.grank + (size_t)board * NPR, NPR * 2, hipMemcpyDeviceToHost));
+    return MTT_OK;
+}
+
 // Generated sweep source of a game's street templates (no device needed).
 int hu_sweep_source(const mtt_hu_game& g, std::string* src) {
     HuTree tr;
@@ -1696,7 +1787,9 @@ int hu_sweep_source(const mtt_hu_game& g, std::string* src) {
     rc = hu_levels(g, tr, two, H0, H1);
     if (rc) return rc;
     const std::vector<JitLevel> lv = jit_levels(H0, H1, two);
-    *src = bt_jit_source(lv.data(), (int)lv.size(), bt_jit_config());
+    JitConfig jc = bt_jit_config();
+    jc.sd_fp32 = river_sweep_mode() ? 1 : 0;
+    *src = bt_jit_source(lv.data(), (int)lv.size(), jc);
     return MTT_OK;
 }
 
@@ -1887,7 +1980,7 @@ struct BatchImpl {
     hipStream_t stream = nullptr;
     KernelTimer timer;
     DeviceArena arena;
-    BatchedDev D;
+    BatchedDev D{};            // value-initialised: pointers a mode does not use stay null
     float* d_ranges = nullptr;
     int16_t* d_S16 = nullptr;        // W - L, [instance][chunk][hero][32]
     int* d_bncol = nullptr;
@@ -2188,6 +2281,18 @@ struct mtt_batch {
 };
 
 extern "C" {
+
+int mtt_river_sweep_program(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* out, size_t cap,
+                            size_t* len) {
+    using namespace mtt;
+    MTT_ARG(cards && grank && len, "mtt_river_sweep_program: null argument");
+    std::vector<uint32_t> prog;
+    MTT_ARG(bt::sd_program(cards, grank, n, &prog), "mtt_river_sweep_program: %d combos, bad cards or unsorted ranks",
+            n);
+    *len = prog.size();
+    if (out) memcpy(out, prog.data(), std::min(cap, prog.size()) * 4);
+    return MTT_OK;
+}
 
 int mtt_hu_sweep_source(const mtt_hu_game* game, char* buf, size_t cap, size_t* len) {
     using namespace mtt;

@@ -103,6 +103,8 @@ void iteration_weights(const mtt_solver_cfg& cfg, int t, float& wp, float& wn, f
 int batched_create(SolverImpl& s);
 int batched_run(SolverImpl& s, int iters);
 int batched_jit_info(const SolverImpl& s, int* active, double* compile_s);
+int batched_river_showdown(SolverImpl& s, int board, float* reach, float* values, uint8_t* cards, uint16_t* grank,
+                           int* nlive, int* ncol);
 int batched_strategy_sums(SolverImpl& s, float* d_S_ref);  // scatter into the reference layout
 int batched_export(SolverImpl& s, float* d_out, int which);  // 0 strategy sums, 1 regrets
 void batched_destroy(BatchedState* b);

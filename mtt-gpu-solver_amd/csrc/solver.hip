@@ -475,6 +475,12 @@ int mtt_solver_jit_info(mtt_solver* h, int* active, double* compile_seconds) {
     return mtt::batched_jit_info(h->impl, active, compile_seconds);
 }
 
+int mtt_solver_river_showdown(mtt_solver* h, int board, float* reach, float* values, uint8_t* cards, uint16_t* grank,
+                              int* nlive, int* ncol) {
+    MTT_ARG(h, "mtt_solver_river_showdown: null handle");
+    return mtt::batched_river_showdown(h->impl, board, reach, values, cards, grank, nlive, ncol);
+}
+
 int mtt_solver_destroy(mtt_solver* h) {
     delete h;
     return MTT_OK;

@@ -16,6 +16,7 @@ LAYOUT_GLOBAL1326 = 0
 LAYOUT_ALIVE1216 = 1
 N_ALIVE = 1176
 N_PAD = 1216
+NPR = 1088          # river combos per board in the batched layout (1081 live, padded)
 MAX_ACTIONS = 4
 
 NODE_DECISION, NODE_FOLD, NODE_SHOWDOWN, NODE_CHANCE = 0, 1, 2, 3

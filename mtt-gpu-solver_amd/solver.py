@@ -56,6 +56,8 @@ for name, args in {
     "mtt_solver_jit_info": [_P, _P, _P],
     "mtt_hu_sweep_source": [_P, _P, ctypes.c_size_t, _P],
     "mtt_hu_sweep_compile": [_P, _P, _P],
+    "mtt_solver_river_showdown": [_P, ctypes.c_int, _P, _P, _P, _P, _P, _P],
+    "mtt_river_sweep_program": [_P, _P, ctypes.c_int, _P, ctypes.c_size_t, _P],
     "mtt_solver_get_state": [_P, _P, _P, ctypes.c_size_t],
     "mtt_solver_timing": [_P, _I],
     "mtt_solver_kernel_times": [_P, _I, ctypes.c_char_p, _I, _P, _P],
@@ -186,6 +188,21 @@ class Solver:
         a, t = ctypes.c_int(), ctypes.c_double()
         mtt.call("mtt_solver_jit_info", self.h, ctypes.byref(a), ctypes.byref(t))
         return bool(a.value), t.value
+
+    def river_showdown(self, board):
+        """Last iteration's river showdown of one board (batched mode):
+        (reach [ncol][1088], values [ncol][1088], cards [1088][2],
+        group ranks [1088], live combo count)."""
+        nl, nc = ctypes.c_int(), ctypes.c_int()
+        mtt.call("mtt_solver_river_showdown", self.h, board, None, None, None, None, ctypes.byref(nl),
+                 ctypes.byref(nc))
+        reach = np.zeros((nc.value, mtt.NPR), np.float32)
+        vals = np.zeros((nc.value, mtt.NPR), np.float32)
+        cards = np.zeros((mtt.NPR, 2), np.uint8)
+        grank = np.zeros(mtt.NPR, np.uint16)
+        mtt.call("mtt_solver_river_showdown", self.h, board, reach.ctypes.data, vals.ctypes.data, cards.ctypes.data,
+                 grank.ctypes.data, ctypes.byref(nl), ctypes.byref(nc))
+        return reach, vals, cards, grank, nl.value
 
     @property
     def iterations(self):

@@ -182,10 +182,14 @@ def test_dropin_kernels_manual_iteration(solver_api, oracle, torch_dev, mtt):
         assert np.abs(Sg - Sc).max() < 1e-4 * max(1.0, np.abs(Sc).max()), t
 
 
-def test_batched_config1_matches_oracle(solver_api, oracle, torch_dev):
-    """BF16 hi+lo split showdown GEMM path on the river subgame."""
+@pytest.mark.parametrize("river", ["gemm", "sweep"])
+def test_batched_config1_matches_oracle(solver_api, oracle, torch_dev, river, monkeypatch):
+    """Batched path on the river subgame, river showdown by the BF16 hi+lo
+    GEMM (default) or by the exact strength-order sweep."""
+    monkeypatch.delenv("MTT_RIVER_GEMM", raising=False)
+    monkeypatch.setenv("MTT_RIVER_SD", river)
     g, r0, r1 = solver_api.config1(0)
-    print("config1 batched bf16-split, seed 0", flush=True)
+    print(f"config1 batched, river showdown {river}, seed 0", flush=True)
     s = solver_api.Solver(g, r0, r1, solver_api.MODE_BF16_SPLIT)
     h32, nodes, ni = _setup(solver_api, oracle, g, r0, r1, 1)
     s.run(200)
@@ -276,6 +280,41 @@ def test_batched_config3_matches_oracle(solver_api, oracle, torch_dev):
     print(f"iteration 2: {len(bad)} strategy sums beyond tolerance", flush=True)
     assert len(bad) <= 1e-7 * Sc.size
     assert amb[bad[:, 2]].all()
+    s.close()
+
+
+@pytest.mark.parametrize("which", ["config1", "config3_small", "config3"])
+def test_river_sweep_matches_dense_showdown(solver_api, torch_dev, which, monkeypatch):
+    """River showdown values by the strength-order sweep (MTT_RIVER_SD=sweep)
+    against S r of the same columns in FP64, S = sign x blocker of the
+    board's strength order (the reference's showdown, equity matrix x reach,
+    honest_multistreet_solver.cu:128-196, as 2 (E - 1/2) r).  The sweep sums
+    in exact 2^48 fixed point and rounds once, so every value agrees to FP32
+    rounding (|err| <= 2^-23 |exact| + 1e-12); padded combos stay 0."""
+    monkeypatch.delenv("MTT_RIVER_GEMM", raising=False)
+    monkeypatch.setenv("MTT_RIVER_SD", "sweep")
+    S = solver_api
+    g, r0, r1 = getattr(S, which)(1 if which == "config3_small" else 0)
+    s = S.Solver(g, r0, r1, S.MODE_BF16_SPLIT)
+    s.run(3)
+    nb = 1 if which == "config1" else 49 * 48
+    for board in sorted({0, nb // 2, nb - 1}):
+        reach, vals, cards, grank, nl = s.river_showdown(board)
+        c = cards[:nl].astype(np.int64)
+        gr = grank[:nl].astype(np.int64)
+        sign = np.sign(gr[:, None] - gr[None, :]).astype(np.float64)
+        cm = np.zeros((nl, 52))
+        cm[np.arange(nl), c[:, 0]] = 1.0
+        cm[np.arange(nl), c[:, 1]] = 1.0
+        Sm = np.where(cm @ cm.T > 0, 0.0, sign)
+        R = np.rint(reach[:, :nl].astype(np.float64) * 2.0 ** 48)
+        exact = (R @ Sm.T) * 2.0 ** -48
+        err = np.abs(vals[:, :nl].astype(np.float64) - exact)
+        print(f"{which} board {board}: {nl} combos, {reach.shape[0]} columns, max |value| {np.abs(exact).max():.4g}, "
+              f"max err {err.max():.3g}", flush=True)
+        assert np.abs(exact).max() > 0
+        assert np.all(err <= 2.0 ** -23 * np.abs(exact) + 1e-12), (board, float(err.max()))
+        assert not np.any(vals[:, nl:])
     s.close()
 
 

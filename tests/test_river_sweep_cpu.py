@@ -1,0 +1,197 @@
+"""River showdown sweep programs (host side, no GPU).
+
+The batched path computes river showdown values C = S r -- S the
+sign x blocker matrix of the board's strength order, as in the reference's
+terminal evaluation (honest_multistreet_solver.cu:128-196, equity matrix x
+reach) -- by a sweep in strength order driven by a per-board program of
+16-word records.  These tests replay the library's programs with the
+kernel's semantics (SINGLE and READ batches issue all their reads, then
+their updates in member order; HALF batches add atomically, a tie group in
+two passes around its READs; rows come from the current 32-row block of the
+columns, blocks in the order the program requests them; values go to a
+16-row output tile flushed by the FLUSH flag) on integer reach in 2^48
+fixed point, and require the exact integers of the dense product: every
+hero's (weaker - stronger) non-blocked reach, on evaluator-ranked river
+boards and on synthetic extremes (no combos, one tie group, no ties, random
+group sizes)."""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+
+REC = 16
+OP_NOP, OP_SINGLE, OP_HALF, OP_READ, OP_END = range(5)
+F_ADV, F_FLUSH = 1 << 8, 1 << 9
+RB, OB = 32, 16           # ring block rows, output block rows
+NO_BLOCK = 0xFF
+
+
+@pytest.fixture(scope="module")
+def lib(mtt):
+    mtt.lib()  # raises if the library has not been built
+    L = ctypes.CDLL(mtt.LIB_PATH)  # own handle: argtypes below stay local
+    L.mtt_eval7.restype = ctypes.c_uint32
+    L.mtt_eval7.argtypes = [ctypes.c_char_p]
+    L.mtt_river_sweep_program.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    return L
+
+
+def program(lib, cards, grank):
+    n = len(grank)
+    c = np.ascontiguousarray(cards, np.uint8).reshape(-1)
+    g = np.ascontiguousarray(grank, np.uint16)
+    ln = ctypes.c_size_t()
+    assert lib.mtt_river_sweep_program(c.ctypes.data, g.ctypes.data, n, None, 0, ctypes.byref(ln)) == 0
+    out = np.zeros(ln.value, np.uint32)
+    assert lib.mtt_river_sweep_program(c.ctypes.data, g.ctypes.data, n, out.ctypes.data, ln.value,
+                                       ctypes.byref(ln)) == 0
+    assert ln.value % REC == 0
+    return out.reshape(-1, REC)
+
+
+def replay(prog, cards, n, R):
+    """The kernel's arithmetic on int64 columns R [ncol][n] (exact)."""
+    ncol = R.shape[0]
+    csT = np.zeros((52, ncol), np.int64)
+    for j in range(n):
+        csT[cards[j, 0]] += R[:, j]
+        csT[cards[j, 1]] += R[:, j]
+    H = -csT
+    HP = -(csT.sum(0) // 2)
+    out = np.zeros((ncol, n), np.int64)
+    tile, tile_rows = {}, set()
+    ring, queue = int(prog[0, 1]), [int(prog[0, 2]), int(prog[0, 3])]  # record 0: first three blocks
+    oblk = 0
+    for rec in prog[1:]:
+        h = int(rec[0])
+        op, k = h & 15, (h >> 4) & 15
+        if op == OP_END:
+            break
+        if h & F_ADV:
+            ring = queue.pop(0)
+            queue.append((h >> 16) & 0xFF)
+            assert ring != NO_BLOCK
+        mem = [(int(w) & 0x7FF, (int(w) >> 11) & 63, (int(w) >> 17) & 63) for w in rec[1:1 + k]]
+        if op in (OP_SINGLE, OP_HALF, OP_READ):
+            assert 1 <= k <= (4 if op == OP_SINGLE else 15)
+            for row, a, b in mem:
+                assert row < n and {a, b} == {int(cards[row, 0]), int(cards[row, 1])}
+                if op != OP_READ:
+                    assert row // RB == ring, "row outside the ring block"
+            if op == OP_SINGLE:  # reads, then writes: cards must not repeat (HALF adds atomically)
+                used = [c for _, a, b in mem for c in (a, b)]
+                assert len(used) == len(set(used)), "cards repeat within a SINGLE batch"
+            va = [H[a].copy() for _, a, _ in mem]
+            vb = [H[b].copy() for _, _, b in mem]
+            for m, (row, a, b) in enumerate(mem):
+                if op == OP_READ:
+                    val = HP - va[m] - vb[m]
+                elif op == OP_SINGLE:
+                    r = R[:, row]
+                    val = HP - va[m] - vb[m] - r
+                    HP = HP + 2 * r
+                    H[a] = va[m] + 2 * r
+                    H[b] = vb[m] + 2 * r
+                else:  # atomic adds: no value read before the batch
+                    r = R[:, row]
+                    HP = HP + r
+                    H[a] = H[a] + r
+                    H[b] = H[b] + r
+                if op != OP_HALF:
+                    assert row // OB == oblk, "value outside the output block"
+                    tile[row % OB] = val
+                    tile_rows.add(row)
+        else:
+            assert op == OP_NOP
+        if h & F_FLUSH:
+            nrows = ((h >> 10) & 15) + 1
+            assert {oblk * OB + x for x in range(nrows)} == tile_rows
+            for x in range(nrows):
+                out[:, oblk * OB + x] = tile[x]
+            tile, tile_rows = {}, set()
+            oblk += 1
+    assert oblk * OB >= n and (n == 0 or oblk == (n + OB - 1) // OB), "every row flushed once"
+    return out
+
+
+def dense(cards, grank, R):
+    n = len(grank)
+    g = np.asarray(grank, np.int64)
+    sign = np.sign(g[:, None] - g[None, :])
+    cm = np.zeros((n, 52), bool)
+    cm[np.arange(n), cards[:, 0]] = True
+    cm[np.arange(n), cards[:, 1]] = True
+    blocked = (cm.astype(np.int32) @ cm.T.astype(np.int32)) > 0
+    S = np.where(blocked, 0, sign).astype(np.int64)
+    return R @ S.T  # [ncol][hero]
+
+
+def fixed_point(r):
+    return np.rint(r.astype(np.float64) * 2.0 ** 48).astype(np.int64)
+
+
+def river_board(lib, board):
+    live = [c for c in range(52) if c not in board]
+    sc = sorted((lib.mtt_eval7(bytes(list(board) + [a, b])), a, b) for a, b in itertools.combinations(live, 2))
+    cards = np.array([[a, b] for _, a, b in sc], np.int64)
+    grank = np.zeros(len(sc), np.int64)
+    for i in range(1, len(sc)):
+        grank[i] = grank[i - 1] + (sc[i][0] != sc[i - 1][0])
+    return cards, grank
+
+
+def columns(rng, n, ncol):
+    r = (rng.random((ncol, n)) * (rng.random((ncol, n)) < 0.8)).astype(np.float32)
+    r[0] = 0.0
+    r[1] = 1.0
+    r[2] = np.float32(1e-7) * rng.random(n).astype(np.float32)  # near the fixed-point resolution
+    return fixed_point(r)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_sweep_matches_dense_on_river_boards(lib, seed):
+    rng = np.random.default_rng(seed)
+    board = [int(x) for x in rng.choice(52, 5, replace=False)]
+    cards, grank = river_board(lib, board)
+    n = len(grank)
+    assert n == 1081
+    R = columns(rng, n, 6)
+    prog = program(lib, cards, grank)
+    assert np.array_equal(replay(prog, cards, n, R), dense(cards, grank, R))
+
+
+def synthetic(rng, n, kind):
+    combos = list(itertools.combinations(range(52), 2))
+    pick = rng.choice(len(combos), n, replace=False)
+    cards = np.array([combos[i] for i in pick], np.int64).reshape(-1, 2)
+    if kind == "one_group":
+        grank = np.zeros(n, np.int64)
+    elif kind == "no_ties":
+        grank = np.arange(n, dtype=np.int64)
+    else:
+        grank = np.sort(rng.integers(0, max(1, n // 7), n))
+    return cards, grank
+
+
+@pytest.mark.parametrize("kind,n", [("one_group", 1081), ("no_ties", 1081), ("random", 1081), ("random", 37),
+                                    ("one_group", 1), ("no_ties", 17), ("random", 0)])
+def test_sweep_matches_dense_synthetic(lib, kind, n):
+    rng = np.random.default_rng(n * 7 + len(kind))
+    cards, grank = synthetic(rng, n, kind)
+    R = columns(rng, n, 4) if n else np.zeros((4, 0), np.int64)
+    prog = program(lib, cards, grank)
+    assert np.array_equal(replay(prog, cards, n, R), dense(cards, grank, R))
+
+
+def test_sweep_program_rejects_bad_input(lib):
+    cards = np.array([[0, 1], [2, 3]], np.uint8)
+    ln = ctypes.c_size_t()
+    bad_rank = np.array([1, 0], np.uint16)
+    assert lib.mtt_river_sweep_program(cards.ctypes.data, bad_rank.ctypes.data, 2, None, 0, ctypes.byref(ln)) != 0
+    same = np.array([[4, 4], [2, 3]], np.uint8)
+    ok_rank = np.array([0, 1], np.uint16)
+    assert lib.mtt_river_sweep_program(same.ctypes.data, ok_rank.ctypes.data, 2, None, 0, ctypes.byref(ln)) != 0
+    assert lib.mtt_river_sweep_program(cards.ctypes.data, ok_rank.ctypes.data, 1089, None, 0,
+                                       ctypes.byref(ln)) != 0
