@@ -296,15 +296,17 @@ int mtt_hu_sweep_compile(const mtt_hu_game* game, size_t* code_bytes, double* se
  * honest_multistreet_solver.cu:128-196; these two are for inspection and
  * tests).  The batched mode computes them by a sign x blocker GEMM, or, with
  * MTT_RIVER_SD=sweep in the environment at create, by an exact sweep in
- * strength order driven by a per-board program.  mtt_river_sweep_program writes the sweep program of one river
- * board -- n live combos in strength order, cards[2 * n], non-decreasing
- * group ranks grank[n] -- as 16-word records: at most cap words to out (may
- * be null), the full length to *len.  mtt_solver_river_showdown copies, for
- * river board `board` of a batched handle, the last iteration's showdown
- * columns (opponent reach, reach[ncol][1088]) and values (values[ncol][1088],
- * weaker minus stronger non-blocked reach) and the board's combo order
- * (cards[1088][2], grank[1088]); *nlive and *ncol receive the live combo and
- * column counts.  Null arrays query the counts only. */
+ * strength order driven by a per-board program.  mtt_river_sweep_program
+ * writes the sweep program of one river board -- n live combos in strength
+ * order, cards[2 * n], non-decreasing group ranks grank[n] -- as 16-word
+ * records, the first a header (words 1-3: the first three column blocks
+ * visited): at most cap words to out (may be null), the full length to
+ * *len.  mtt_solver_river_showdown copies, for river board `board` of a
+ * batched handle, the last iteration's showdown columns (opponent reach,
+ * reach[ncol][1088]) and values (values[ncol][1088], weaker minus stronger
+ * non-blocked reach) and the board's combo order (cards[1088][2],
+ * grank[1088]); *nlive and *ncol receive the live combo and column counts.
+ * Null arrays query the counts only. */
 int mtt_river_sweep_program(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* out, size_t cap,
                             size_t* len);
 int mtt_solver_river_showdown(mtt_solver* h, int board, float* reach, float* values, uint8_t* cards,

@@ -33,25 +33,28 @@
 namespace mtt {
 namespace bt {
 
-// Appends the sweep program of one board (16-word records, padded to whole
-// chunks of SD_CHUNK records): n live combos in strength order, cards
-// c[2 * j], c[2 * j + 1] and group ranks g[j] (non-decreasing).  Returns
-// false if n exceeds NPR, a card is invalid or the ranks are not sorted.
-bool sd_program(const uint8_t* cards, const uint16_t* grank, int n, std::vector<uint32_t>* prog);
+// Appends the sweep program of one board -- n live combos in strength
+// order, cards c[2 * j], c[2 * j + 1] and group ranks g[j] (non-decreasing)
+// -- as a sequence of visits: each visit names one 16-row block of the
+// columns and holds at most SD_VISIT 16-word records (the first a header).
+// hdr receives the board's first record and the first three blocks visited.
+// Returns false if n exceeds NPR, a card is invalid or the ranks are not
+// sorted.
+bool sd_program(const uint8_t* cards, const uint16_t* grank, int n, std::vector<uint32_t>* prog, uint32_t hdr[4]);
 
-// Words per program record; records per chunk the kernel stages in LDS.
+// Words per program record; records per visit (the kernel stages one visit in LDS).
 constexpr int SD_REC = 16;
-constexpr int SD_CHUNK = 16;
+constexpr int SD_VISIT = 16;
 // Records the kernel may read past the end of the last program.
-constexpr int SD_PAD_RECORDS = 2 * SD_CHUNK;
+constexpr int SD_PAD_RECORDS = SD_VISIT;
 
 // Launches the sweep for every board of level 1: columns from D.Bf, card
 // totals from the forward's card sums (L.part), values to D.C.
-// prog / poff: the concatenated programs (followed by SD_PAD_RECORDS zero
-// records) and each board's first record;
-// colinfo[col] = terminal | player << 31 of each showdown column.
-void sd_launch(const BatchedDev& D, const uint32_t* prog, const uint32_t* poff, const int* colinfo, int nboards,
-               int ncol, hipStream_t st);
+// prog: the concatenated programs followed by SD_PAD_RECORDS zero records;
+// hdrs[board] = first record, first three blocks; colinfo[col] =
+// terminal | player << 31 of each showdown column.
+void sd_launch(const BatchedDev& D, const uint32_t* prog, const uint4* hdrs, const int* colinfo, int nboards, int ncol,
+               hipStream_t st);
 
 }  // namespace bt
 }  // namespace mtt

@@ -8,32 +8,40 @@ namespace mtt {
 namespace bt {
 namespace {
 
-// Program records: 16 words each.  Word 0: opcode (bits 0-3), member count k
-// (bits 4-7), ADV flag (bit 8: first make the next block of the program's
-// block sequence the ring block, and request the block in bits 16-23, 0xFF
-// for none, two steps ahead), FLUSH flag (bit 9: afterwards write rows 0..n-1
-// of the 16-row output tile to C, n - 1 in bits 10-13).  Words 1..k: members
-// as row | card a << 11 | card b << 17.  Record 0 holds the first three
-// blocks of the sequence in words 1-3.
+// A program is a sequence of visits.  A visit makes one 32-row block of the
+// columns the ring block and runs its records; its first record is a header:
+// word 0 = OP_VISIT, word 1 = the block to request for two visits ahead
+// (NO_BLOCK for none), word 2 = the visit's record count (header included).
+// Other records: word 0 = opcode (bits 0-3), member count k (bits 4-7: 5, or
+// 15 for HALF and READ), FLUSH flag (bit 9: afterwards write rows 0..n-1 of the
+// 16-row output tile to C, n - 1 in bits 10-13); words 1..k = members as
+// row | card a << 11 | card b << 17 | valid << 31.  Records are padded to
+// 5 or 15 members with copies of their first member, marked invalid and
+// placed first: an invalid member adds no reach, and whatever it writes is
+// written again by the member it copies.
 enum : uint32_t {
-    OP_NOP = 0,      // flags only
+    OP_NOP = 0,
     OP_SINGLE = 1,   // k singleton groups (consecutive rows, distinct cards): read, value, add 2R
     OP_HALF = 2,     // k members of a tie group: add R (LDS atomics: cards may repeat)
     OP_READ = 3,     // k members of a tie group: value at the half state
     OP_END = 4,
+    OP_VISIT = 5,
 };
-constexpr uint32_t F_ADV = 1u << 8, F_FLUSH = 1u << 9;
+constexpr uint32_t F_FLUSH = 1u << 9;
 constexpr uint32_t NO_BLOCK = 0xFFu;
 constexpr int REC = SD_REC;
-constexpr int CHUNK = SD_CHUNK;     // records per program chunk (one uint4 per lane)
+constexpr int VISIT = SD_VISIT;
 
 constexpr int RB = 32;              // rows per ring block
 constexpr int OB = 16;              // rows per output block
-constexpr int MAXM = REC - 1;       // members per HALF / READ record
-constexpr int MAXS = 4;             // members per SINGLE record
+constexpr int GRP = 5;              // members of a small record (and of every SINGLE record)
+constexpr int MAXM = REC - 1;       // members of a large HALF / READ record
+constexpr int MAXS = 4;             // real members per SINGLE record (one group)
+constexpr uint32_t M_VALID = 1u << 31;  // member flag: clear on padding members (their reach counts as 0)
 constexpr int OSTR = 68;            // output tile row stride (floats): two lanes per bank on write and flush
 constexpr int NRB = NPR / RB;       // 34 ring blocks
 static_assert(NPR % RB == 0 && RB % OB == 0, "row blocks");
+static_assert(VISIT * REC / 4 == 64, "one visit is one uint4 per lane");
 
 __device__ __forceinline__ unsigned long long fx_of(float r) { return __double2ull_rn((double)r * FX); }
 // C = W - L, summed in 2^48 units (two's complement)
@@ -47,82 +55,73 @@ __device__ __forceinline__ uint32_t qword(const uint4& q0, const uint4& q1, cons
 }
 
 // One wave per (64 columns, board).  LDS: card sums H[card][lane] (2^48
-// fixed point, two's complement, offset by -csT), the current 32-row block
-// of the columns ring[row][lane], the 16-row output tile otile[row][lane]
-// and the current 16-record chunk of the program (read by every lane at one
-// address, then made uniform).  Column blocks arrive two blocks ahead and
-// program chunks one chunk ahead, by vector loads.
+// fixed point, two's complement, offset by -csT), the ring block of the
+// columns ring[row][lane] (FP32, taken to fixed point where used), the output
+// tile otile[row][lane]
+// and the current visit's records (read by every lane at one address, then
+// made uniform).  At each visit boundary the next visit's records and the
+// column block two visits ahead are requested; the register set a block
+// arrives in alternates with the visit's parity in a loop unrolled by two,
+// so every wait on them is for loads issued a whole visit earlier.
 __global__ __launch_bounds__(64) void k_sd_sweep(BatchedDev D, const uint32_t* __restrict__ prog,
-                                                 const uint32_t* __restrict__ poff, const int* __restrict__ colinfo,
+                                                 const uint4* __restrict__ hdrs, const int* __restrict__ colinfo,
                                                  int ncol) {
     __shared__ unsigned long long H[52 * 64];
     __shared__ float ring[RB * 64];
     __shared__ float otile[OB * OSTR];
-    __shared__ uint4 pbuf[CHUNK * REC / 4];
+    __shared__ uint4 pbuf[VISIT * REC / 4];
     const LevelDev& L = D.L[1];
     const int b = blockIdx.y, lane = threadIdx.x, c0 = blockIdx.x * 64, col = c0 + lane;
     const int ncolp = L.ncolp;
     const bool cl = col < ncol;
-    const float* rc = D.Bf + ((size_t)b * ncolp + col) * NPR;  // read only when cl
+    const uint4* gp = reinterpret_cast<const uint4*>(prog);   // record r = gp[4r .. 4r+3]
+    const uint4 hd = hdrs[b];
 
-    // program: chunk 0 into LDS, chunk 1 in flight
-    const uint4* gp = reinterpret_cast<const uint4*>(prog) + (size_t)poff[b] * (REC / 4);
-    pbuf[lane] = gp[lane];
-    uint4 pch = gp[CHUNK * REC / 4 + lane];
+    // visit 0 into LDS
+    pbuf[lane] = gp[(size_t)hd.x * (REC / 4) + lane];
 
-    // H = -csT (the column's card totals, both forward halves), HP = -Ptot
+    // H = -csT (the column's card totals, both forward halves), HP = -Ptot; the totals are
+    // requested 26 cards at a time (from a valid address for every lane) before any is used
     unsigned long long tot = 0;
     {
-        const int ci = cl ? colinfo[col] : 0;
+        const int ci = cl ? colinfo[col] : colinfo[0];
         const int p = (int)((unsigned)ci >> 31), t = ci & 0x7FFFFFFF;
         const unsigned long long* P0 = L.part + ((size_t)b * L.nterm + t) * 208 + (1 - p) * 52;
-#pragma unroll 4
-        for (int c = 0; c < 52; ++c) {
-            const unsigned long long v = cl ? P0[c] + P0[104 + c] : 0ull;
-            H[c * 64 + lane] = 0ull - v;
-            tot += v;
+        const unsigned long long km = cl ? ~0ull : 0ull;
+#pragma unroll
+        for (int cb = 0; cb < 52; cb += 26) {
+            unsigned long long v0[26], v1[26];
+#pragma unroll
+            for (int x = 0; x < 26; ++x) {
+                v0[x] = P0[cb + x];
+                v1[x] = P0[104 + cb + x];
+            }
+#pragma unroll
+            for (int x = 0; x < 26; ++x) {
+                const unsigned long long v = (v0[x] + v1[x]) & km;
+                H[(cb + x) * 64 + lane] = 0ull - v;
+                tot += v;
+            }
         }
     }
     unsigned long long HP = 0ull - (tot >> 1);
 
-#define SD_LOAD(blk, set_)                                                                        \
+    // a block of this lane's column, loaded unconditionally (from a valid address) so that the
+    // number of loads between two visit boundaries never varies; invalid lanes or blocks read as 0
+    const float* rcv = D.Bf + ((size_t)b * ncolp + (cl ? col : 0)) * NPR;
+#define SD_LOAD(blk, set_, ok_)                                                                \
     do {                                                                                       \
-        if (cl && (int)(blk) < NRB) {                                                               \
-            const float4* q_ = reinterpret_cast<const float4*>(rc + (size_t)(blk) * RB);       \
-            _Pragma("unroll") for (int i_ = 0; i_ < RB / 4; ++i_) set_[i_] = q_[i_];              \
-        } else {                                                                               \
-            _Pragma("unroll") for (int i_ = 0; i_ < RB / 4; ++i_) set_[i_] = make_float4(0.f, 0.f, 0.f, 0.f); \
-        }                                                                                      \
+        ok_ = cl && (int)(blk) < NRB;                                                          \
+        const float4* q_ = reinterpret_cast<const float4*>(rcv + (size_t)min((int)(blk), NRB - 1) * RB); \
+        _Pragma("unroll") for (int i_ = 0; i_ < RB / 4; ++i_) set_[i_] = q_[i_];               \
     } while (0)
-#define SD_RING(set_)                                                                             \
+#define SD_RING(set_, ok_)                                                                     \
     do {                                                                                       \
         _Pragma("unroll") for (int i_ = 0; i_ < RB / 4; ++i_) {                                \
-            ring[(4 * i_ + 0) * 64 + lane] = set_[i_].x;                                          \
-            ring[(4 * i_ + 1) * 64 + lane] = set_[i_].y;                                          \
-            ring[(4 * i_ + 2) * 64 + lane] = set_[i_].z;                                          \
-            ring[(4 * i_ + 3) * 64 + lane] = set_[i_].w;                                          \
-        }                                                                                      \
-    } while (0)
-
-    // the sequence's first block into the ring, the next two in flight (set v & 1 holds the
-    // block of ADV number v + 1)
-    const uint32_t b0 = rfl(pbuf[0].y), b1 = rfl(pbuf[0].z), b2 = rfl(pbuf[0].w);
-    float4 sa[RB / 4], sb[RB / 4];
-    SD_LOAD(b0, sa);
-    SD_RING(sa);
-    SD_LOAD(b1, sb);
-    SD_LOAD(b2, sa);
-    int nadv = 0, oblk = 0;
-
-#define SD_ADV(nb)                                                                             \
-    do {                                                                                       \
-        ++nadv;                                                                                \
-        if (nadv & 1) {                                                                        \
-            SD_RING(sb);                                                                       \
-            SD_LOAD(nb, sb);                                                                   \
-        } else {                                                                               \
-            SD_RING(sa);                                                                       \
-            SD_LOAD(nb, sa);                                                                   \
+            ring[(4 * i_ + 0) * 64 + lane] = ok_ ? set_[i_].x : 0.0f;                          \
+            ring[(4 * i_ + 1) * 64 + lane] = ok_ ? set_[i_].y : 0.0f;                          \
+            ring[(4 * i_ + 2) * 64 + lane] = ok_ ? set_[i_].z : 0.0f;                          \
+            ring[(4 * i_ + 3) * 64 + lane] = ok_ ? set_[i_].w : 0.0f;                          \
         }                                                                                      \
     } while (0)
 #define SD_FLUSH(h)                                                                            \
@@ -136,146 +135,174 @@ __global__ __launch_bounds__(64) void k_sd_sweep(BatchedDev D, const uint32_t* _
         }                                                                                      \
         ++oblk;                                                                                \
     } while (0)
-    // record ri + 1 becomes current (cq*); then the read of the next one is issued (at a
-    // chunk boundary the next chunk is written to LDS first and the one after requested)
-#define SD_NEXT()                                                                              \
-    do {                                                                                       \
-        cq0 = n0; cq1 = n1; cq2 = n2; cq3 = n3;                                                \
-        const int nr_ = ri + 1;                                                                \
-        if ((nr_ & (CHUNK - 1)) == 0) {                                                        \
-            pbuf[lane] = pch;                                                                  \
-            pch = gp[(size_t)(nr_ + CHUNK) * (REC / 4) + lane];                                \
-        }                                                                                      \
-        const int pb_ = (nr_ & (CHUNK - 1)) * (REC / 4);                                       \
-        n0 = pbuf[pb_]; n1 = pbuf[pb_ + 1]; n2 = pbuf[pb_ + 2]; n3 = pbuf[pb_ + 3];            \
-    } while (0)
 #define QW(i) qword(cq0, cq1, cq2, cq3, (i))
+    // a HALF / READ record of exactly NM members (5 or 15): every read issued before any update
+#define SD_HALF(NM)                                                                            \
+    do {                                                                                       \
+        unsigned long long rr[NM];                                                             \
+        _Pragma("unroll") for (int m = 0; m < NM; ++m) {                                       \
+            const uint32_t w_ = QW(1 + m);                                                     \
+            rr[m] = fx_of(ring[(int)(w_ & (RB - 1)) * 64 + lane]) & ((w_ & M_VALID) ? ~0ull : 0ull); \
+        }                                                                                      \
+        _Pragma("unroll") for (int m = 0; m < NM; ++m) {                                       \
+            const uint32_t w_ = QW(1 + m);                                                     \
+            HP += rr[m];                                                                       \
+            atomicAdd(&H[(int)((w_ >> 11) & 63u) * 64 + lane], rr[m]);                         \
+            atomicAdd(&H[(int)((w_ >> 17) & 63u) * 64 + lane], rr[m]);                         \
+        }                                                                                      \
+    } while (0)
+#define SD_READ(NM)                                                                            \
+    do {                                                                                       \
+        unsigned long long va[NM], vb[NM];                                                     \
+        _Pragma("unroll") for (int m = 0; m < NM; ++m) {                                       \
+            const uint32_t w_ = QW(1 + m);                                                     \
+            va[m] = H[(int)((w_ >> 11) & 63u) * 64 + lane];                                    \
+            vb[m] = H[(int)((w_ >> 17) & 63u) * 64 + lane];                                    \
+        }                                                                                      \
+        _Pragma("unroll") for (int m = 0; m < NM; ++m)                                         \
+            otile[(int)(QW(1 + m) & (OB - 1)) * OSTR + lane] = val_of(HP - va[m] - vb[m]);     \
+    } while (0)
+    // the records 1..nrec-1 of the visit in pbuf; done = an END was met.  A record holds 5 or
+    // 15 members (static word indices, no per-member branches).
+#define SD_RUN()                                                                               \
+    do {                                                                                       \
+        uint4 n0 = pbuf[4], n1 = pbuf[5], n2 = pbuf[6], n3 = pbuf[7];                          \
+        for (int r_ = 1; r_ < nrec; ++r_) {                                                    \
+            const uint4 cq0 = n0, cq1 = n1, cq2 = n2, cq3 = n3;                                \
+            const int nx_ = min(r_ + 1, VISIT - 1) * (REC / 4);                                \
+            n0 = pbuf[nx_]; n1 = pbuf[nx_ + 1]; n2 = pbuf[nx_ + 2]; n3 = pbuf[nx_ + 3];        \
+            const uint32_t h = QW(0);                                                          \
+            const uint32_t op = h & 15u;                                                       \
+            const int nm = (int)((h >> 4) & 15u);                                              \
+            if (op == OP_END) {                                                                \
+                done = true;                                                                   \
+                break;                                                                         \
+            }                                                                                  \
+            if (op == OP_HALF) {                                                               \
+                if (nm == MAXM) SD_HALF(MAXM); else SD_HALF(GRP);                              \
+            } else if (op == OP_READ) {                                                        \
+                if (nm == MAXM) SD_READ(MAXM); else SD_READ(GRP);                              \
+            } else if (op == OP_SINGLE) {                                                      \
+                unsigned long long rr[GRP], va[GRP], vb[GRP];                                  \
+                _Pragma("unroll") for (int m = 0; m < GRP; ++m) {                              \
+                    const uint32_t w_ = QW(1 + m);                                             \
+                    rr[m] = fx_of(ring[(int)(w_ & (RB - 1)) * 64 + lane]) & ((w_ & M_VALID) ? ~0ull : 0ull); \
+                    va[m] = H[(int)((w_ >> 11) & 63u) * 64 + lane];                            \
+                    vb[m] = H[(int)((w_ >> 17) & 63u) * 64 + lane];                            \
+                }                                                                              \
+                _Pragma("unroll") for (int m = 0; m < GRP; ++m) {                              \
+                    const uint32_t w_ = QW(1 + m);                                             \
+                    otile[(int)(w_ & (OB - 1)) * OSTR + lane] = val_of(HP - va[m] - vb[m] - rr[m]); \
+                    HP += 2 * rr[m];                                                           \
+                    H[(int)((w_ >> 11) & 63u) * 64 + lane] = va[m] + 2 * rr[m];                \
+                    H[(int)((w_ >> 17) & 63u) * 64 + lane] = vb[m] + 2 * rr[m];                \
+                }                                                                              \
+            }                                                                                  \
+            if (h & F_FLUSH) SD_FLUSH(h);                                                      \
+        }                                                                                      \
+    } while (0)
+    // next visit: its records from the prefetch register into LDS, its header read, the
+    // visit after it requested; the ring block from set_, which then requests the header's block
+#define SD_CROSS(set_, ok_)                                                                    \
+    do {                                                                                       \
+        vo += nrec;                                                                            \
+        pbuf[lane] = pch;                                                                      \
+        const uint4 vh_ = pbuf[0];                                                             \
+        nrec = (int)rfl(vh_.z);                                                                \
+        const uint32_t pf_ = rfl(vh_.y);                                                       \
+        pch = gp[(size_t)(vo + nrec) * (REC / 4) + lane];                                      \
+        SD_RING(set_, ok_);                                                                    \
+        SD_LOAD(pf_, set_, ok_);                                                               \
+    } while (0)
 
-    uint4 cq0, cq1, cq2, cq3;
-    uint4 n0 = pbuf[REC / 4], n1 = pbuf[REC / 4 + 1], n2 = pbuf[REC / 4 + 2], n3 = pbuf[REC / 4 + 3];
-    int ri = 1;  // record 0 is the header
-    for (;;) {
-        SD_NEXT();
-        const uint32_t h = QW(0);
-        const uint32_t op = h & 15u;
-        const int k = (int)((h >> 4) & 15u);
-        if (op == OP_END) break;
-        if (h & F_ADV) SD_ADV(h >> 16);
-        if (op == OP_HALF) {
-            float r[MAXM];
-#pragma unroll
-            for (int m = 0; m < MAXM; ++m)
-                if (m < k) r[m] = ring[(int)(QW(1 + m) & (RB - 1)) * 64 + lane];
-#pragma unroll
-            for (int m = 0; m < MAXM; ++m) {
-                if (m < k) {
-                    const uint32_t w_ = QW(1 + m);
-                    const unsigned long long R = fx_of(r[m]);
-                    HP += R;
-                    atomicAdd(&H[(int)((w_ >> 11) & 63u) * 64 + lane], R);
-                    atomicAdd(&H[(int)((w_ >> 17) & 63u) * 64 + lane], R);
-                }
-            }
-        } else if (op == OP_READ) {
-            unsigned long long va[MAXM], vb[MAXM];
-#pragma unroll
-            for (int m = 0; m < MAXM; ++m) {
-                if (m < k) {
-                    const uint32_t w_ = QW(1 + m);
-                    va[m] = H[(int)((w_ >> 11) & 63u) * 64 + lane];
-                    vb[m] = H[(int)((w_ >> 17) & 63u) * 64 + lane];
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < MAXM; ++m)
-                if (m < k) otile[(int)(QW(1 + m) & (OB - 1)) * OSTR + lane] = val_of(HP - va[m] - vb[m]);
-        } else if (op == OP_SINGLE) {
-            float r[MAXS];
-            unsigned long long va[MAXS], vb[MAXS];
-#pragma unroll
-            for (int m = 0; m < MAXS; ++m) {
-                if (m < k) {
-                    const uint32_t w_ = QW(1 + m);
-                    r[m] = ring[(int)(w_ & (RB - 1)) * 64 + lane];
-                    va[m] = H[(int)((w_ >> 11) & 63u) * 64 + lane];
-                    vb[m] = H[(int)((w_ >> 17) & 63u) * 64 + lane];
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < MAXS; ++m) {
-                if (m < k) {
-                    const uint32_t w_ = QW(1 + m);
-                    const unsigned long long R = fx_of(r[m]);
-                    otile[(int)(w_ & (OB - 1)) * OSTR + lane] = val_of(HP - va[m] - vb[m] - R);
-                    HP += 2 * R;
-                    H[(int)((w_ >> 11) & 63u) * 64 + lane] = va[m] + 2 * R;
-                    H[(int)((w_ >> 17) & 63u) * 64 + lane] = vb[m] + 2 * R;
-                }
-            }
-        }
-        if (h & F_FLUSH) SD_FLUSH(h);
-        ++ri;
+    uint32_t vo = hd.x;                                  // first record of the current visit
+    int nrec = (int)rfl(pbuf[0].z);
+    uint4 pch = gp[(size_t)(vo + nrec) * (REC / 4) + lane];  // visit 1
+    // block of visit 0 into the ring; visit 1's block arrives in set a, visit 2's in set b
+    // (issued in this order: the waits of the first visit boundaries count on set a's loads
+    // being older than set b's)
+    float4 sa[RB / 4], sb[RB / 4];
+    bool oka, okb;
+    SD_LOAD(hd.y, sa, oka);
+    SD_RING(sa, oka);
+    __builtin_amdgcn_sched_barrier(0);
+    SD_LOAD(hd.z, sa, oka);
+    __builtin_amdgcn_sched_barrier(0);
+    SD_LOAD(hd.w, sb, okb);
+    __builtin_amdgcn_sched_barrier(0);
+    int oblk = 0;
+    bool done = false;
+    SD_RUN();
+    while (!done) {
+        SD_CROSS(sa, oka);
+        SD_RUN();
+        if (done) break;
+        SD_CROSS(sb, okb);
+        SD_RUN();
     }
 #undef SD_LOAD
 #undef SD_RING
-#undef SD_ADV
 #undef SD_FLUSH
-#undef SD_NEXT
 #undef QW
+#undef SD_RUN
+#undef SD_CROSS
+#undef SD_HALF
+#undef SD_READ
 }
 
 }  // namespace
 
-bool sd_program(const uint8_t* cards, const uint16_t* grank, int n, std::vector<uint32_t>* out) {
+bool sd_program(const uint8_t* cards, const uint16_t* grank, int n, std::vector<uint32_t>* out, uint32_t hdr[4]) {
     if (n < 0 || n > NPR) return false;
     for (int j = 0; j < n; ++j)
         if (cards[2 * j] >= 52 || cards[2 * j + 1] >= 52 || cards[2 * j] == cards[2 * j + 1]) return false;
     for (int j = 1; j < n; ++j)
         if (grank[j] < grank[j - 1]) return false;
-    std::vector<uint32_t>& P = *out;
-    const size_t start = P.size();
-    uint32_t pend = 0;                   // flags for the next record
-    size_t last = 0;                     // first word of the last record
-    std::vector<int> seq{0};             // ring blocks in the order the program visits them
-    std::vector<size_t> adv_at;          // first word of each ADV record
-    auto record = [&](uint32_t hdr, const uint32_t* ws, int nw) {
-        last = P.size();
-        P.resize(last + REC, 0u);
-        P[last] = hdr | pend;
-        if (pend & F_ADV) adv_at.push_back(last);
-        pend = 0;
-        for (int x = 0; x < nw; ++x) P[last + 1 + x] = ws[x];
+    struct Rec {
+        uint32_t w[REC];
     };
-    record(OP_NOP, nullptr, 0);          // header: first blocks, filled in below
-    auto mword = [&](int j) { return (uint32_t)j | (uint32_t)cards[2 * j] << 11 | (uint32_t)cards[2 * j + 1] << 17; };
+    struct Visit {
+        int block;
+        std::vector<Rec> recs;
+    };
+    std::vector<Visit> visits;
+    auto mword = [&](int j) {
+        return (uint32_t)j | (uint32_t)cards[2 * j] << 11 | (uint32_t)cards[2 * j + 1] << 17 | M_VALID;
+    };
     auto cmask = [&](int j) { return (1ull << cards[2 * j]) | (1ull << cards[2 * j + 1]); };
-    int oblk = 0;
-    auto need_block = [&](int bl) {      // ring block bl current
-        if (seq.back() == bl) return;
-        if (pend & F_ADV) record(OP_NOP, nullptr, 0);
-        pend |= F_ADV;
-        seq.push_back(bl);
+    // a record needing ring block bl (-1: any), in a new visit if the current one is another block or full
+    auto add = [&](const Rec& r, int bl) {
+        if (visits.empty()) visits.push_back({bl < 0 ? 0 : bl, {}});
+        if ((bl >= 0 && visits.back().block != bl) || (int)visits.back().recs.size() == VISIT - 1) {
+            const int nb = bl >= 0 ? bl : visits.back().block;
+            visits.push_back({nb, {}});
+        }
+        visits.back().recs.push_back(r);
     };
+    int oblk = 0;
     // after the value of `row` went to the output tile: flush a finished output block
     auto flush_after = [&](int row) {
         if ((row + 1) % OB == 0 || row == n - 1) {
-            P[last] |= F_FLUSH | (uint32_t)(row - oblk * OB) << 10;
+            visits.back().recs.back().w[0] |= F_FLUSH | (uint32_t)(row - oblk * OB) << 10;
             ++oblk;
         }
     };
-    auto batch = [&](uint32_t op, const std::vector<int>& mem) {
-        uint32_t w[MAXM];
-        for (size_t x = 0; x < mem.size(); ++x) w[x] = mword(mem[x]);
-        record(op | (uint32_t)mem.size() << 4, w, (int)mem.size());
+    // padded to 5 or 15 members: invalid copies of the first member ahead of the members
+    auto batch = [&](uint32_t op, const std::vector<int>& mem, int bl) {
+        Rec r{};
+        const int k = (int)mem.size(), pad = (k <= GRP ? GRP : MAXM) - k;
+        r.w[0] = op | (uint32_t)(k + pad) << 4;
+        for (int x = 0; x < pad; ++x) r.w[1 + x] = mword(mem[0]) & ~M_VALID;
+        for (int x = 0; x < k; ++x) r.w[1 + pad + x] = mword(mem[x]);
+        add(r, bl);
     };
     auto half_pass = [&](int i, int e) {  // half of every member's reach, ring block by ring block
         for (int bs = i; bs < e;) {
             const int be = std::min(e, (bs / RB + 1) * RB);
-            need_block(bs / RB);
             for (int s0 = bs; s0 < be; s0 += MAXM) {
                 std::vector<int> mem;
                 for (int j = s0; j < std::min(be, s0 + MAXM); ++j) mem.push_back(j);
-                batch(OP_HALF, mem);
+                batch(OP_HALF, mem, bs / RB);
             }
             bs = be;
         }
@@ -284,8 +311,7 @@ bool sd_program(const uint8_t* cards, const uint16_t* grank, int n, std::vector<
     uint64_t smask = 0;
     auto flush_single = [&]() {
         if (single.empty()) return;
-        need_block(single[0] / RB);
-        batch(OP_SINGLE, single);
+        batch(OP_SINGLE, single, single[0] / RB);
         flush_after(single.back());
         single.clear();
         smask = 0;
@@ -310,31 +336,41 @@ bool sd_program(const uint8_t* cards, const uint16_t* grank, int n, std::vector<
             for (int s0 = bs; s0 < be; s0 += MAXM) {
                 std::vector<int> mem;
                 for (int j = s0; j < std::min(be, s0 + MAXM); ++j) mem.push_back(j);
-                batch(OP_READ, mem);
+                batch(OP_READ, mem, -1);
             }
             flush_after(be - 1);
             bs = be;
         }
-        half_pass(i, e);  // the other half (re-visits the group's ring blocks)
+        half_pass(i, e);  // the other half (re-visits the group's blocks)
         i = e;
     }
     flush_single();
-    record(OP_END, nullptr, 0);
-    record(OP_NOP, nullptr, 0);  // spare: read ahead by the kernel
-    while ((P.size() - start) % (CHUNK * REC)) record(OP_NOP, nullptr, 0);
-    // the block each ADV requests (two steps ahead in the sequence), and the first three
-    auto at = [&](size_t v) { return v < seq.size() ? (uint32_t)seq[v] : NO_BLOCK; };
-    for (size_t v = 0; v < adv_at.size(); ++v) P[adv_at[v]] |= at(v + 3) << 16;
-    P[start + 1] = at(0);
-    P[start + 2] = at(1);
-    P[start + 3] = at(2);
+    Rec end{};
+    end.w[0] = OP_END;
+    add(end, -1);
+    // serialise: a header record per visit (the block two visits ahead, the record count)
+    std::vector<uint32_t>& P = *out;
+    hdr[0] = (uint32_t)(P.size() / REC);
+    auto blk = [&](size_t v) { return v < visits.size() ? (uint32_t)visits[v].block : NO_BLOCK; };
+    hdr[1] = blk(0);
+    hdr[2] = blk(1);
+    hdr[3] = blk(2);
+    for (size_t v = 0; v < visits.size(); ++v) {
+        const size_t at = P.size();
+        P.resize(at + (size_t)REC * (1 + visits[v].recs.size()), 0u);
+        P[at] = OP_VISIT;
+        P[at + 1] = blk(v + 2);
+        P[at + 2] = (uint32_t)(1 + visits[v].recs.size());
+        for (size_t r = 0; r < visits[v].recs.size(); ++r)
+            for (int x = 0; x < REC; ++x) P[at + (size_t)REC * (1 + r) + x] = visits[v].recs[r].w[x];
+    }
     return true;
 }
 
-void sd_launch(const BatchedDev& D, const uint32_t* prog, const uint32_t* poff, const int* colinfo, int nboards,
-               int ncol, hipStream_t st) {
+void sd_launch(const BatchedDev& D, const uint32_t* prog, const uint4* hdrs, const int* colinfo, int nboards, int ncol,
+               hipStream_t st) {
     const dim3 g((ncol + 63) / 64, nboards);
-    hipLaunchKernelGGL(k_sd_sweep, g, dim3(64), 0, st, D, prog, poff, colinfo, ncol);
+    hipLaunchKernelGGL(k_sd_sweep, g, dim3(64), 0, st, D, prog, hdrs, colinfo, ncol);
 }
 
 }  // namespace bt

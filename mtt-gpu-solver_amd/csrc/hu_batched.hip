@@ -1009,7 +1009,7 @@ struct BatchedState {
     int gemm_shape = 100;       // river GEMM form and shape (autotuned; see river_gemm)
     bool sd_sweep = false;      // river showdown by the strength-order sweep (else the GEMM)
     uint32_t* d_sdprog = nullptr;   // sweep programs of the river boards (SD_REC-word records)
-    uint32_t* d_sdoff = nullptr;    // [boards1] first record of each board's program
+    uint32_t* d_sdhdr = nullptr;    // [boards1][4] first record and first three blocks of each board's program
     int* d_colinfo = nullptr;       // [ncol] terminal | player << 31 of each river showdown column
 };
 
@@ -1409,12 +1409,11 @@ int batched_create(SolverImpl& s) {
             rc = A.alloc(nbc, &B.D.Bf);
             if (rc) return rc;
             MTT_HIP(hipMemsetAsync(B.D.Bf, 0, nbc * 4, s.stream));
-            std::vector<uint32_t> prog, poff(nb1);
-            for (int bi = 0; bi < nb1; ++bi) {
-                poff[bi] = (uint32_t)(prog.size() / SD_REC);
-                MTT_ARG(sd_program(&cc1[(size_t)bi * NPR * 2], &rank1[(size_t)bi * NPR], nlive1[bi], &prog),
+            std::vector<uint32_t> prog, hdr((size_t)nb1 * 4);
+            for (int bi = 0; bi < nb1; ++bi)
+                MTT_ARG(sd_program(&cc1[(size_t)bi * NPR * 2], &rank1[(size_t)bi * NPR], nlive1[bi], &prog,
+                                   &hdr[(size_t)bi * 4]),
                         "river board %d: combos not in strength order", bi);
-            }
             prog.resize(prog.size() + (size_t)SD_PAD_RECORDS * SD_REC, 0u);
             std::vector<int> ci(std::max(L1.ncol, 1), 0);
             for (const DNode& d : H1.nodes)
@@ -1423,7 +1422,7 @@ int batched_create(SolverImpl& s) {
                     ci[d.col + 1] = (int)((uint32_t)d.sidx | 0x80000000u);
                 }
             rc = A.upload(prog, &B.d_sdprog);
-            rc = rc ? rc : A.upload(poff, &B.d_sdoff);
+            rc = rc ? rc : A.upload(hdr, &B.d_sdhdr);
             rc = rc ? rc : A.upload(ci, &B.d_colinfo);
             if (rc) return rc;
         } else {
@@ -1683,7 +1682,8 @@ int batched_run(SolverImpl& s, int iters) {
         }
         if (B.sd_sweep) {
             MTT_TIMED(s, "river_showdown",
-                      sd_launch(B.D, B.d_sdprog, B.d_sdoff, B.d_colinfo, B.nb1, B.D.L[1].ncol, s.stream));
+                      sd_launch(B.D, B.d_sdprog, reinterpret_cast<const uint4*>(B.d_sdhdr), B.d_colinfo, B.nb1,
+                                B.D.L[1].ncol, s.stream));
         } else {
             MTT_TIMED(s, "river_showdown", rc = river_gemm(B, B.gemm_shape, s.stream));
         }
@@ -2286,9 +2286,11 @@ int mtt_river_sweep_program(const uint8_t* cards, const uint16_t* grank, int n, 
                             size_t* len) {
     using namespace mtt;
     MTT_ARG(cards && grank && len, "mtt_river_sweep_program: null argument");
-    std::vector<uint32_t> prog;
-    MTT_ARG(bt::sd_program(cards, grank, n, &prog), "mtt_river_sweep_program: %d combos, bad cards or unsorted ranks",
-            n);
+    std::vector<uint32_t> prog(bt::SD_REC, 0u);  // record 0: the board header
+    uint32_t hdr[4];
+    MTT_ARG(bt::sd_program(cards, grank, n, &prog, hdr),
+            "mtt_river_sweep_program: %d combos, bad cards or unsorted ranks", n);
+    for (int x = 0; x < 4; ++x) prog[x] = hdr[x];
     *len = prog.size();
     if (out) memcpy(out, prog.data(), std::min(cap, prog.size()) * 4);
     return MTT_OK;

@@ -8,8 +8,8 @@ reach) -- by a sweep in strength order driven by a per-board program of
 kernel's semantics (SINGLE and READ batches issue all their reads, then
 their updates in member order; HALF batches add atomically, a tie group in
 two passes around its READs; rows come from the current 32-row block of the
-columns, blocks in the order the program requests them; values go to a
-16-row output tile flushed by the FLUSH flag) on integer reach in 2^48
+columns, one block per visit of the program; values go to a 16-row output
+tile flushed by the FLUSH flag) on integer reach in 2^48
 fixed point, and require the exact integers of the dense product: every
 hero's (weaker - stronger) non-blocked reach, on evaluator-ranked river
 boards and on synthetic extremes (no combos, one tie group, no ties, random
@@ -21,9 +21,9 @@ import numpy as np
 import pytest
 
 REC = 16
-OP_NOP, OP_SINGLE, OP_HALF, OP_READ, OP_END = range(5)
-F_ADV, F_FLUSH = 1 << 8, 1 << 9
-RB, OB = 32, 16           # ring block rows, output block rows
+OP_NOP, OP_SINGLE, OP_HALF, OP_READ, OP_END, OP_VISIT = range(6)
+F_FLUSH = 1 << 9
+RB, OB = 32, 16           # rows per column block, per output block
 NO_BLOCK = 0xFF
 
 
@@ -63,39 +63,45 @@ def replay(prog, cards, n, R):
     out = np.zeros((ncol, n), np.int64)
     tile, tile_rows = {}, set()
     ring, queue = int(prog[0, 1]), [int(prog[0, 2]), int(prog[0, 3])]  # record 0: first three blocks
-    oblk = 0
-    for rec in prog[1:]:
-        h = int(rec[0])
-        op, k = h & 15, (h >> 4) & 15
-        if op == OP_END:
-            break
-        if h & F_ADV:
+    oblk, pos, visit, done = 0, 1, 0, False
+    while not done:
+        vh = prog[pos]
+        assert int(vh[0]) == OP_VISIT and 2 <= int(vh[2]) <= 16
+        if visit:  # the crossing: next block into the ring, the header's block requested
             ring = queue.pop(0)
-            queue.append((h >> 16) & 0xFF)
-            assert ring != NO_BLOCK
-        mem = [(int(w) & 0x7FF, (int(w) >> 11) & 63, (int(w) >> 17) & 63) for w in rec[1:1 + k]]
-        if op in (OP_SINGLE, OP_HALF, OP_READ):
-            assert 1 <= k <= (4 if op == OP_SINGLE else 15)
-            for row, a, b in mem:
+            queue.append(int(vh[1]))
+        assert ring != NO_BLOCK
+        for rec in prog[pos + 1:pos + int(vh[2])]:
+            h = int(rec[0])
+            op, k = h & 15, (h >> 4) & 15
+            if op == OP_END:
+                done = True
+                break
+            mem = [(int(w) & 0x7FF, (int(w) >> 11) & 63, (int(w) >> 17) & 63, int(w) >> 31) for w in rec[1:1 + k]]
+            assert op in (OP_SINGLE, OP_HALF, OP_READ)
+            assert k == 5 or (k == 15 and op != OP_SINGLE)
+            valid = [v for *_, v in mem]
+            assert sum(valid) >= (1 if k == 5 else 6) and all(valid[valid.index(1):]), "padding: invalid copies first"
+            for row, a, b, v in mem:
                 assert row < n and {a, b} == {int(cards[row, 0]), int(cards[row, 1])}
+                assert v or (row, a, b) == mem[valid.index(1)][:3], "padding copies the first member"
                 if op != OP_READ:
                     assert row // RB == ring, "row outside the ring block"
             if op == OP_SINGLE:  # reads, then writes: cards must not repeat (HALF adds atomically)
-                used = [c for _, a, b in mem for c in (a, b)]
-                assert len(used) == len(set(used)), "cards repeat within a SINGLE batch"
-            va = [H[a].copy() for _, a, _ in mem]
-            vb = [H[b].copy() for _, _, b in mem]
-            for m, (row, a, b) in enumerate(mem):
+                used = [c for _, a, b, v in mem if v for c in (a, b)]
+                assert len(used) == len(set(used)) and len(used) <= 8, "cards repeat within a SINGLE batch"
+            va = [H[a].copy() for _, a, _, _ in mem]
+            vb = [H[b].copy() for _, _, b, _ in mem]
+            for m, (row, a, b, v) in enumerate(mem):
+                r = R[:, row] if v else 0 * R[:, row]
                 if op == OP_READ:
                     val = HP - va[m] - vb[m]
                 elif op == OP_SINGLE:
-                    r = R[:, row]
                     val = HP - va[m] - vb[m] - r
                     HP = HP + 2 * r
                     H[a] = va[m] + 2 * r
                     H[b] = vb[m] + 2 * r
                 else:  # atomic adds: no value read before the batch
-                    r = R[:, row]
                     HP = HP + r
                     H[a] = H[a] + r
                     H[b] = H[b] + r
@@ -103,15 +109,15 @@ def replay(prog, cards, n, R):
                     assert row // OB == oblk, "value outside the output block"
                     tile[row % OB] = val
                     tile_rows.add(row)
-        else:
-            assert op == OP_NOP
-        if h & F_FLUSH:
-            nrows = ((h >> 10) & 15) + 1
-            assert {oblk * OB + x for x in range(nrows)} == tile_rows
-            for x in range(nrows):
-                out[:, oblk * OB + x] = tile[x]
-            tile, tile_rows = {}, set()
-            oblk += 1
+            if h & F_FLUSH:
+                nrows = ((h >> 10) & 15) + 1
+                assert {oblk * OB + x for x in range(nrows)} == tile_rows
+                for x in range(nrows):
+                    out[:, oblk * OB + x] = tile[x]
+                tile, tile_rows = {}, set()
+                oblk += 1
+        pos += int(vh[2])
+        visit += 1
     assert oblk * OB >= n and (n == 0 or oblk == (n + OB - 1) // OB), "every row flushed once"
     return out
 
