@@ -35,10 +35,12 @@ struct JitLevel {
 // backward runs ceil(1152 / bwd_nt) workgroups of bwd_nt threads (576 or
 // 384) per (board, template); the river backward does too when
 // bwd_passes = 1, and with bwd_passes = 2 (the default) runs one workgroup of
-// 576 per (board, template) that walks the two halves of the combos in turn,
-// so the card sums of the template's terminals are staged and totalled once
-// instead of twice.  *_waves > 0 asks the compiler to fit that many waves per
-// SIMD (amdgpu_waves_per_eu).
+// bwd_nt threads per (board, template) that walks the combos in
+// ceil(1088 / bwd_nt) slices in turn, so the card sums of the template's
+// terminals are staged and totalled once (576 threads: 9 waves per CU with
+// a few spilled registers beat 384 threads without spills, 6 waves per CU).
+// *_waves > 0 asks the compiler to fit that many waves per SIMD
+// (amdgpu_waves_per_eu).
 struct JitConfig {
     int bwd_nt = 576;
     int bwd_waves = 0;
@@ -49,12 +51,14 @@ struct JitConfig {
 };
 
 // Passes of the backward of a level (0: turn, 1: river) under cfg.
-inline int bwd_passes(int level, const JitConfig& cfg) { return level == 1 ? cfg.bwd_passes : 1; }
+inline int bwd_passes(int level, const JitConfig& cfg) {
+    return level == 1 && cfg.bwd_passes > 1 ? (1088 + cfg.bwd_nt - 1) / cfg.bwd_nt : 1;
+}
 
 // Workgroups per (board, template) of the backward of a level, of bwd_nt
 // threads each.
 inline int bwd_groups(int level, const JitConfig& cfg) {
-    return bwd_passes(level, cfg) == 2 ? 1 : (1152 + cfg.bwd_nt - 1) / cfg.bwd_nt;
+    return bwd_passes(level, cfg) > 1 ? 1 : (1152 + cfg.bwd_nt - 1) / cfg.bwd_nt;
 }
 
 struct JitSweeps {

@@ -203,13 +203,13 @@ void emit_up(Src& o, const Tmpl& T, int n, int p) {
 void gen_backward_case(Src& o, const JitLevel& J, int passes, int t) {
     Tmpl T(J, t);
     o("case %d: {", t);
-    if (passes == 2) {
-        // card sums and totals once; then each half of the combos in turn (the showdown values a thread stages
+    if (passes > 1) {
+        // card sums and totals once; then each slice of the combos in turn (the showdown values a thread stages
         // in LDS are its own, so the passes need no barrier)
         o("bwd_sums<NTB>(L, ssum, b, %d, %d);", T.nt, T.t0);
         o("bwd_totals<NTB>(ssum, %d);", T.nt);
         o("#pragma unroll 1");  // one walk's registers at a time
-        o("for (int pass = 0; pass < 2; ++pass) {");
+        o("for (int pass = 0; pass < %d; ++pass) {", passes);
         // an offset the compiler cannot see is zero: the card-sum reads are not hoisted out of the pass loop
         // (where they would all stay live across the walk)
         o("int soff = 0;");
@@ -305,7 +305,7 @@ void gen_backward_case(Src& o, const JitLevel& J, int passes, int t) {
         o("}");
         emit_up(o, T, n, p);
     }
-    if (passes == 2) {
+    if (passes > 1) {
         o("bwd_root_out<LEVEL>(D, L, b, tm, k, NP, live, out0, out1);");
         o("}");
     }
@@ -353,7 +353,7 @@ void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
     o("const LevelDev& L = D.L[LEVEL];");
     o("extern __shared__ unsigned long long ssum[];");
     const int passes = bwd_passes(J.level, cfg);
-    if (passes == 2) {
+    if (passes > 1) {
         o("const int inst = blockIdx.x;");
         o("const int b = inst / %d, tm = inst %% %d;", J.ntmpl, J.ntmpl);
         o("const int tid = threadIdx.x;");
@@ -422,9 +422,7 @@ JitConfig bt_jit_config() {
     if (c.bwd_nt != 384 && c.bwd_nt != 576) c.bwd_nt = 576;
     c.bwd_waves = env("MTT_JIT_BWD_WAVES", 0, 8, c.bwd_waves);
     c.fwd_waves = env("MTT_JIT_FWD_WAVES", 0, 8, c.fwd_waves);
-    if (c.bwd_nt != 576) c.bwd_passes = 1;  // two passes need 576 threads
     c.bwd_passes = env("MTT_JIT_BWD_PASSES", 1, 2, c.bwd_passes);
-    if (c.bwd_passes == 2) c.bwd_nt = 576;
     return c;
 }
 
@@ -489,8 +487,7 @@ int bt_jit_build(const JitLevel* levels, int n, const JitConfig& cfg, JitSweeps*
     if (rc) return rc;
     MTT_ARG(cfg.bwd_nt == 384 || cfg.bwd_nt == 576, "generated sweeps: backward workgroup %d not 384 or 576",
             cfg.bwd_nt);
-    MTT_ARG(cfg.bwd_passes == 1 || (cfg.bwd_passes == 2 && cfg.bwd_nt == 576),
-            "generated sweeps: two-pass backward needs 576-thread workgroups");
+    MTT_ARG(cfg.bwd_passes == 1 || cfg.bwd_passes == 2, "generated sweeps: backward passes %d not 1 or 2", cfg.bwd_passes);
     const std::string src = bt_jit_source(levels, n, cfg);
     int dev = 0;
     MTT_HIP(hipGetDevice(&dev));
