@@ -901,6 +901,153 @@ __global__ __launch_bounds__(512) void k_bt_gemm6s(BatchedDev D, const uint32_t*
     }
 }
 
+// k_bt_gemm6s with the reach stream fetched two chunks ahead: its 236
+// registers leave one workgroup per CU, so the two waves of a SIMD belong to
+// the same workgroup and meet at every chunk's barrier; a chunk's L2 latency
+// is then covered by two chunks of MFMA work instead of one.  The staged
+// pieces alternate between two register sets (P: even chunks, Q: odd), as do
+// the S words (E, O; each also two chunks ahead), and the loop is unrolled by
+// two, so every set is named statically and no register is copied between
+// iterations (a copy of a register still being loaded would wait for every
+// load issued before it); all loads are issued unconditionally (clamped chunk
+// and piece indices, the surplus never stored), so the wait before each LDS
+// store counts exactly the loads issued after it (254 registers, no spills).
+// Accumulation order is that of k_bt_gemm6s, so the results are identical.
+template <int NTL>
+__global__ __launch_bounds__(512) void k_bt_gemm6d(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical) {
+    constexpr int NTILE_H = NPR / 32;
+    constexpr int APT = (9 * 256 + 511) / 512;
+    static_assert(APT == 5, "reach staging registers");
+    static_assert(GNCH >= 4 && GNCH % 2 == 0, "chunk pairs");
+    __shared__ uint4 As[2][9][2][2][64];
+    const int per = (nlogical + 7) >> 3;
+    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (logical >= nlogical) return;
+    const int b = logical / G6_HB, hb = logical % G6_HB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, half = lane >> 5, l32 = lane & 31;
+    const int ncolp = D.L[1].ncolp, ntiles = ncolp >> 5;
+    const int nfirst = (ntiles + 1) >> 1;
+    const int ct0 = wave < 4 ? 0 : nfirst;
+    const int nct = wave < 4 ? nfirst : ntiles - nfirst;
+    const int ht0 = hb * G6_HT + 2 * (wave & 3);
+    const bool on0 = __builtin_amdgcn_readfirstlane(ht0 < NTILE_H ? 1 : 0) != 0;   // same across the wave
+    const bool on1 = __builtin_amdgcn_readfirstlane(ht0 + 1 < NTILE_H ? 1 : 0) != 0;
+    const uint32_t* sw = S2 + (size_t)b * GNCH * NPR * 2;
+    const size_t so0 = (size_t)((on0 ? ht0 : 0) * 32 + l32) * 2 + half;
+    const size_t so1 = (size_t)((on1 ? ht0 + 1 : 0) * 32 + l32) * 2 + half;
+    constexpr int CHW = NPR * 2;
+    const uint4* bh = reinterpret_cast<const uint4*>(D.Bh + (size_t)b * ncolp * NPR);
+    const uint4* bl = reinterpret_cast<const uint4*>(D.Bl + (size_t)b * ncolp * NPR);
+    const int npieces = ntiles * 256;
+    // per staged piece q: its source in chunk 0, as a plane (hi or lo) and a
+    // uint4 offset, and whether it is stored (a piece past the board's tiles
+    // reads piece 0's source).  Pieces come in whole 64-piece blocks, so the
+    // plane and the flag are the same across a wave.
+    const uint4* pbase[APT];
+    uint32_t poff[APT];
+    bool pon[APT];
+#pragma unroll
+    for (int q = 0; q < APT; ++q) {
+        const int e = threadIdx.x + 512 * q;
+        pon[q] = __builtin_amdgcn_readfirstlane(e < npieces ? 1 : 0) != 0;
+        const int ee = pon[q] ? e : (int)threadIdx.x;
+        const int blk = ee >> 6, ln = ee & 63;
+        const int col = (blk >> 2) * 32 + (ln & 31), j = 2 * (ln >> 5) + (blk & 1);
+        pbase[q] = __builtin_amdgcn_readfirstlane((blk >> 1) & 1) ? bl : bh;
+        poff[q] = (uint32_t)(col * (NPR / 8) + j);
+    }
+    f32x16_t acc[NTL][2];
+#pragma unroll
+    for (int i = 0; i < NTL; ++i)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][t][r] = 0.0f;
+    uint4 p0, p1, p2, p3, p4, q0, q1, q2, q3, q4;
+#define G6D_LOAD(C_, R0, R1, R2, R3, R4)                                                          \
+    do {                                                                                          \
+        const uint32_t c4_ = (uint32_t)(C_) * 4u;                                                 \
+        R0 = pbase[0][poff[0] + c4_]; R1 = pbase[1][poff[1] + c4_]; R2 = pbase[2][poff[2] + c4_]; \
+        R3 = pbase[3][poff[3] + c4_]; R4 = pbase[4][poff[4] + c4_];                               \
+    } while (0)
+#define G6D_STORE(BUF_, R0, R1, R2, R3, R4)                                                       \
+    do {                                                                                          \
+        uint4* as_ = &As[BUF_][0][0][0][0] + threadIdx.x;                                         \
+        if (pon[0]) as_[0] = R0;                                                                  \
+        if (pon[1]) as_[512] = R1;                                                                \
+        if (pon[2]) as_[1024] = R2;                                                               \
+        if (pon[3]) as_[1536] = R3;                                                               \
+        if (pon[4]) as_[2048] = R4;                                                               \
+    } while (0)
+#define G6D_MMA(BUF_, W0_, W1_)                                                                   \
+    do {                                                                                          \
+        if (on0) {                                                                                \
+            const bf16x8_t b00 = s2_frag(W0_), b01 = s2_frag((W0_) >> 16);                        \
+            const bf16x8_t b10 = s2_frag(W1_), b11 = s2_frag((W1_) >> 16);                        \
+            _Pragma("unroll") for (int i = 0; i < NTL; ++i) {                                     \
+                if (i < nct) {                                                                    \
+                    const int ct = ct0 + i;                                                       \
+                    _Pragma("unroll") for (int ks = 0; ks < 2; ++ks) {                            \
+                        bf16x8_t ah, al;                                                          \
+                        __builtin_memcpy(&ah, &As[BUF_][ct][0][ks][lane], 16);                    \
+                        __builtin_memcpy(&al, &As[BUF_][ct][1][ks][lane], 16);                    \
+                        const bf16x8_t& bq0 = ks ? b01 : b00;                                     \
+                        const bf16x8_t& bq1 = ks ? b11 : b10;                                     \
+                        acc[i][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bq0, acc[i][0], 0, 0, 0); \
+                        acc[i][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bq1, acc[i][1], 0, 0, 0); \
+                        acc[i][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bq0, acc[i][0], 0, 0, 0); \
+                        acc[i][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bq1, acc[i][1], 0, 0, 0); \
+                    }                                                                             \
+                }                                                                                 \
+            }                                                                                     \
+        }                                                                                         \
+    } while (0)
+    // S words of even (E) and odd (O) chunks, each fetched two chunks ahead
+    uint32_t wE0 = sw[so0], wE1 = sw[so1];
+    uint32_t wO0 = sw[CHW + so0], wO1 = sw[CHW + so1];
+    G6D_LOAD(0, p0, p1, p2, p3, p4);
+    G6D_LOAD(1, q0, q1, q2, q3, q4);
+    G6D_STORE(0, p0, p1, p2, p3, p4);
+    __syncthreads();
+    for (int c = 0; c < GNCH; c += 2) {
+        // even step: multiply chunk c (buffer 0); P and E take chunk c + 2; Q (chunk c + 1) into buffer 1
+        const uint32_t ce = (uint32_t)min(c + 2, GNCH - 1);
+        G6D_LOAD(ce, p0, p1, p2, p3, p4);
+        G6D_MMA(0, wE0, wE1);
+        wE0 = sw[(size_t)ce * CHW + so0];
+        wE1 = sw[(size_t)ce * CHW + so1];
+        G6D_STORE(1, q0, q1, q2, q3, q4);
+        __syncthreads();
+        // odd step: multiply chunk c + 1 (buffer 1); Q and O take chunk c + 3; P (chunk c + 2) into buffer 0
+        const uint32_t co = (uint32_t)min(c + 3, GNCH - 1);
+        G6D_LOAD(co, q0, q1, q2, q3, q4);
+        G6D_MMA(1, wO0, wO1);
+        wO0 = sw[(size_t)co * CHW + so0];
+        wO1 = sw[(size_t)co * CHW + so1];
+        if (c + 2 < GNCH) G6D_STORE(0, p0, p1, p2, p3, p4);
+        __syncthreads();
+    }
+#undef G6D_LOAD
+#undef G6D_STORE
+#undef G6D_MMA
+    float* Cb = D.C + (size_t)b * ncolp * NPR;
+#pragma unroll
+    for (int i = 0; i < NTL; ++i) {
+        if (i < nct) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                if (t == 0 ? on0 : on1) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int cc = (ct0 + i) * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+                        Cb[(size_t)cc * NPR + (ht0 + t) * 32 + l32] = 2.0f * acc[i][t][r];
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Turn all-in showdowns: C0[t][col][i] = sum_j E[t][j][i] * B0[t][col][j].
 template <int MAXC>
 __global__ __launch_bounds__(256) void k_bt_turn_sd(BatchedDev D, int ncol) {
@@ -1553,10 +1700,24 @@ int batched_create(SolverImpl& s) {
 
 // River showdown GEMM in one of its forms, by shape code: nw * 10 + pd for
 // the column-major form (k_bt_gemm5: NW column tiles per workgroup, PD K
-// chunks prefetched), 100 for the hero-major form (k_bt_gemm6) or 101 for
-// the hero-major form reading S at two bits per pair (k_bt_gemm6s).
+// chunks prefetched), 100 for the hero-major form (k_bt_gemm6), 101 for
+// the hero-major form reading S at two bits per pair (k_bt_gemm6s) and 102
+// for the latter with the reach fetched two chunks ahead (k_bt_gemm6d).
 int river_gemm(const BatchedState& B, int shape, hipStream_t st) {
     const int ntiles = B.D.L[1].ncolp / 32;
+    if (shape == 102) {
+        const int nlog = B.nb1 * G6_HB;
+        const dim3 g(((nlog + 7) / 8) * 8);
+        switch ((ntiles + 1) / 2) {
+        case 1: hipLaunchKernelGGL(k_bt_gemm6d<1>, g, dim3(512), 0, st, B.D, B.d_S2, nlog); break;
+        case 2: hipLaunchKernelGGL(k_bt_gemm6d<2>, g, dim3(512), 0, st, B.D, B.d_S2, nlog); break;
+        case 3: hipLaunchKernelGGL(k_bt_gemm6d<3>, g, dim3(512), 0, st, B.D, B.d_S2, nlog); break;
+        case 4: hipLaunchKernelGGL(k_bt_gemm6d<4>, g, dim3(512), 0, st, B.D, B.d_S2, nlog); break;
+        default: hipLaunchKernelGGL(k_bt_gemm6d<5>, g, dim3(512), 0, st, B.D, B.d_S2, nlog); break;
+        }
+        MTT_HIP(hipGetLastError());
+        return MTT_OK;
+    }
     if (shape == 101) {
         const int nlog = B.nb1 * G6_HB;
         const dim3 g(((nlog + 7) / 8) * 8);
@@ -1607,7 +1768,7 @@ int river_gemm(const BatchedState& B, int shape, hipStream_t st) {
 int river_gemm_autotune(BatchedState& B, hipStream_t st) {
     const int ntiles = B.D.L[1].ncolp / 32;
     auto valid = [&](int v) {
-        return v == 100 || v == 101 || (v / 10 >= 1 && v / 10 <= std::min(ntiles, 9) && v % 10 >= 2 && v % 10 <= 3);
+        return (v >= 100 && v <= 102) || (v / 10 >= 1 && v / 10 <= std::min(ntiles, 9) && v % 10 >= 2 && v % 10 <= 3);
     };
     if (const char* e = getenv("MTT_RIVER_GEMM")) {
         const int v = atoi(e);
@@ -1616,7 +1777,7 @@ int river_gemm_autotune(BatchedState& B, hipStream_t st) {
     const std::string key = "river_gemm_shape/tiles" + std::to_string(ntiles);
     const int v = tune_lookup(key);
     if (valid(v)) { B.gemm_shape = v; return MTT_OK; }
-    std::vector<int> cand{101, 100};
+    std::vector<int> cand{102, 101, 100};
     for (int d = 1; d <= 3; ++d) {
         const int nw = (ntiles + d - 1) / d;
         for (int pd = 2; pd <= 3; ++pd)

@@ -321,14 +321,15 @@ def test_river_sweep_matches_dense_showdown(solver_api, torch_dev, which, monkey
 @pytest.mark.parametrize("which", ["config3_small", "config3"])
 def test_river_gemm_forms_agree(solver_api, torch_dev, which, monkeypatch):
     """Every river showdown GEMM form the autotuner chooses from (hero-major,
-    with S as bytes or at two bits per pair;
+    with S as bytes or at two bits per pair, the latter with the reach
+    fetched one or two chunks ahead;
     column-major with all, half or a third of the column tiles per
     workgroup, prefetching 2 or 3 chunks) accumulates in the same order, so
     the solver states after 3 iterations are identical."""
     S = solver_api
     g, r0, r1 = getattr(S, which)(1 if which == "config3_small" else 0)
     ref = None
-    for shape in ("101", "100", "92", "93", "52", "32", "33"):
+    for shape in ("102", "101", "100", "92", "93", "52", "32", "33"):
         monkeypatch.setenv("MTT_RIVER_GEMM", shape)
         s = S.Solver(g, r0, r1, S.MODE_BF16_SPLIT)
         s.run(3)
