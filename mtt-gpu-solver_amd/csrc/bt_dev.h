@@ -86,7 +86,7 @@ struct BatchedDev {
     float* leafval;            // [49][nleaf][2][NPT] turn-leaf values (mean over rivers)
     float* rootval;            // [boards1][ntmpl1][2][NPR]
     float* rootval0;           // [49][2][NPT]
-    const uint16_t* rmap;      // [boards1][NPR] river combo -> turn combo
+    const uint16_t* rmap;      // [boards1][NPR] river combo -> turn combo (padding 0)
     const uint16_t* imap;      // [boards1][NPT] turn combo -> river combo (0xFFFF dead)
     bf16_t* Bh;                // [boards1][ncolp][NPR]
     bf16_t* Bl;
@@ -130,6 +130,30 @@ __device__ __forceinline__ float root_reach(const BatchedDev& D, const float* __
         return D.leafreach[(((size_t)t * D.nleaf + leaf) * 2 + p) * NPT + ti];
     }
     return ranges[((size_t)b * 2 + p) * np + k];
+}
+
+// Branch-free prologue pieces of the generated sweeps.  A load guarded by a
+// per-lane condition becomes a branch around it, and the compiler then waits
+// for each such load before the next; these forms load from an in-bounds
+// address on every lane (the combo index clamped to the padded count) and
+// dead lanes select zero afterwards, so all of a thread's prologue loads are
+// in flight together.
+//
+// Where combo kc's root reach lives: its turn combo (two-level game, from the
+// river-to-turn map, whose padding entries are 0) or kc itself.
+__device__ __forceinline__ int root_index(const BatchedDev& D, int level, int b, int kc) {
+    if (level == 1 && D.two_level) return D.rmap[(size_t)b * NPR + kc];
+    return kc;
+}
+
+// Root reach of player p at index ri (from root_index), as root_reach.
+__device__ __forceinline__ float root_reach_at(const BatchedDev& D, const float* __restrict__ ranges, int level,
+                                               int b, int tm, int np, int ri, int p) {
+    if (level == 1 && D.two_level) {
+        const int t = b / 48, leaf = D.tmpl_leaf[tm];
+        return D.leafreach[(((size_t)t * D.nleaf + leaf) * 2 + p) * NPT + ri];
+    }
+    return ranges[((size_t)b * 2 + p) * np + ri];
 }
 
 // BF16 of a finite float, rounded to nearest even, as raw bits.
@@ -183,6 +207,45 @@ __device__ __forceinline__ void bwd_cs(const BatchedDev& D, const LevelDev& L, f
     constexpr int cstride = LEVEL == 2 ? NPF : (LEVEL == 1 ? NPR : NPT);
     const float* Cb = (LEVEL >= 1 ? D.C : D.C0) + ((size_t)b * L.ncolp + cbase) * cstride + k;
     for (int x = 0; x < ncolt; ++x) Cs[x * NTB] = live ? Cb[(size_t)x * cstride] : 0.0f;
+}
+
+// bwd_sums for a template of NT terminals known at compile time: all of the
+// thread's partial loads first (indices clamped), then the LDS writes.
+template <int NTB, int NT>
+__device__ __forceinline__ void bwd_sums_n(const LevelDev& L, unsigned long long* ssum, int b, int tb) {
+    static_assert(NT > 0, "terminals");
+    constexpr int N = NT * 104, IT = (N + NTB - 1) / NTB;
+    const unsigned long long* P = L.part + ((size_t)b * L.nterm + tb) * 208;
+    unsigned long long s[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = min((int)threadIdx.x + i * NTB, N - 1);
+        const int tl = e / 104, r = e % 104;
+        s[i] = P[(size_t)tl * 208 + r] + P[(size_t)tl * 208 + 104 + r];
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int e = (int)threadIdx.x + i * NTB;
+        if (e < N) {
+            const int tl = e / 104, r = e % 104;
+            ssum[tl * 106 + (r / 52) * 53 + r % 52] = s[i];
+        }
+    }
+}
+
+// bwd_cs for NCOLT columns known at compile time, from the clamped combo
+// index kc: all loads first, then the LDS writes (zero on dead lanes).
+template <int LEVEL, int NTB, int NCOLT>
+__device__ __forceinline__ void bwd_cs_n(const BatchedDev& D, const LevelDev& L, float* Cs, int b, int kc,
+                                         bool live, int cbase) {
+    static_assert(NCOLT > 0, "columns");
+    constexpr int cstride = LEVEL == 2 ? NPF : (LEVEL == 1 ? NPR : NPT);
+    const float* Cb = (LEVEL >= 1 ? D.C : D.C0) + ((size_t)b * L.ncolp + cbase) * cstride + kc;
+    float v[NCOLT];
+#pragma unroll
+    for (int x = 0; x < NCOLT; ++x) v[x] = Cb[(size_t)x * cstride];
+#pragma unroll
+    for (int x = 0; x < NCOLT; ++x) Cs[x * NTB] = live ? v[x] : 0.0f;
 }
 
 // Both parts (the interpreted kernels' prologue).  No barrier: the caller

@@ -130,9 +130,16 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int t) {
     o("case %d: {", t);
     o("fwd_bins_clear<NT>(bins, %d);", T.nt);
     o("const size_t rb = ((size_t)b * L.nslot + %d) * NP + k;", T.s0);
-    o("const float* __restrict__ Rg = L.R + rb;");
+    o("const float* __restrict__ Rl = L.R + ((size_t)b * L.nslot + %d) * NP + kc;", T.s0);
     o("float* __restrict__ Sg = L.S + rb;");
-    for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rg[%d] : 0.0f;", j, j * J.np);
+    o("const int c1r = L.cc[((size_t)b * NP + kc) * 2], c2r = L.cc[((size_t)b * NP + kc) * 2 + 1];");
+    o("const int ri = root_index(D, LEVEL, b, kc);");
+    for (int j = 0; j < T.nsl; ++j) o("const float Rv%d = Rl[%d];", j, j * J.np);
+    o("const float rr0 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 0);");
+    o("const float rr1 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 1);");
+    o("const float root0 = live ? rr0 : 0.0f, root1 = live ? rr1 : 0.0f;");
+    o("const int c1 = live ? c1r : 0, c2 = live ? c2r : 0;");
+    for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rv%d : 0.0f;", j, j);
     o("__syncthreads();");
     int stack[16];
     for (int e = T.e0; e < T.e1; ++e) {
@@ -206,7 +213,7 @@ void gen_backward_case(Src& o, const JitLevel& J, int passes, int t) {
     if (passes > 1) {
         // card sums and totals once; then each slice of the combos in turn (the showdown values a thread stages
         // in LDS are its own, so the passes need no barrier)
-        o("bwd_sums<NTB>(L, ssum, b, %d, %d);", T.nt, T.t0);
+        if (T.nt > 0) o("bwd_sums_n<NTB, %d>(L, ssum, b, %d);", T.nt, T.t0);
         o("bwd_totals<NTB>(ssum, %d);", T.nt);
         o("#pragma unroll 1");  // one walk's registers at a time
         o("for (int pass = 0; pass < %d; ++pass) {", passes);
@@ -215,19 +222,23 @@ void gen_backward_case(Src& o, const JitLevel& J, int passes, int t) {
         o("int soff = 0;");
         o("asm volatile(\"\" : \"+v\"(soff));");
         o("const unsigned long long* ssv = ssum + soff;");
+        // branch-free prologue (clamped combo index, dead lanes zeroed after the loads): the cards and the
+        // root index first, then the regrets and the showdown values, then the root reaches, so every load
+        // of the slice is in flight before the first wait
         o("const int k = pass * NTB + tid;");
         o("const bool live = k < nl;");
-        o("int c1 = 0, c2 = 0;");
-        o("float root0 = 0.0f, root1 = 0.0f;");
-        o("if (live) {");
-        o("c1 = L.cc[((size_t)b * NP + k) * 2];");
-        o("c2 = L.cc[((size_t)b * NP + k) * 2 + 1];");
-        o("root0 = root_reach(D, ranges, LEVEL, b, tm, NP, k, 0);");
-        o("root1 = root_reach(D, ranges, LEVEL, b, tm, NP, k, 1);");
-        o("}");
-        o("bwd_cs<LEVEL, NTB>(D, L, Cs, b, k, live, %d, %d);", T.c0, T.ncolt);
+        o("const int kc = min(k, NP - 1);");
+        o("const int c1r = L.cc[((size_t)b * NP + kc) * 2], c2r = L.cc[((size_t)b * NP + kc) * 2 + 1];");
+        o("const int ri = root_index(D, LEVEL, b, kc);");
+        o("const float* __restrict__ Rl = L.R + ((size_t)b * L.nslot + %d) * NP + kc;", T.s0);
+        for (int j = 0; j < T.nsl; ++j) o("const float Rv%d = Rl[%d];", j, j * J.np);
+        if (T.ncolt > 0) o("bwd_cs_n<LEVEL, NTB, %d>(D, L, Cs, b, kc, live, %d);", T.ncolt, T.c0);
+        o("const float rr0 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 0);");
+        o("const float rr1 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 1);");
+        o("const int c1 = live ? c1r : 0, c2 = live ? c2r : 0;");
+        o("const float root0 = live ? rr0 : 0.0f, root1 = live ? rr1 : 0.0f;");
         o("float* __restrict__ Rg = L.R + ((size_t)b * L.nslot + %d) * NP + k;", T.s0);
-        for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rg[%d] : 0.0f;", j, j * J.np);
+        for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rv%d : 0.0f;", j, j);
         o("float out0 = 0.0f, out1 = 0.0f;");
     } else {
         o("bwd_stage<LEVEL, NTB>(D, L, ssum, Cs, b, k, live, %d, %d, %d, %d);", T.nt, T.t0, T.c0, T.ncolt);
@@ -331,14 +342,9 @@ void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
     o("const int k = half * NT + (int)threadIdx.x;");
     o("const bool inb = k < NP;");
     o("const bool live = k < ldu(L.nlive + b);");
-    o("int c1 = 0, c2 = 0;");
-    o("float root0 = 0.0f, root1 = 0.0f;");
-    o("if (live) {");
-    o("c1 = L.cc[((size_t)b * NP + k) * 2];");
-    o("c2 = L.cc[((size_t)b * NP + k) * 2 + 1];");
-    o("root0 = root_reach(D, ranges, LEVEL, b, tm, NP, k, 0);");
-    o("root1 = root_reach(D, ranges, LEVEL, b, tm, NP, k, 1);");
-    o("}");
+    // branch-free prologue: clamped combo index, dead lanes zeroed after the loads (each case requests the
+    // cards and the root index, then its regrets, then the root reaches)
+    o("const int kc = min(k, NP - 1);");
     o("(void)inb;");
     o("switch (tm) {");
     for (int t = 0; t < J.ntmpl; ++t) gen_forward_case(o, J, cfg.sd_fp32 != 0, t);

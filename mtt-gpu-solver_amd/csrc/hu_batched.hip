@@ -1455,7 +1455,8 @@ int batched_create(SolverImpl& s) {
     // ---------------------------------------------------------- boards --
     const int nb1 = two ? 49 * 48 : 1;
     B.nb1 = nb1;
-    std::vector<uint16_t> rank1((size_t)nb1 * NPR, 0), rmap((size_t)nb1 * NPR, 0xFFFF);
+    // rmap padding is 0 (a valid turn combo): the generated sweeps read it on every lane
+    std::vector<uint16_t> rank1((size_t)nb1 * NPR, 0), rmap((size_t)nb1 * NPR, 0);
     std::vector<uint32_t> cx((size_t)nb1 * NPR, 0x80808080u), cy((size_t)nb1 * NPR, 0x40404040u);
     std::vector<uint8_t> cc1((size_t)nb1 * NPR * 2, 0);
     std::vector<int> nlive1(nb1);
