@@ -150,7 +150,7 @@ __device__ __forceinline__ int root_index(const BatchedDev& D, int level, int b,
 __device__ __forceinline__ float root_reach_at(const BatchedDev& D, const float* __restrict__ ranges, int level,
                                                int b, int tm, int np, int ri, int p) {
     if (level == 1 && D.two_level) {
-        const int t = b / 48, leaf = D.tmpl_leaf[tm];
+        const int t = b / 48, leaf = ldu(D.tmpl_leaf + tm);  // scalar load: no wait on the vector loads
         return D.leafreach[(((size_t)t * D.nleaf + leaf) * 2 + p) * NPT + ri];
     }
     return ranges[((size_t)b * 2 + p) * np + ri];
