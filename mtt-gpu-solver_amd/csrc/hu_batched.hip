@@ -1049,32 +1049,58 @@ __global__ __launch_bounds__(512) void k_bt_gemm6d(BatchedDev D, const uint32_t*
 }
 
 // Turn all-in showdowns: C0[t][col][i] = sum_j E[t][j][i] * B0[t][col][j].
+// One workgroup = 64 heroes of one turn; its four waves each take a quarter
+// of the villains (882 workgroups for 49 turns, several waves per SIMD),
+// stage their columns' reach 32 villains at a time in LDS (every load issued
+// from an in-bounds column before the LDS writes), and the quarters are
+// added in a fixed order at the end, so the result is deterministic.
 template <int MAXC>
 __global__ __launch_bounds__(256) void k_bt_turn_sd(BatchedDev D, int ncol) {
+    constexpr int KQ = NPT / 4, CH = 32, NST = 4 * MAXC * CH / 256;
+    static_assert(NPT % 64 == 0 && KQ % CH == 0 && (4 * MAXC * CH) % 256 == 0, "turn showdown tiling");
+    if (ncol <= 0) return;
     const int t = blockIdx.y;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    __shared__ float bcol[MAXC][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 64 + lane;
+    __shared__ float bcol[4][MAXC][CH];
+    __shared__ float red[3][MAXC][64];
     float acc[MAXC];
 #pragma unroll
     for (int c = 0; c < MAXC; ++c) acc[c] = 0.0f;
     const float* E = D.Eturn + (size_t)t * NPT * NPT;
     const float* B = D.B0 + (size_t)t * D.L[0].ncolp * NPT;
-    for (int j0 = 0; j0 < NPT; j0 += 64) {
-        for (int e = threadIdx.x; e < MAXC * 64; e += 256) {
-            int c = e / 64, jj = e % 64;
-            bcol[c][jj] = c < ncol ? B[(size_t)c * NPT + j0 + jj] : 0.0f;
+    for (int st = 0; st < KQ / CH; ++st) {
+        float v[NST];
+#pragma unroll
+        for (int q = 0; q < NST; ++q) {
+            const int e = threadIdx.x + 256 * q;
+            const int w = e / (MAXC * CH), c = (e / CH) % MAXC, jj = e % CH;
+            v[q] = B[(size_t)min(c, ncol - 1) * NPT + w * KQ + st * CH + jj];
+        }
+#pragma unroll
+        for (int q = 0; q < NST; ++q) {
+            const int e = threadIdx.x + 256 * q;
+            const int c = (e / CH) % MAXC;
+            (&bcol[0][0][0])[e] = c < ncol ? v[q] : 0.0f;
         }
         __syncthreads();
-        if (i < NPT)
-            for (int jj = 0; jj < 64; ++jj) {
-                float ev = E[(size_t)(j0 + jj) * NPT + i];
+        const float* ew = E + (size_t)(wave * KQ + st * CH) * NPT + i;
+#pragma unroll 8
+        for (int jj = 0; jj < CH; ++jj) {
+            const float ev = ew[(size_t)jj * NPT];
 #pragma unroll
-                for (int c = 0; c < MAXC; ++c) acc[c] = fmaf(ev, bcol[c][jj], acc[c]);
-            }
+            for (int c = 0; c < MAXC; ++c) acc[c] = fmaf(ev, bcol[wave][c][jj], acc[c]);
+        }
         __syncthreads();
     }
-    if (i >= NPT) return;
-    for (int c = 0; c < ncol; ++c) D.C0[((size_t)t * D.L[0].ncolp + c) * NPT + i] = acc[c];
+    if (wave) {
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) red[wave - 1][c][lane] = acc[c];
+    }
+    __syncthreads();
+    if (wave) return;
+    for (int c = 0; c < ncol; ++c)
+        D.C0[((size_t)t * D.L[0].ncolp + c) * NPT + i] = ((acc[c] + red[0][c][lane]) + red[1][c][lane]) + red[2][c][lane];
 }
 
 // Turn equity over the 44 rivers in the turn-compact layout (villain-major).
@@ -1864,7 +1890,7 @@ int batched_run(SolverImpl& s, int iters) {
             MTT_TIMED(s, "turn_leafval",
                       hipLaunchKernelGGL(k_bt_leafval, dim3(NPT / 256 + 1, 49 * B.D.nleaf), dim3(256), 0, s.stream, B.D));
             MTT_TIMED(s, "turn_showdown",
-                      hipLaunchKernelGGL(k_bt_turn_sd<16>, dim3(NPT / 256 + 1, 49), dim3(256), 0, s.stream, B.D,
+                      hipLaunchKernelGGL(k_bt_turn_sd<16>, dim3(NPT / 64, 49), dim3(256), 0, s.stream, B.D,
                                          L0.ncol));
             const int jnt = B.jit.cfg.bwd_nt, jnh = bwd_groups(0, B.jit.cfg);
             const size_t sh = B.jit.active ? bwd_lds(L0, jnt) : bwd_lds(L0);
