@@ -186,6 +186,18 @@ __device__ __forceinline__ void fwd_bins_flush(const LevelDev& L, const unsigned
     for (int e = threadIdx.x; e < nt * 104; e += NT) out[(size_t)(e / 104) * 208 + e % 104] = bins[e];
 }
 
+// The same with the bins kept in two copies (lanes of even and odd index
+// add into copy 0 and 1, halving same-address LDS atomics within a wave);
+// the copies are added here (exact 64-bit sums).
+template <int NT>
+__device__ __forceinline__ void fwd_bins_flush2(const LevelDev& L, const unsigned long long* bins, int b, int tb,
+                                                int half, int nt) {
+    __syncthreads();
+    unsigned long long* out = L.part + ((size_t)b * L.nterm + tb) * 208 + half * 104;
+    for (int e = threadIdx.x; e < nt * 104; e += NT)
+        out[(size_t)(e / 104) * 208 + e % 104] = bins[e] + bins[nt * 104 + e];
+}
+
 // Backward sweep, first part of the prologue: the card sums of the
 // template's nt terminals (both forward halves added) into ssum
 // ([nt][2][53], slot 52 left for the total).  No barrier.

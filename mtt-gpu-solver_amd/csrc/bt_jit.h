@@ -48,6 +48,8 @@ struct JitConfig {
     int bwd_passes = 2;
     // river showdown columns: 0 = BF16 hi + lo (Bh, Bl) for the GEMM, 1 = FP32 (Bf) for the strength-order sweep
     int sd_fp32 = 0;
+    // copies of the forward's card-sum bins in LDS (1 or 2, by lane parity)
+    int fwd_bins = 2;
 };
 
 // Passes of the backward of a level (0: turn, 1: river) under cfg.
@@ -71,6 +73,9 @@ struct JitSweeps {
 
 // Threads per workgroup of the generated forward sweeps.
 constexpr int JIT_NT = 576;
+
+// LDS bytes of a generated forward over templates of at most maxterm terminals.
+inline size_t fwd_lds(int maxterm, const JitConfig& cfg) { return (size_t)maxterm * 104 * 8 * cfg.fwd_bins; }
 
 // Generates, compiles and loads the sweeps of the given levels for the
 // current device.  Compiled modules are cached per (device, source) for the

@@ -117,7 +117,7 @@ void emit_strategy(Src& o, const Tmpl& T, int n) {
 void emit_bins(Src& o, const Tmpl& T, int n) {
     const DNode& nd = T.node(n);
     o("if (live) {");
-    o("unsigned long long* bn = bins + %d;", (nd.sidx - T.t0) * 104);
+    o("unsigned long long* bn = binl + %d;", (nd.sidx - T.t0) * 104);
     o("const unsigned long long u0 = (unsigned long long)__double2ull_rn((double)x0_%d * FX);", n);
     o("const unsigned long long u1 = (unsigned long long)__double2ull_rn((double)x1_%d * FX);", n);
     o("if (u0) { atomicAdd(&bn[c1], u0); atomicAdd(&bn[c2], u0); }");
@@ -125,10 +125,12 @@ void emit_bins(Src& o, const Tmpl& T, int n) {
     o("}");
 }
 
-void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int t) {
+void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int t) {
     Tmpl T(J, t);
     o("case %d: {", t);
-    o("fwd_bins_clear<NT>(bins, %d);", T.nt);
+    o("fwd_bins_clear<NT>(bins, %d);", T.nt * nbins);
+    if (nbins == 2) o("unsigned long long* const binl = bins + (threadIdx.x & 1) * %d;", T.nt * 104);
+    else o("unsigned long long* const binl = bins;");
     o("const size_t rb = ((size_t)b * L.nslot + %d) * NP + k;", T.s0);
     o("const float* __restrict__ Rl = L.R + ((size_t)b * L.nslot + %d) * NP + kc;", T.s0);
     o("float* __restrict__ Sg = L.S + rb;");
@@ -188,7 +190,8 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int t) {
         }
         emit_bins(o, T, n);
     }
-    o("fwd_bins_flush<NT>(L, bins, b, %d, half, %d);", T.t0, T.nt);
+    if (nbins == 2) o("fwd_bins_flush2<NT>(L, bins, b, %d, half, %d);", T.t0, T.nt);
+    else o("fwd_bins_flush<NT>(L, bins, b, %d, half, %d);", T.t0, T.nt);
     o("break;");
     o("}");
 }
@@ -347,7 +350,7 @@ void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
     o("const int kc = min(k, NP - 1);");
     o("(void)inb;");
     o("switch (tm) {");
-    for (int t = 0; t < J.ntmpl; ++t) gen_forward_case(o, J, cfg.sd_fp32 != 0, t);
+    for (int t = 0; t < J.ntmpl; ++t) gen_forward_case(o, J, cfg.sd_fp32 != 0, cfg.fwd_bins, t);
     o("default: break;");
     o("}");
     o("}");
@@ -429,6 +432,7 @@ JitConfig bt_jit_config() {
     c.bwd_waves = env("MTT_JIT_BWD_WAVES", 0, 8, c.bwd_waves);
     c.fwd_waves = env("MTT_JIT_FWD_WAVES", 0, 8, c.fwd_waves);
     c.bwd_passes = env("MTT_JIT_BWD_PASSES", 1, 2, c.bwd_passes);
+    c.fwd_bins = env("MTT_JIT_FWD_BINS", 1, 2, c.fwd_bins);
     return c;
 }
 
@@ -494,6 +498,7 @@ int bt_jit_build(const JitLevel* levels, int n, const JitConfig& cfg, JitSweeps*
     MTT_ARG(cfg.bwd_nt == 384 || cfg.bwd_nt == 576, "generated sweeps: backward workgroup %d not 384 or 576",
             cfg.bwd_nt);
     MTT_ARG(cfg.bwd_passes == 1 || cfg.bwd_passes == 2, "generated sweeps: backward passes %d not 1 or 2", cfg.bwd_passes);
+    MTT_ARG(cfg.fwd_bins == 1 || cfg.fwd_bins == 2, "generated sweeps: forward bin copies %d not 1 or 2", cfg.fwd_bins);
     const std::string src = bt_jit_source(levels, n, cfg);
     int dev = 0;
     MTT_HIP(hipGetDevice(&dev));

@@ -388,7 +388,9 @@ __global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* 
 }
 
 // Turn-leaf values: mean over the 44 rivers that leave the combo alive of
-// the river-line root values (rivers are summed in deck order).
+// the river-line root values (rivers are summed in deck order).  Rivers go
+// 16 at a time: their map entries, then (from an in-bounds index on every
+// lane) their root values, then the sums, so no load waits alone.
 __global__ void k_bt_leafval(BatchedDev D) {
     const int t = blockIdx.y / D.nleaf, leaf = blockIdx.y % D.nleaf;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -396,13 +398,24 @@ __global__ void k_bt_leafval(BatchedDev D) {
     const int rt = D.leaf_tmpl[leaf];
     const int nt1 = D.L[1].ntmpl;
     float a0 = 0.0f, a1 = 0.0f;
-    for (int rk = 0; rk < 48; ++rk) {
-        const int bb = t * 48 + rk;
-        const int ri = D.imap[(size_t)bb * NPT + k];
-        if (ri == 0xFFFF) continue;
-        const float* rv = D.rootval + (((size_t)bb * nt1 + rt) * 2) * NPR + ri;
-        a0 += rv[0];
-        a1 += rv[NPR];
+    constexpr int RG = 16;
+    for (int r0 = 0; r0 < 48; r0 += RG) {
+        int ri[RG];
+        float v0[RG], v1[RG];
+#pragma unroll
+        for (int q = 0; q < RG; ++q) ri[q] = D.imap[(size_t)(t * 48 + r0 + q) * NPT + k];
+#pragma unroll
+        for (int q = 0; q < RG; ++q) {
+            const float* rv = D.rootval + (((size_t)(t * 48 + r0 + q) * nt1 + rt) * 2) * NPR + (ri[q] == 0xFFFF ? 0 : ri[q]);
+            v0[q] = rv[0];
+            v1[q] = rv[NPR];
+        }
+#pragma unroll
+        for (int q = 0; q < RG; ++q) {
+            const bool on = ri[q] != 0xFFFF;
+            a0 = on ? a0 + v0[q] : a0;
+            a1 = on ? a1 + v1[q] : a1;
+        }
     }
     float* lv = D.leafval + ((size_t)t * D.nleaf + leaf) * 2 * NPT + k;
     lv[0] = a0 * (1.0f / 44.0f);
@@ -1851,7 +1864,7 @@ int batched_run(SolverImpl& s, int iters) {
         const float* ranges = B.d_ranges;
         if (B.two_level) {
             const LevelDev& L0 = B.D.L[0];
-            const size_t sh = (size_t)L0.maxterm * 104 * 8;
+            const size_t sh = B.jit.active ? fwd_lds(L0.maxterm, B.jit.cfg) : (size_t)L0.maxterm * 104 * 8;
             const dim3 g(49 * L0.ntmpl * 2);
             void* args[] = {&B.D, &ranges, &ws};
             MTT_TIMED(s, "turn_forward",
@@ -1860,7 +1873,7 @@ int batched_run(SolverImpl& s, int iters) {
                       else hipLaunchKernelGGL((k_bt_forward<0, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
         {
-            const size_t sh = (size_t)L1.maxterm * 104 * 8;
+            const size_t sh = B.jit.active ? fwd_lds(L1.maxterm, B.jit.cfg) : (size_t)L1.maxterm * 104 * 8;
             const dim3 g(B.nb1 * L1.ntmpl * 2);
             void* args[] = {&B.D, &ranges, &ws};
             MTT_TIMED(s, "river_forward",
