@@ -32,17 +32,19 @@ struct JitLevel {
 
 // Launch shapes of the generated sweeps.  The forward always runs two
 // workgroups of 576 per board (as the interpreted kernels).  The turn
-// backward runs ceil(1152 / bwd_nt) workgroups of bwd_nt threads (576 or
-// 384) per (board, template); the river backward does too when
-// bwd_passes = 1, and with bwd_passes = 2 (the default) runs one workgroup of
-// bwd_nt threads per (board, template) that walks the combos in
+// backward runs ceil(1152 / bwd_nt) workgroups of bwd_nt threads (a
+// multiple of 64 in 192..576) per (board, template); the river backward does
+// too when bwd_passes = 1, and with bwd_passes = 2 (the default) runs one
+// workgroup of bwd_nt threads per (board, template) that walks the combos in
 // ceil(1088 / bwd_nt) slices in turn, so the card sums of the template's
-// terminals are staged and totalled once (576 threads: 9 waves per CU with
-// a few spilled registers beat 384 threads without spills, 6 waves per CU).
-// *_waves > 0 asks the compiler to fit that many waves per SIMD
+// terminals are staged and totalled once.  256 threads (the default): the
+// walk fits its 187 registers without spills and two independent 4-wave
+// workgroups share a CU, so one's loads overlap the other's walk; 576
+// threads (one 9-wave workgroup per CU, 168 registers with spills) took 29%
+// longer.  *_waves > 0 asks the compiler to fit that many waves per SIMD
 // (amdgpu_waves_per_eu).
 struct JitConfig {
-    int bwd_nt = 576;
+    int bwd_nt = 256;
     int bwd_waves = 0;
     int fwd_waves = 0;
     int bwd_passes = 2;

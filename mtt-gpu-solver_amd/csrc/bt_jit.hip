@@ -228,6 +228,8 @@ void gen_backward_case(Src& o, const JitLevel& J, int passes, int t) {
         // branch-free prologue (clamped combo index, dead lanes zeroed after the loads): the cards and the
         // root index first, then the regrets and the showdown values, then the root reaches, so every load
         // of the slice is in flight before the first wait
+        // a wave whose combos all lie past the padded count has nothing to walk (no barrier follows)
+        o("if (__builtin_amdgcn_readfirstlane(pass * NTB + (tid & ~63)) >= NP) continue;");
         o("const int k = pass * NTB + tid;");
         o("const bool live = k < nl;");
         o("const int kc = min(k, NP - 1);");
@@ -427,8 +429,8 @@ JitConfig bt_jit_config() {
         const long v = strtol(e, &end, 10);
         return (end && *end == 0 && v >= lo && v <= hi) ? (int)v : def;
     };
-    c.bwd_nt = env("MTT_JIT_BWD_NT", 384, 576, c.bwd_nt);
-    if (c.bwd_nt != 384 && c.bwd_nt != 576) c.bwd_nt = 576;
+    c.bwd_nt = env("MTT_JIT_BWD_NT", 192, 576, c.bwd_nt);
+    if (c.bwd_nt % 64) c.bwd_nt = JitConfig().bwd_nt;
     c.bwd_waves = env("MTT_JIT_BWD_WAVES", 0, 8, c.bwd_waves);
     c.fwd_waves = env("MTT_JIT_FWD_WAVES", 0, 8, c.fwd_waves);
     c.bwd_passes = env("MTT_JIT_BWD_PASSES", 1, 2, c.bwd_passes);
@@ -495,8 +497,8 @@ int bt_jit_build(const JitLevel* levels, int n, const JitConfig& cfg, JitSweeps*
     *out = JitSweeps();
     int rc = check_levels(levels, n);
     if (rc) return rc;
-    MTT_ARG(cfg.bwd_nt == 384 || cfg.bwd_nt == 576, "generated sweeps: backward workgroup %d not 384 or 576",
-            cfg.bwd_nt);
+    MTT_ARG(cfg.bwd_nt >= 192 && cfg.bwd_nt <= 576 && cfg.bwd_nt % 64 == 0,
+            "generated sweeps: backward workgroup %d not a multiple of 64 in 192..576", cfg.bwd_nt);
     MTT_ARG(cfg.bwd_passes == 1 || cfg.bwd_passes == 2, "generated sweeps: backward passes %d not 1 or 2", cfg.bwd_passes);
     MTT_ARG(cfg.fwd_bins == 1 || cfg.fwd_bins == 2, "generated sweeps: forward bin copies %d not 1 or 2", cfg.fwd_bins);
     const std::string src = bt_jit_source(levels, n, cfg);
