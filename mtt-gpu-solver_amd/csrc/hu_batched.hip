@@ -929,7 +929,8 @@ __global__ __launch_bounds__(512) void k_bt_gemm6s(BatchedDev D, const uint32_t*
 template <int NTL>
 __global__ __launch_bounds__(512) void k_bt_gemm6d(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical) {
     constexpr int NTILE_H = NPR / 32;
-    constexpr int APT = (9 * 256 + 511) / 512;
+    constexpr int NT = 512;
+    constexpr int APT = (9 * 256 + NT - 1) / NT;
     static_assert(APT == 5, "reach staging registers");
     static_assert(GNCH >= 4 && GNCH % 2 == 0, "chunk pairs");
     __shared__ uint4 As[2][9][2][2][64];
@@ -941,7 +942,7 @@ __global__ __launch_bounds__(512) void k_bt_gemm6d(BatchedDev D, const uint32_t*
     const int ncolp = D.L[1].ncolp, ntiles = ncolp >> 5;
     const int nfirst = (ntiles + 1) >> 1;
     const int ct0 = wave < 4 ? 0 : nfirst;
-    const int nct = wave < 4 ? nfirst : ntiles - nfirst;
+    const int nct = wave < 4 ? nfirst : ntiles - nfirst;       // NTL or NTL - 1
     const int ht0 = hb * G6_HT + 2 * (wave & 3);
     const bool on0 = __builtin_amdgcn_readfirstlane(ht0 < NTILE_H ? 1 : 0) != 0;   // same across the wave
     const bool on1 = __builtin_amdgcn_readfirstlane(ht0 + 1 < NTILE_H ? 1 : 0) != 0;
@@ -961,7 +962,7 @@ __global__ __launch_bounds__(512) void k_bt_gemm6d(BatchedDev D, const uint32_t*
     bool pon[APT];
 #pragma unroll
     for (int q = 0; q < APT; ++q) {
-        const int e = threadIdx.x + 512 * q;
+        const int e = threadIdx.x + NT * q;
         pon[q] = __builtin_amdgcn_readfirstlane(e < npieces ? 1 : 0) != 0;
         const int ee = pon[q] ? e : (int)threadIdx.x;
         const int blk = ee >> 6, ln = ee & 63;
@@ -987,10 +988,10 @@ __global__ __launch_bounds__(512) void k_bt_gemm6d(BatchedDev D, const uint32_t*
     do {                                                                                          \
         uint4* as_ = &As[BUF_][0][0][0][0] + threadIdx.x;                                         \
         if (pon[0]) as_[0] = R0;                                                                  \
-        if (pon[1]) as_[512] = R1;                                                                \
-        if (pon[2]) as_[1024] = R2;                                                               \
-        if (pon[3]) as_[1536] = R3;                                                               \
-        if (pon[4]) as_[2048] = R4;                                                               \
+        if (pon[1]) as_[NT] = R1;                                                                 \
+        if (pon[2]) as_[2 * NT] = R2;                                                             \
+        if (pon[3]) as_[3 * NT] = R3;                                                             \
+        if (pon[4]) as_[4 * NT] = R4;                                                             \
     } while (0)
 #define G6D_MMA(BUF_, W0_, W1_)                                                                   \
     do {                                                                                          \
@@ -998,7 +999,7 @@ __global__ __launch_bounds__(512) void k_bt_gemm6d(BatchedDev D, const uint32_t*
             const bf16x8_t b00 = s2_frag(W0_), b01 = s2_frag((W0_) >> 16);                        \
             const bf16x8_t b10 = s2_frag(W1_), b11 = s2_frag((W1_) >> 16);                        \
             _Pragma("unroll") for (int i = 0; i < NTL; ++i) {                                     \
-                if (i < nct) {                                                                    \
+                if (i < NTL - 1 || i < nct) {  /* only the last tile can be absent */             \
                     const int ct = ct0 + i;                                                       \
                     _Pragma("unroll") for (int ks = 0; ks < 2; ++ks) {                            \
                         bf16x8_t ah, al;                                                          \
