@@ -155,9 +155,12 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int t)
             stack[d] = n;
             emit_strategy(o, T, n);
             const int ls = nd.slot - T.s0;
+            // a zero reach adds +0 to every strategy sum (all sums are >= +0), so its atomics are skipped
             o("if (live) {");
             o("const float wr = ws * x%d_%d;", nd.player, n);
+            o("if (wr != 0.0f) {");
             for (int a = 0; a < nd.nact; ++a) o("unsafeAtomicAdd(Sg + %d, wr * s%d_%d);", (ls + a) * J.np, n, a);
+            o("}");
             o("}");
             continue;
         }
@@ -310,10 +313,11 @@ void gen_backward_case(Src& o, const JitLevel& J, int passes, int t) {
         o("{");
         o("float nv = 0.0f;");
         for (int a = 0; a < nd.nact; ++a) o("nv = fmaf(s%d_%d, cv%d_%d, nv);", n, a, n, a);
+        // a regret whose update leaves its bits unchanged is not stored again
         o("if (live) {");
         for (int a = 0; a < nd.nact; ++a) {
-            o("{ const float q = R%d + (cv%d_%d - nv); Rg[%d] = q * (q > 0.0f ? wp : wn); }", ls + a, n, a,
-              (ls + a) * J.np);
+            o("{ const float q = R%d + (cv%d_%d - nv), nr = q * (q > 0.0f ? wp : wn);", ls + a, n, a);
+            o("if (__float_as_uint(nr) != __float_as_uint(R%d)) Rg[%d] = nr; }", ls + a, (ls + a) * J.np);
         }
         o("}");
         o("v%d_%d = nv;", pl, n);
