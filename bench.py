@@ -145,7 +145,7 @@ def main():
         print(json.dumps({"copy_calibration": {"bytes_read": n, "bytes_written": n, "launches": 3}}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+        print(json.dumps({"error": "this solver runs on one GPU; sharding across GPUs is designed but not built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     S = load("mtt_solver", os.path.join(ROOT, "mtt-gpu-solver_amd", "solver.py"))
