@@ -223,7 +223,8 @@ __device__ __forceinline__ void bwd_cs(const BatchedDev& D, const LevelDev& L, f
 
 // bwd_sums for a template of NT terminals known at compile time: all of the
 // thread's partial loads first (indices clamped), then the LDS writes.
-template <int NTB, int NT>
+// HALVES = 1 when the forward wrote its sums to the first half only.
+template <int NTB, int NT, int HALVES>
 __device__ __forceinline__ void bwd_sums_n(const LevelDev& L, unsigned long long* ssum, int b, int tb) {
     static_assert(NT > 0, "terminals");
     constexpr int N = NT * 104, IT = (N + NTB - 1) / NTB;
@@ -233,7 +234,7 @@ __device__ __forceinline__ void bwd_sums_n(const LevelDev& L, unsigned long long
     for (int i = 0; i < IT; ++i) {
         const int e = min((int)threadIdx.x + i * NTB, N - 1);
         const int tl = e / 104, r = e % 104;
-        s[i] = P[(size_t)tl * 208 + r] + P[(size_t)tl * 208 + 104 + r];
+        s[i] = HALVES == 2 ? P[(size_t)tl * 208 + r] + P[(size_t)tl * 208 + 104 + r] : P[(size_t)tl * 208 + r];
     }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {

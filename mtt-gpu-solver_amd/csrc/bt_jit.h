@@ -52,6 +52,14 @@ struct JitConfig {
     int sd_fp32 = 0;
     // copies of the forward's card-sum bins in LDS (1 or 2, by lane parity)
     int fwd_bins = 2;
+    // river forward workgroup: 576 = two workgroups per (board, template),
+    // one per half of the combos; less = one workgroup walking the combos in
+    // slices (its card sums go to the first half of the partials only, which
+    // the backward then reads alone).  384 (three slices, the default): river
+    // forward 3.48 -> 3.30 ms and backward 2.31 -> 2.26 ms against 576.  The
+    // turn level keeps the two half workgroups (49 boards give too few
+    // workgroups to slice).
+    int fwd_nt = 384;
 };
 
 // Passes of the backward of a level (0: turn, 1: river) under cfg.
@@ -73,8 +81,14 @@ struct JitSweeps {
     hipFunction_t bwd[2] = {nullptr, nullptr};
 };
 
-// Threads per workgroup of the generated forward sweeps.
-constexpr int JIT_NT = 576;
+// Whether the generated forward of a level walks the combos in slices (one
+// workgroup per (board, template)) or runs two half workgroups, and its
+// workgroup shape.
+inline bool fwd_sliced(int level, const JitConfig& cfg) { return level == 1 && cfg.fwd_nt != 576; }
+
+// Threads per workgroup of the generated forward at a level.
+inline int fwd_threads(int level, const JitConfig& cfg) { return fwd_sliced(level, cfg) ? cfg.fwd_nt : 576; }
+inline int fwd_groups(int level, const JitConfig& cfg) { return fwd_sliced(level, cfg) ? 1 : 2; }
 
 // LDS bytes of a generated forward over templates of at most maxterm terminals.
 inline size_t fwd_lds(int maxterm, const JitConfig& cfg) { return (size_t)maxterm * 104 * 8 * cfg.fwd_bins; }

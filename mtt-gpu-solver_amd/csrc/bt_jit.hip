@@ -125,12 +125,23 @@ void emit_bins(Src& o, const Tmpl& T, int n) {
     o("}");
 }
 
-void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int t) {
+void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int slices, int t) {
     Tmpl T(J, t);
     o("case %d: {", t);
     o("fwd_bins_clear<NT>(bins, %d);", T.nt * nbins);
     if (nbins == 2) o("unsigned long long* const binl = bins + (threadIdx.x & 1) * %d;", T.nt * 104);
     else o("unsigned long long* const binl = bins;");
+    if (slices > 1) {
+        // one workgroup walks the combos in slices, adding every slice into the same bins
+        o("__syncthreads();");
+        o("#pragma unroll 1");
+        o("for (int pass = 0; pass < %d; ++pass) {", slices);
+        o("if (__builtin_amdgcn_readfirstlane(pass * NT + ((int)threadIdx.x & ~63)) >= NP) continue;");
+        o("const int k = pass * NT + (int)threadIdx.x;");
+        o("const bool inb = k < NP, live = k < nl;");
+        o("const int kc = min(k, NP - 1);");
+        o("(void)inb;");
+    }
     o("const size_t rb = ((size_t)b * L.nslot + %d) * NP + k;", T.s0);
     o("const float* __restrict__ Rl = L.R + ((size_t)b * L.nslot + %d) * NP + kc;", T.s0);
     o("float* __restrict__ Sg = L.S + rb;");
@@ -142,7 +153,7 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int t)
     o("const float root0 = live ? rr0 : 0.0f, root1 = live ? rr1 : 0.0f;");
     o("const int c1 = live ? c1r : 0, c2 = live ? c2r : 0;");
     for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rv%d : 0.0f;", j, j);
-    o("__syncthreads();");
+    if (slices == 1) o("__syncthreads();");
     int stack[16];
     for (int e = T.e0; e < T.e1; ++e) {
         const int ev = (*J.events)[e], n = ev >> 2, kind = ev & 3;
@@ -193,8 +204,10 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int t)
         }
         emit_bins(o, T, n);
     }
-    if (nbins == 2) o("fwd_bins_flush2<NT>(L, bins, b, %d, half, %d);", T.t0, T.nt);
-    else o("fwd_bins_flush<NT>(L, bins, b, %d, half, %d);", T.t0, T.nt);
+    if (slices > 1) o("}");  // pass loop; the sums of all slices go to the first half of the partials
+    const char* half = slices > 1 ? "0" : "half";
+    if (nbins == 2) o("fwd_bins_flush2<NT>(L, bins, b, %d, %s, %d);", T.t0, half, T.nt);
+    else o("fwd_bins_flush<NT>(L, bins, b, %d, %s, %d);", T.t0, half, T.nt);
     o("break;");
     o("}");
 }
@@ -213,13 +226,13 @@ void emit_up(Src& o, const Tmpl& T, int n, int p) {
     o("op%d += v%d_%d;", p, 1 - pp, n);
 }
 
-void gen_backward_case(Src& o, const JitLevel& J, int passes, int t) {
+void gen_backward_case(Src& o, const JitLevel& J, const JitConfig& cfg, int passes, int t) {
     Tmpl T(J, t);
     o("case %d: {", t);
     if (passes > 1) {
         // card sums and totals once; then each slice of the combos in turn (the showdown values a thread stages
         // in LDS are its own, so the passes need no barrier)
-        if (T.nt > 0) o("bwd_sums_n<NTB, %d>(L, ssum, b, %d);", T.nt, T.t0);
+        if (T.nt > 0) o("bwd_sums_n<NTB, %d, %d>(L, ssum, b, %d);", T.nt, fwd_sliced(J.level, cfg) ? 1 : 2, T.t0);
         o("bwd_totals<NTB>(ssum, %d);", T.nt);
         o("#pragma unroll 1");  // one walk's registers at a time
         o("for (int pass = 0; pass < %d; ++pass) {", passes);
@@ -341,22 +354,29 @@ void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
     const int L = J.level;
     // ---- forward
     o("extern \"C\" __global__ __launch_bounds__(%d)%s void mtt_jit_fwd%d(mtt::bt::BatchedDev D, "
-      "const float* __restrict__ ranges, float ws) {", JIT_NT, waves_attr(cfg.fwd_waves).c_str(), L);
+      "const float* __restrict__ ranges, float ws) {", fwd_threads(J.level, cfg), waves_attr(cfg.fwd_waves).c_str(), L);
     o("using namespace mtt::bt;");
-    o("constexpr int LEVEL = %d, NT = %d, NP = %d;", L, JIT_NT, J.np);
+    o("constexpr int LEVEL = %d, NT = %d, NP = %d;", L, fwd_threads(J.level, cfg), J.np);
     o("const LevelDev& L = D.L[LEVEL];");
     o("extern __shared__ unsigned long long bins[];");
-    o("const int inst = blockIdx.x >> 1, half = blockIdx.x & 1;");
-    o("const int b = inst / %d, tm = inst %% %d;", J.ntmpl, J.ntmpl);
-    o("const int k = half * NT + (int)threadIdx.x;");
-    o("const bool inb = k < NP;");
-    o("const bool live = k < ldu(L.nlive + b);");
-    // branch-free prologue: clamped combo index, dead lanes zeroed after the loads (each case requests the
-    // cards and the root index, then its regrets, then the root reaches)
-    o("const int kc = min(k, NP - 1);");
-    o("(void)inb;");
+    const int fsl = fwd_sliced(J.level, cfg) ? (J.np + cfg.fwd_nt - 1) / cfg.fwd_nt : 1;
+    if (fsl > 1) {
+        o("const int inst = blockIdx.x;");
+        o("const int b = inst / %d, tm = inst %% %d;", J.ntmpl, J.ntmpl);
+        o("const int nl = ldu(L.nlive + b);");
+    } else {
+        o("const int inst = blockIdx.x >> 1, half = blockIdx.x & 1;");
+        o("const int b = inst / %d, tm = inst %% %d;", J.ntmpl, J.ntmpl);
+        o("const int k = half * NT + (int)threadIdx.x;");
+        o("const bool inb = k < NP;");
+        o("const bool live = k < ldu(L.nlive + b);");
+        // branch-free prologue: clamped combo index, dead lanes zeroed after the loads (each case requests the
+        // cards and the root index, then its regrets, then the root reaches)
+        o("const int kc = min(k, NP - 1);");
+        o("(void)inb;");
+    }
     o("switch (tm) {");
-    for (int t = 0; t < J.ntmpl; ++t) gen_forward_case(o, J, cfg.sd_fp32 != 0, cfg.fwd_bins, t);
+    for (int t = 0; t < J.ntmpl; ++t) gen_forward_case(o, J, cfg.sd_fp32 != 0, cfg.fwd_bins, fsl, t);
     o("default: break;");
     o("}");
     o("}");
@@ -377,7 +397,7 @@ void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
         o("const float hp0 = 0.5f * D.pot0;");
         o("(void)hp0;");
         o("switch (tm) {");
-        for (int t = 0; t < J.ntmpl; ++t) gen_backward_case(o, J, passes, t);
+        for (int t = 0; t < J.ntmpl; ++t) gen_backward_case(o, J, cfg, passes, t);
         o("default: break;");
         o("}");
         o("}");
@@ -402,7 +422,7 @@ void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
     o("(void)hp0;");
     o("float out0 = 0.0f, out1 = 0.0f;");
     o("switch (tm) {");
-    for (int t = 0; t < J.ntmpl; ++t) gen_backward_case(o, J, passes, t);
+    for (int t = 0; t < J.ntmpl; ++t) gen_backward_case(o, J, cfg, passes, t);
     o("default: break;");
     o("}");
     o("bwd_root_out<LEVEL>(D, L, b, tm, k, NP, live, out0, out1);");
@@ -439,6 +459,8 @@ JitConfig bt_jit_config() {
     c.fwd_waves = env("MTT_JIT_FWD_WAVES", 0, 8, c.fwd_waves);
     c.bwd_passes = env("MTT_JIT_BWD_PASSES", 1, 2, c.bwd_passes);
     c.fwd_bins = env("MTT_JIT_FWD_BINS", 1, 2, c.fwd_bins);
+    c.fwd_nt = env("MTT_JIT_FWD_NT", 192, 576, c.fwd_nt);
+    if (c.fwd_nt % 64) c.fwd_nt = JitConfig().fwd_nt;
     return c;
 }
 
@@ -505,6 +527,8 @@ int bt_jit_build(const JitLevel* levels, int n, const JitConfig& cfg, JitSweeps*
             "generated sweeps: backward workgroup %d not a multiple of 64 in 192..576", cfg.bwd_nt);
     MTT_ARG(cfg.bwd_passes == 1 || cfg.bwd_passes == 2, "generated sweeps: backward passes %d not 1 or 2", cfg.bwd_passes);
     MTT_ARG(cfg.fwd_bins == 1 || cfg.fwd_bins == 2, "generated sweeps: forward bin copies %d not 1 or 2", cfg.fwd_bins);
+    MTT_ARG(cfg.fwd_nt >= 192 && cfg.fwd_nt <= 576 && cfg.fwd_nt % 64 == 0,
+            "generated sweeps: forward workgroup %d not a multiple of 64 in 192..576", cfg.fwd_nt);
     const std::string src = bt_jit_source(levels, n, cfg);
     int dev = 0;
     MTT_HIP(hipGetDevice(&dev));

@@ -1350,6 +1350,9 @@ int upload_level(DeviceArena& A, hipStream_t st, LevelHost& H, LevelDev& L, int 
     if (r) return r;
     MTT_HIP(hipMemsetAsync(L.R, 0, ns * sizeof(float), st));
     MTT_HIP(hipMemsetAsync(L.S, 0, ns * sizeof(float), st));
+    // a forward walking the combos in slices writes the first half of the
+    // partials only; the second stays zero for the readers that add both
+    MTT_HIP(hipMemsetAsync(L.part, 0, (size_t)nboards * std::max(L.nterm, 1) * 208 * 8, st));
     return MTT_OK;
 }
 
@@ -1866,19 +1869,19 @@ int batched_run(SolverImpl& s, int iters) {
         if (B.two_level) {
             const LevelDev& L0 = B.D.L[0];
             const size_t sh = B.jit.active ? fwd_lds(L0.maxterm, B.jit.cfg) : (size_t)L0.maxterm * 104 * 8;
-            const dim3 g(49 * L0.ntmpl * 2);
+            const dim3 g(49 * L0.ntmpl * (B.jit.active ? fwd_groups(0, B.jit.cfg) : 2));
             void* args[] = {&B.D, &ranges, &ws};
             MTT_TIMED(s, "turn_forward",
-                      if (B.jit.active) rc = jit_launch(B.jit.fwd[0], g, JIT_NT, sh, s.stream, args);
+                      if (B.jit.active) rc = jit_launch(B.jit.fwd[0], g, fwd_threads(0, B.jit.cfg), sh, s.stream, args);
                       else if (B.dm[0] == 4) hipLaunchKernelGGL((k_bt_forward<0, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
                       else hipLaunchKernelGGL((k_bt_forward<0, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
         {
             const size_t sh = B.jit.active ? fwd_lds(L1.maxterm, B.jit.cfg) : (size_t)L1.maxterm * 104 * 8;
-            const dim3 g(B.nb1 * L1.ntmpl * 2);
+            const dim3 g(B.nb1 * L1.ntmpl * (B.jit.active ? fwd_groups(1, B.jit.cfg) : 2));
             void* args[] = {&B.D, &ranges, &ws};
             MTT_TIMED(s, "river_forward",
-                      if (B.jit.active) rc = jit_launch(B.jit.fwd[1], g, JIT_NT, sh, s.stream, args);
+                      if (B.jit.active) rc = jit_launch(B.jit.fwd[1], g, fwd_threads(1, B.jit.cfg), sh, s.stream, args);
                       else if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_forward<1, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
                       else hipLaunchKernelGGL((k_bt_forward<1, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
