@@ -5,17 +5,20 @@ BF16 hi+lo split showdown path vs the plain FP32 path, on one MI355X.
 Workload (seed 0): a seeded flop; the root deals the turn (49 cards), turn
 betting, river chance (48), river betting, showdown.  Pot 6bb, stacks 40bb,
 per street 33%/75% pot, one 2.5x raise and all-in; Dirichlet(0.5) ranges
-over the 1326 combos.  CFR+ (simultaneous updates, linear averaging) needs
-1250 iterations to bring the FP64 best-response exploitability of this game
-under 0.5% of the pot, so the default run is 10 warmup + 1240 timed
-iterations, after which the exploitability is measured on the GPU.
+over the 1326 combos (926,542 nodes).
 
-Prints ONE JSON line.  `value` is the batched path's iterations/s with all
-state resident in HBM.  Extra fields: the plain FP32 path's iterations/s on
-the same game (timed over a shorter sample: same recurrence, same cost per
-iteration), the exploitability reached, the per-kernel roofline of the
-dominant kernel (HIP events on the solver's stream), and the CPU oracle
-timed on a bounded sample on this host.
+Each path: `--warmup` untimed iterations, `--steps` timed iterations
+(`value` = the batched path's timed iterations/s, all state resident in
+HBM), then untimed iterations up to `--solve-iters` (default 1250, what
+CFR+ needs on this game to bring the FP64 best-response exploitability
+under 0.5% of the pot) with the exploitability measured on the GPU every
+50 iterations, so that every run reports the iteration count at which the
+qualifier is met, the exploitability at the configuration's 500 iterations
+and at the end, for both paths, and the strategy L-inf between them.
+
+Prints ONE JSON line, with the per-kernel roofline of the dominant kernel
+(HIP events on the solver's stream) and the CPU oracle timed on a bounded
+sample on this host at one thread and at all threads.
 """
 import argparse
 import json
@@ -96,37 +99,118 @@ def hbm_traffic(kernel):
         return None
 
 
-def cpu_baseline(S, O, game, r0, r1, seconds=20.0):
-    """Oracle (FP32 host restatement) on the same game: whole iterations over
-    a bounded number of turn cards is not the same recurrence, so the sample is
-    whole iterations of the full game, as many as fit in `seconds`."""
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return model, os.cpu_count(), avail
+
+
+def cpu_baseline(S, O, game, r0, r1, seconds=15.0):
+    """Oracle (FP32 host restatement of the same recurrence) on the same
+    game: whole CFR+ iterations of the full game, as many as fit in
+    `seconds` after the first, at all OpenMP threads and at one thread."""
     nodes, ni, _ = S.tree(game)
     board = list(game.board)[:3]
     gi, _ = S.alive_index(board)
+    all_threads = O.max_threads()
+
     h = O.HU(nodes, ni, board, r0[gi], r1[gi], 1)
     t0 = time.time()
-    h.iterate(1)
+    h.iterate(1)                        # first touch of the regret / strategy-sum pages
     first = time.time() - t0
-    n = 1
-    t1 = time.time()
-    while time.time() - t1 + first < seconds and n < 50:
-        h.iterate(1)
-        n += 1
-    dt = time.time() - t1
-    per = dt / max(n - 1, 1) if n > 1 else first
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / per, "unit": "iter/s", "cores": threads, "kind": "port",
-            "sample": f"{n} full CFR+ iterations of the same turn+river game (FP32 oracle, OpenMP over turn cards); "
-                      f"first iteration incl. board setup {first:.1f}s"}
+
+    def sample(threads, budget, cap):
+        O.set_threads(threads)
+        n, t1 = 0, time.time()
+        while n < cap and (n == 0 or time.time() - t1 < budget):
+            h.iterate(1)
+            n += 1
+        return n / (time.time() - t1), n
+
+    v_all, n_all = sample(all_threads, seconds, 50)
+    v_one, n_one = sample(1, seconds, 1)
+    O.set_threads(all_threads)
+    h.close()
+    model, ncpu, avail = cpu_info()
+    return {"value": v_all, "unit": "iter/s", "cores": all_threads, "kind": "port",
+            "single_thread_value": v_one, "cpu_model": model, "nproc": ncpu, "cpus_available": avail,
+            "sample": f"full CFR+ iterations of the same turn+river game on the FP32 oracle "
+                      f"(OpenMP over turn cards), one handle: an untimed first iteration ({first:.1f}s, first "
+                      f"touch of the state), then {n_all} timed at {all_threads} threads and {n_one} at 1 thread"}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def solve_checkpoints(solver, total, every=50, name=""):
+    """Untimed iterations up to `total` with the FP64 best-response
+    exploitability (% of the starting pot) every `every` iterations and at
+    500: returns [(iterations, exploitability)]."""
+    marks = sorted(set([c for c in range(every, total + 1, every)] + [500, total]))
+    out = []
+    for c in marks:
+        if c < solver.iterations or c > total:
+            continue
+        if c > solver.iterations:
+            solver.run(c - solver.iterations)
+        out.append((solver.iterations, float(solver.exploitability()[5])))
+        log(f"{name}: {out[-1][0]} iterations, exploitability {out[-1][1]:.4f}% pot")
+    return out
+
+
+def expl_summary(curve):
+    first = next((n for n, e in curve if e <= 0.5), None)
+    at = {n: e for n, e in curve}
+    return {"iterations_to_0p5pct": first, "exploitability_at_500": at.get(500),
+            "exploitability_pct_pot": curve[-1][1] if curve else None,
+            "iterations": curve[-1][0] if curve else None,
+            "meets_0p5pct": bool(curve and curve[-1][1] <= 0.5),
+            "curve": [[n, round(e, 5)] for n, e in curve]}
+
+
+def strategy_linf(a, b, S_ref, ni, n_alive, chunk=8192):
+    """L-inf of the average-strategy difference over (infoset, combo) entries
+    with strategy mass (sum over actions of the reference path's strategy
+    sums >= 1e-5 of the infoset's largest), and over all entries."""
+    a = a.reshape(ni, 4, -1)
+    b = b.reshape(ni, 4, -1)
+    w = S_ref.reshape(ni, 4, -1)
+    live, raw = 0.0, 0.0
+    for lo in range(0, ni, chunk):
+        hi = min(ni, lo + chunk)
+        d = np.abs(a[lo:hi, :, :n_alive] - b[lo:hi, :, :n_alive]).max(1)
+        m = w[lo:hi, :, :n_alive].sum(1)
+        keep = m >= 1e-5 * m.max(1, keepdims=True)
+        keep &= m > 0
+        raw = max(raw, float(d.max()))
+        if keep.any():
+            live = max(live, float(d[keep].max()))
+    return live, raw
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1240)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--plain-steps", type=int, default=150, help="timed plain FP32 iterations")
+    ap.add_argument("--plain-steps", type=int, default=None,
+                    help="timed plain FP32 iterations (default: the same as --steps)")
+    ap.add_argument("--solve-iters", type=int, default=1250,
+                    help="untimed iterations after the timed window, up to this total, with the FP64 "
+                         "exploitability every 50 iterations (0: stop after the timed window)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -152,6 +236,8 @@ def main():
     game, r0, r1 = S.config3(args.seed)
     total = args.warmup + args.steps
 
+    solve_total = max(args.solve_iters, total) if args.solve_iters > 0 else total
+
     # ------------------------------------------------ batched BF16 split ---
     t = time.time()
     sb = S.Solver(game, r0, r1, S.MODE_BF16_SPLIT)
@@ -164,17 +250,22 @@ def main():
     ktimes = sb.kernel_times()
     sb.timing(False)
     ips_b = args.steps / dt_b
-    e_b = sb.exploitability()
+    t = time.time()
+    log(f"batched: setup {setup_b:.1f}s, {args.steps} timed iterations {ips_b:.2f} it/s")
+    curve_b = solve_checkpoints(sb, solve_total, name="batched")
+    solve_b_s = time.time() - t
     info = sb.info.tolist()
+    st_b = sb.strategy() if not args.no_plain else None
     sb.close()
     del sb
 
     # ----------------------------------------------------- plain FP32 ---
     plain = None
+    linf = None
     if not args.no_plain:
         sp = S.Solver(game, r0, r1, S.MODE_PLAIN_FP32)
-        pw = min(args.warmup, 2)
-        psteps = args.plain_steps
+        pw = args.warmup
+        psteps = args.plain_steps or args.steps
         sp.run(pw)
         sp.timing(True)
         t = time.time()
@@ -182,12 +273,24 @@ def main():
         dt_p = time.time() - t
         ktp = sp.kernel_times()
         sp.timing(False)
-        e_p = sp.exploitability() if sp.iterations >= total else None
-        plain = {"iter_per_s": psteps / dt_p, "iterations": sp.iterations,
-                 "exploitability_pct_pot": None if e_p is None else float(e_p[5]),
-                 "kernel_ms_per_iter": {k: v[0] / max(psteps, 1) for k, v in ktp.items()}}
+        t = time.time()
+        log(f"plain: {psteps} timed iterations {psteps / dt_p:.2f} it/s")
+        curve_p = solve_checkpoints(sp, solve_total, name="plain")
+        solve_p_s = time.time() - t
+        plain = {"iter_per_s": psteps / dt_p, "timed_steps": psteps, "warmup": pw,
+                 "kernel_ms_per_iter": {k: v[0] / max(psteps, 1) for k, v in ktp.items()},
+                 "untimed_solve_s": solve_p_s, **expl_summary(curve_p)}
+        if st_b is not None and sp.iterations == solve_total:
+            st_p = sp.strategy()
+            _, S_p = sp.state()
+            live, raw = strategy_linf(st_b, st_p, S_p, int(info[1]), int(info[2]))
+            linf = {"iterations": solve_total, "live_entries": live, "all_entries": raw,
+                    "note": "average strategies of the batched BF16-split and plain FP32 paths; live = "
+                            "(infoset, combo) entries whose plain strategy sums are >= 1e-5 of the infoset's largest"}
+            del st_p, S_p
         sp.close()
         del sp
+    del st_b
 
     # ------------------------------------------------------- roofline ---
     work = algorithmic_work(S, game)
@@ -219,6 +322,7 @@ def main():
     if not args.no_cpu:
         try:
             O = load("oracle_ctypes", os.path.join(ROOT, "oracle", "oracle_ctypes.py"))
+            log("cpu baseline: FP32 oracle on the host")
             cpu = cpu_baseline(S, O, game, r0, r1)
         except Exception as ex:  # the baseline is reported, never required
             cpu = {"error": str(ex)}
@@ -238,13 +342,15 @@ def main():
         "data": "synthetic (seeded board and Dirichlet(0.5) ranges)",
         "config": {"workload": "hu_turn_river_cfr_plus", "seed": args.seed, "board": [int(c) for c in game.board[:3]],
                    "pot_bb": 6, "stack_bb": 40, "bets": "33%/75% + 2.5x raise + all-in per street",
-                   "iterations": total, "nodes": info[0], "infosets": info[1]},
-        "exploitability_pct_pot": float(e_b[5]),
-        "meets_0p5pct": bool(e_b[5] <= 0.5),
+                   "timed_iterations": args.steps, "solve_iterations": solve_total,
+                   "nodes": info[0], "infosets": info[1]},
+        **expl_summary(curve_b),
+        "untimed_solve_s": solve_b_s,
         "setup_s": setup_b,
         "kernel_ms_per_iter": per_iter,
         "plain_fp32": plain,
         "speedup_vs_plain_fp32": (ips_b / plain["iter_per_s"]) if plain else None,
+        "strategy_linf_batched_vs_plain": linf,
         "roofline": roof,
         "roofline_showdown": sd_roof,
         "work_per_iter": work,

@@ -11,6 +11,7 @@
  * 44 rivers, flop boards with the exhaustive counts of or_flop_counts.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -240,6 +241,12 @@ int or_hu_n_combos(const or_hu* h) { return h->N; }
 /* scheme 0 = CFR+ (R floored, linear averaging), 1 = DCFR(alpha,beta,gamma)
  * with beta = -INFINITY meaning negatives are floored (the reference form,
  * preflop_grid_generator.cu:402). */
+/* OpenMP threads used by the oracle's parallel loops (timing at a stated
+ * thread count). */
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+int or_max_threads(void) { return omp_get_max_threads(); }
+
 void or_hu_iterate(or_hu* h, int iters, int scheme, double alpha, double beta, double gamma) {
     if (h->prec == 0) cfr_iterations_d(h, iters, h->iters_done, scheme, alpha, beta, gamma);
     else cfr_iterations_f(h, iters, h->iters_done, scheme, alpha, beta, gamma);
@@ -354,6 +361,60 @@ static void br_rec(br_ctx* C, int node, int depth, uint8_t* dealt, int nd, const
     }
 }
 
+/* br_rec at a chance root with its subtrees on OpenMP threads: each child's
+ * values land in their own slot and are summed in card order afterwards, so
+ * the result does not depend on the thread count.  Same arithmetic as the
+ * chance branch of br_rec (children summed over the cards a combo does not
+ * hold, scaled by 1 / (children - 4)). */
+static void br_root_chance(const or_hu* h, const br_ctx* C0, double* v) {
+    const mtt_flat_node* x = &h->nodes[0];
+    int N = h->N, nch = x->num_actions;
+    double* kid = (double*)malloc(sizeof(double) * 2 * N * (size_t)nch);
+#pragma omp parallel
+    {
+        br_ctx C = *C0;
+        C.reach = (double**)malloc(sizeof(double*) * h->max_depth);
+        C.vals = (double**)malloc(sizeof(double*) * h->max_depth);
+        C.acc = (double**)malloc(sizeof(double*) * h->max_depth);
+        for (int d = 0; d < h->max_depth; ++d) {
+            C.reach[d] = (double*)malloc(sizeof(double) * 2 * N);
+            C.vals[d] = (double*)malloc(sizeof(double) * 2 * N);
+            C.acc[d] = (double*)malloc(sizeof(double) * 2 * N);
+        }
+        uint8_t dealt[4];
+#pragma omp for schedule(dynamic, 1)
+        for (int k = 0; k < nch; ++k) {
+            int di = (x->turn_idx >= 0 && k >= x->turn_idx) ? k + 1 : k;
+            uint64_t cb = 1ull << h->deck[di];
+            double* cr = C.reach[0];
+            for (int i = 0; i < N; ++i) {
+                int dead = (h->cmask[i] & cb) != 0;
+                cr[i] = dead ? 0 : h->range[0][i];
+                cr[N + i] = dead ? 0 : h->range[1][i];
+            }
+            dealt[0] = (uint8_t)di;
+            double* kv = kid + (size_t)k * 2 * N;
+            br_rec(&C, x->children_offset + k, 1, dealt, 1, cr, cr + N, kv, kv + N);
+        }
+        for (int d = 0; d < h->max_depth; ++d) { free(C.reach[d]); free(C.vals[d]); free(C.acc[d]); }
+        free(C.reach); free(C.vals); free(C.acc);
+    }
+    for (int i = 0; i < 2 * N; ++i) v[i] = 0;
+    for (int k = 0; k < nch; ++k) {
+        int di = (x->turn_idx >= 0 && k >= x->turn_idx) ? k + 1 : k;
+        uint64_t cb = 1ull << h->deck[di];
+        const double* kv = kid + (size_t)k * 2 * N;
+        for (int i = 0; i < N; ++i) {
+            if (h->cmask[i] & cb) continue;
+            v[i] += kv[i];
+            v[N + i] += kv[N + i];
+        }
+    }
+    double inv = 1.0 / (double)(nch - 4);
+    for (int i = 0; i < 2 * N; ++i) v[i] *= inv;
+    free(kid);
+}
+
 /* Values at the root for a profile `strat` (reference layout, normalised):
  * out[0..1] = best-response values BR0(strat1), BR1(strat0);
  * out[2..3] = the profile's own values; out[4] = pair mass M (sum over
@@ -375,9 +436,11 @@ void or_hu_exploitability(const or_hu* h, const float* strat, double pot, double
     double* v = (double*)malloc(sizeof(double) * 2 * N);
     uint8_t dealt[4];
     double res[3][2];
+    const mtt_flat_node* root = &h->nodes[0];
     for (int mode = 0; mode < 3; ++mode) {
         C.br = mode == 2 ? -1 : mode;
-        br_rec(&C, 0, 0, dealt, 0, h->range[0], h->range[1], v, v + N);
+        if (root->type == MTT_NODE_CHANCE) br_root_chance(h, &C, v);
+        else br_rec(&C, 0, 0, dealt, 0, h->range[0], h->range[1], v, v + N);
         for (int p = 0; p < 2; ++p) {
             double s = 0;
             for (int i = 0; i < N; ++i) s += h->range[p][i] * v[p * N + i];

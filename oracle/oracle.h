@@ -63,6 +63,8 @@ or_hu* or_hu_create(const mtt_flat_node* nodes, int n_nodes, int n_infosets, con
                     const float* range0, const float* range1, int precision);
 void or_hu_destroy(or_hu* h);
 int or_hu_n_combos(const or_hu* h);
+void or_set_threads(int n);
+int or_max_threads(void);
 void or_hu_iterate(or_hu* h, int iters, int scheme, double alpha, double beta, double gamma);
 void or_hu_avg_strategy(const or_hu* h, float* out);
 void or_hu_state(const or_hu* h, float* R_out, float* S_out);

@@ -30,6 +30,8 @@ def lib():
         L.or_combo_index.argtypes = [ctypes.c_int, ctypes.c_int]
         L.or_flop_counts.argtypes = [_P, _P, _P]
         L.or_turn_matrix.argtypes = [_P, ctypes.c_int, _P]
+        L.or_set_threads.argtypes = [ctypes.c_int]
+        L.or_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -43,6 +45,15 @@ def eval7_batch(cards):
     out = np.zeros(cards.shape[0], np.uint32)
     lib().or_eval7_batch(_p(cards), cards.shape[0], _p(out))
     return out
+
+
+def set_threads(n):
+    """OpenMP threads for the oracle's parallel loops."""
+    lib().or_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().or_max_threads())
 
 
 def category_counts():

@@ -84,6 +84,9 @@ def _multistreet_check(solver_api, oracle, mode, tol_early):
     e_gpu = s.exploitability()
     e_cpu = h32.exploitability(st, 6.0)  # the oracle's FP64 BR of the GPU strategy
     assert np.allclose(e_gpu[:5], e_cpu[:5], rtol=1e-7)
+    # constant sum: the two players' chips add to the starting pot times the
+    # pair mass (FP32-normalised strategies and FP32 pots in the tree: 1e-6)
+    assert abs(e_gpu[2] + e_gpu[3] - 6.0 * e_gpu[4]) <= 1e-6 * 6.0 * e_gpu[4]
     e_ref = h32.exploitability(h32.strategy(), 6.0)
     assert abs(e_gpu[5] - e_ref[5]) < 0.02, (e_gpu[5], e_ref[5])
     s.close()
@@ -281,6 +284,48 @@ def test_batched_config3_matches_oracle(solver_api, oracle, torch_dev):
     assert len(bad) <= 1e-7 * Sc.size
     assert amb[bad[:, 2]].all()
     s.close()
+
+
+def test_plain_config3_matches_oracle(solver_api, oracle, torch_dev):
+    """The plain FP32 path on the benchmark's turn+river game against the
+    FP32 oracle after one iteration (every regret and strategy sum), and
+    both GPU paths' average strategies after 10 iterations against the FP64
+    oracle (strategy L-inf over entries with strategy mass), with their FP64
+    exploitabilities and the constant-sum identity of their profiles."""
+    g, r0, r1 = solver_api.config3(0)
+    print("config3 plain fp32 vs oracle, 1 iteration", flush=True)
+    s = solver_api.Solver(g, r0, r1, solver_api.MODE_PLAIN_FP32)
+    h32, nodes, ni = _setup(solver_api, oracle, g, r0, r1, 1)
+    s.run(1)
+    h32.iterate(1)
+    Rg, Sg = s.state()
+    Rc, Sc = h32.state()
+    assert np.abs(Rg - Rc).max() <= STATE_REL_FP32 * np.abs(Rc).max()
+    assert np.abs(Sg - Sc).max() <= STATE_REL_FP32 * np.abs(Sc).max()
+    del Rg, Sg, Rc, Sc, h32
+    b = solver_api.Solver(g, r0, r1, solver_api.MODE_BF16_SPLIT)
+    b.run(10)
+    s.run(9)
+    print("config3: FP64 oracle, 10 iterations", flush=True)
+    h64, _, _ = _setup(solver_api, oracle, g, r0, r1, 0)
+    h64.iterate(10)
+    ref = h64.strategy()
+    S64 = h64.state()[1]
+    e_ref = h64.exploitability(ref, 6.0)
+    n_alive = int(s.info[2])
+    for name, x in (("plain", s), ("batched", b)):
+        st = x.strategy()
+        d = _live_linf(st, ref, nodes, n_alive, S64)
+        e = x.exploitability()
+        print(f"config3 {name}: strategy L-inf vs FP64 {d:.3g}, exploitability {e[5]:.4f}% "
+              f"(FP64 oracle {e_ref[5]:.4f}%)", flush=True)
+        # 10 iterations leave room for a few regret-matching threshold ties
+        # (see test_batched_config3_matches_oracle) to move live entries
+        assert d < 2e-2
+        assert abs(e[5] - e_ref[5]) < 0.05
+        assert abs(e[2] + e[3] - 6.0 * e[4]) <= 1e-6 * 6.0 * e[4]
+    s.close()
+    b.close()
 
 
 @pytest.mark.parametrize("which", ["config1", "config3_small", "config3"])
