@@ -181,23 +181,29 @@ def expl_summary(curve):
 
 
 def strategy_linf(a, b, S_ref, ni, n_alive, chunk=8192):
-    """L-inf of the average-strategy difference over (infoset, combo) entries
-    with strategy mass (sum over actions of the reference path's strategy
-    sums >= 1e-5 of the infoset's largest), and over all entries."""
+    """Average-strategy differences over (infoset, combo) entries with
+    strategy mass (sum over actions of the reference path's strategy sums
+    >= 1e-5, resp. >= 0.1, of the infoset's largest): L-inf over each set and
+    the fraction of the first set that differs by more than 1e-2."""
     a = a.reshape(ni, 4, -1)
     b = b.reshape(ni, 4, -1)
     w = S_ref.reshape(ni, 4, -1)
-    live, raw = 0.0, 0.0
+    live, heavy, n_live, n_off = 0.0, 0.0, 0, 0
     for lo in range(0, ni, chunk):
         hi = min(ni, lo + chunk)
         d = np.abs(a[lo:hi, :, :n_alive] - b[lo:hi, :, :n_alive]).max(1)
         m = w[lo:hi, :, :n_alive].sum(1)
-        keep = m >= 1e-5 * m.max(1, keepdims=True)
-        keep &= m > 0
-        raw = max(raw, float(d.max()))
+        mm = m.max(1, keepdims=True)
+        keep = (m >= 1e-5 * mm) & (m > 0)
+        big = (m >= 0.1 * mm) & (m > 0)
         if keep.any():
             live = max(live, float(d[keep].max()))
-    return live, raw
+            n_live += int(keep.sum())
+            n_off += int((d[keep] > 1e-2).sum())
+        if big.any():
+            heavy = max(heavy, float(d[big].max()))
+    return {"linf_mass_1e-5": live, "linf_mass_0.1": heavy,
+            "frac_mass_1e-5_off_by_1e-2": n_off / max(n_live, 1)}
 
 
 def main():
@@ -283,10 +289,9 @@ def main():
         if st_b is not None and sp.iterations == solve_total:
             st_p = sp.strategy()
             _, S_p = sp.state()
-            live, raw = strategy_linf(st_b, st_p, S_p, int(info[1]), int(info[2]))
-            linf = {"iterations": solve_total, "live_entries": live, "all_entries": raw,
-                    "note": "average strategies of the batched BF16-split and plain FP32 paths; live = "
-                            "(infoset, combo) entries whose plain strategy sums are >= 1e-5 of the infoset's largest"}
+            linf = {"iterations": solve_total, **strategy_linf(st_b, st_p, S_p, int(info[1]), int(info[2])),
+                    "note": "average strategies of the batched BF16-split and plain FP32 paths over (infoset, "
+                            "combo) entries whose plain strategy sums are >= 1e-5 (>= 0.1) of the infoset's largest"}
             del st_p, S_p
         sp.close()
         del sp
