@@ -45,177 +45,274 @@ __global__ void k_eval7_batch(const uint8_t* __restrict__ cards, int64_t n, uint
 }
 
 // ------------------------------------------------ exhaustive flop counts --
-// Per flop f: deck[49] (cards not on the flop, ascending), the 1176 alive
+// Per flop: deck[49] (cards not on the flop, ascending), the 1176 alive
 // combos (lexicographic) and their output indices.  Runout b = (x<y) pair of
-// deck indices in lexicographic order: 1176 runouts, padded to kRunPad.
+// deck indices in lexicographic order: 1176 runouts, taken in 588 pairs.
+//
+// Counting (all integer VALU):
+//   1. k_fc_codes: per (runout pair, hand) the two scores as order-preserving
+//      16-bit codes, packed in one dword: v = code + 1, or 0xFFFF when the
+//      hand holds a runout card (or is padding).
+//   2. k_fc_pairs: per unordered pair {i, j} (upper-triangle tiles only) and
+//      two runouts at a time in 16-bit lanes:
+//          x = min(sat_sub(v_i + 1, v_j), 2)  =  2 (i wins), 1 (tie), 0 (i loses
+//                                                or either hand is dead)
+//          s1 += x,  s2 += x*x                    (v_pk_sub_u16 clamp,
+//                                                  v_pk_min_u16, v_pk_add_u16,
+//                                                  v_pk_mad_u16)
+//      so s1 = 2W + T and s2 = 4W + T give W[i][j] and T[i][j] from one pass;
+//      W[j][i] = 990 - W[i][j] - T[i][j] (every unblocked pair has exactly
+//      C(45,2) = 990 live runouts).  Two VALU ops per pair-runout.
+//   3. k_fc_border: zeroes the rows and columns of the output that are not
+//      alive combos of the flop.
 constexpr int kRunouts = 1176;
-constexpr int kRunPad = 1184;     // multiple of the LDS chunk
+constexpr int kRunPairs = 588;
 constexpr int kHandPad = 1280;    // 10 tiles of 128
 constexpr int kTile = 128;
-constexpr int kChunk = 16;
+constexpr int kTiles = 10;
+constexpr int kTilePairs = kTiles * (kTiles + 1) / 2;
+constexpr int kChunk = 12;        // runout pairs per LDS stage, 588 = 49 x 12
+constexpr int kFcBatch = 64;      // flops per launch of the allocating entry point
+constexpr int kFcLive = 990;      // runouts avoiding both hands of an unblocked pair
 
 struct FlopTables {
-    uint8_t deck[49];
-    uint8_t pad[3];
+    uint8_t deck[52];
+    uint8_t pad[12];
     uint8_t c1[kHandPad];
     uint8_t c2[kHandPad];
     int16_t out_idx[kHandPad];     // output row/col index of alive combo
-    int16_t alive_of_out[1326];    // inverse: output index -> alive index or -1
+    int16_t alive_of_out[1328];    // inverse: output index -> alive index or -1
+};
+static_assert(sizeof(FlopTables) % 16 == 0, "FlopTables keeps 16-byte alignment in arrays");
+
+struct FlopBatch {                  // flop cards by value (no host buffer to upload)
+    uint8_t c[3 * kFcBatch];
 };
 
-__constant__ uint8_t c_runout_x[kRunouts];
-__constant__ uint8_t c_runout_y[kRunouts];
-
-// scores[f][b][h]: score of alive hand h on runout b, 0 if the hand holds a
-// runout card, and 0 for padded hands and padded runouts.
-__global__ void k_fc_scores(const FlopTables* __restrict__ tabs, const uint8_t* __restrict__ flops,
-                            uint32_t* __restrict__ scores) {
-    int b = blockIdx.x, f = blockIdx.y;
-    const FlopTables& t = tabs[f];
-    uint32_t* row = scores + ((size_t)f * kRunPad + b) * kHandPad;
-    if (b >= kRunouts) {
-        for (int h = threadIdx.x; h < kHandPad; h += blockDim.x) row[h] = 0;
-        return;
-    }
-    int x = t.deck[c_runout_x[b]], y = t.deck[c_runout_y[b]];
-    uint64_t board = card_bits16(flops[3 * f]) | card_bits16(flops[3 * f + 1]) |
-                     card_bits16(flops[3 * f + 2]) | card_bits16(x) | card_bits16(y);
-    uint64_t dead = (1ull << x) | (1ull << y);
-    for (int h = threadIdx.x; h < kHandPad; h += blockDim.x) {
-        uint32_t s = 0;
-        if (h < kAliveFlop) {
-            int a = t.c1[h], c = t.c2[h];
-            if (!(dead & ((1ull << a) | (1ull << c)))) s = score_packed(board | card_bits16(a) | card_bits16(c));
-        }
-        row[h] = s;
-    }
+struct RunoutTable {
+    uint8_t x[kRunouts], y[kRunouts];
+};
+constexpr RunoutTable make_runouts() {
+    RunoutTable t{};
+    int b = 0;
+    for (int x = 0; x < 49; ++x)
+        for (int y = x + 1; y < 49; ++y) { t.x[b] = (uint8_t)x; t.y[b] = (uint8_t)y; b++; }
+    return t;
 }
+__constant__ RunoutTable c_runouts = make_runouts();
 
-// W tile: 128 heroes x 128 villains per workgroup, 8x8 per thread.
-// Hero score 0 (conflict/pad) never wins; villain conflict becomes
-// 0xFFFFFFFF so it is never beaten.
-__global__ __launch_bounds__(256) void k_fc_compare(const FlopTables* __restrict__ tabs,
-                                                    const uint32_t* __restrict__ scores,
-                                                    int32_t* __restrict__ W, int S) {
-    __shared__ uint4 lds_h[kChunk][kTile / 4];
-    __shared__ uint4 lds_v[kChunk][kTile / 4];
-    const int f = blockIdx.z;
-    const int I0 = blockIdx.y * kTile, J0 = blockIdx.x * kTile;
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    const uint32_t* sf = scores + (size_t)f * kRunPad * kHandPad;
-    uint32_t acc[8][8];
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-        for (int b = 0; b < 8; ++b) acc[a][b] = 0;
-
-    for (int b0 = 0; b0 < kRunPad; b0 += kChunk) {
-        // 16 runouts x (128 hero + 128 villain) scores -> 4096 u32, 16 per thread
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            int e = tid + 256 * k;         // 0..1023 uint4 slots
-            int r = e >> 6, q = e & 63;    // runout in chunk, uint4 within 256 scores
-            const uint4* src = reinterpret_cast<const uint4*>(sf + (size_t)(b0 + r) * kHandPad);
-            if (q < 32) {
-                lds_h[r][q] = src[(I0 >> 2) + q];
-            } else {
-                uint4 v = src[(J0 >> 2) + (q - 32)];
-                v.x = v.x ? v.x : 0xFFFFFFFFu;
-                v.y = v.y ? v.y : 0xFFFFFFFFu;
-                v.z = v.z ? v.z : 0xFFFFFFFFu;
-                v.w = v.w ? v.w : 0xFFFFFFFFu;
-                lds_v[r][q - 32] = v;
-            }
-        }
-        __syncthreads();
-#pragma unroll 2
-        for (int r = 0; r < kChunk; ++r) {
-            uint4 h0 = lds_h[r][ty * 2], h1 = lds_h[r][ty * 2 + 1];
-            uint4 v0 = lds_v[r][tx * 2], v1 = lds_v[r][tx * 2 + 1];
-            uint32_t hv[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
-            uint32_t vv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-#pragma unroll
-            for (int a = 0; a < 8; ++a)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) acc[a][c] += (hv[a] > vv[c]) ? 1u : 0u;
-        }
-        __syncthreads();
-    }
-    const FlopTables& t = tabs[f];
-    int32_t* Wf = W + (size_t)f * S * S;
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        int i = I0 + ty * 8 + a;
-        if (i >= kAliveFlop) continue;
-        uint64_t mi = (1ull << t.c1[i]) | (1ull << t.c2[i]);
-        int oi = t.out_idx[i];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            int j = J0 + tx * 8 + c;
-            if (j >= kAliveFlop) continue;
-            uint64_t mj = (1ull << t.c1[j]) | (1ull << t.c2[j]);
-            Wf[(size_t)oi * S + t.out_idx[j]] = (mi & mj) ? 0 : (int32_t)acc[a][c];
-        }
-    }
+struct TilePairTable {
+    uint8_t i[kTilePairs], j[kTilePairs];
+};
+constexpr TilePairTable make_tile_pairs() {
+    TilePairTable t{};
+    int k = 0;
+    for (int i = 0; i < kTiles; ++i)
+        for (int j = i; j < kTiles; ++j) { t.i[k] = (uint8_t)i; t.j[k] = (uint8_t)j; k++; }
+    return t;
 }
+__constant__ TilePairTable c_tile_pairs = make_tile_pairs();
 
-// T = 990 - W - W^T on unblocked pairs; zero every entry outside the alive
-// set (W included) and on blocked pairs.
-__global__ void k_fc_finish(const FlopTables* __restrict__ tabs, int32_t* __restrict__ W,
-                            int32_t* __restrict__ T, int S) {
-    int f = blockIdx.z;
-    int oi = blockIdx.y;
-    const FlopTables& t = tabs[f];
-    int32_t* Wf = W + (size_t)f * S * S;
-    int32_t* Tf = T + (size_t)f * S * S;
-    int ai = oi < 1326 ? t.alive_of_out[oi] : -1;
-    for (int oj = blockIdx.x * blockDim.x + threadIdx.x; oj < S; oj += gridDim.x * blockDim.x) {
-        int aj = oj < 1326 ? t.alive_of_out[oj] : -1;
-        size_t ij = (size_t)oi * S + oj;
-        if (ai < 0 || aj < 0) {
-            Wf[ij] = 0;
-            Tf[ij] = 0;
-            continue;
-        }
-        uint64_t mi = (1ull << t.c1[ai]) | (1ull << t.c2[ai]);
-        uint64_t mj = (1ull << t.c1[aj]) | (1ull << t.c2[aj]);
-        if (mi & mj) { Tf[ij] = 0; continue; }
-        Tf[ij] = 990 - Wf[ij] - Wf[(size_t)oj * S + oi];
+// One workgroup per flop; the lexicographic walk is one lane's (1326 steps).
+__global__ void k_fc_tables(FlopBatch fb, int layout, FlopTables* __restrict__ tabs) {
+    const int f = blockIdx.x;
+    FlopTables& t = tabs[f];
+    const uint64_t fm = (1ull << fb.c[3 * f]) | (1ull << fb.c[3 * f + 1]) | (1ull << fb.c[3 * f + 2]);
+    for (int o = threadIdx.x; o < 1328; o += blockDim.x)
+        t.alive_of_out[o] = (layout == MTT_LAYOUT_ALIVE1216 && o < kAliveFlop) ? (int16_t)o : (int16_t)-1;
+    for (int h = kAliveFlop + threadIdx.x; h < kHandPad; h += blockDim.x) {
+        t.c1[h] = 0; t.c2[h] = 0; t.out_idx[h] = -1;
     }
-}
-
-static void build_flop_tables(const uint8_t* flop, int layout, FlopTables& t) {
-    memset(&t, 0, sizeof(t));
-    uint64_t fm = (1ull << flop[0]) | (1ull << flop[1]) | (1ull << flop[2]);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     int nd = 0;
     for (int c = 0; c < 52; ++c)
         if (!(fm & (1ull << c))) t.deck[nd++] = (uint8_t)c;
-    int n = alive_combos(fm, t.c1, t.c2);
-    for (int k = 0; k < 1326; ++k) t.alive_of_out[k] = -1;
-    for (int i = 0; i < n; ++i) {
-        int o = layout == MTT_LAYOUT_GLOBAL1326 ? combo_index(t.c1[i], t.c2[i]) : i;
-        t.out_idx[i] = (int16_t)o;
-        t.alive_of_out[o] = (int16_t)i;
+    for (int c = nd; c < 52; ++c) t.deck[c] = 0;
+    int n = 0, g = 0;
+    for (int a = 0; a < 52; ++a)
+        for (int b = a + 1; b < 52; ++b, ++g) {
+            if (fm & ((1ull << a) | (1ull << b))) continue;
+            t.c1[n] = (uint8_t)a;
+            t.c2[n] = (uint8_t)b;
+            int o = layout == MTT_LAYOUT_GLOBAL1326 ? g : n;
+            t.out_idx[n] = (int16_t)o;
+            if (layout == MTT_LAYOUT_GLOBAL1326) t.alive_of_out[o] = (int16_t)n;
+            n++;
+        }
+}
+
+// codes[f][rp][h] = v(runout 2rp) | v(runout 2rp+1) << 16.
+__global__ __launch_bounds__(256) void k_fc_codes(const FlopTables* __restrict__ tabs, FlopBatch fb,
+                                                  uint32_t* __restrict__ codes) {
+    const int rp = blockIdx.x, f = blockIdx.y;
+    const FlopTables& t = tabs[f];
+    const uint64_t flop = card_bits16(fb.c[3 * f]) | card_bits16(fb.c[3 * f + 1]) | card_bits16(fb.c[3 * f + 2]);
+    const int x0 = t.deck[c_runouts.x[2 * rp]], y0 = t.deck[c_runouts.y[2 * rp]];
+    const int x1 = t.deck[c_runouts.x[2 * rp + 1]], y1 = t.deck[c_runouts.y[2 * rp + 1]];
+    const uint64_t b0 = flop | card_bits16(x0) | card_bits16(y0);
+    const uint64_t b1 = flop | card_bits16(x1) | card_bits16(y1);
+    const uint64_t d0 = (1ull << x0) | (1ull << y0), d1 = (1ull << x1) | (1ull << y1);
+    uint32_t* row = codes + ((size_t)f * kRunPairs + rp) * kHandPad;
+    for (int h = threadIdx.x; h < kHandPad; h += blockDim.x) {
+        uint32_t v0 = 0xFFFFu, v1 = 0xFFFFu;
+        if (h < kAliveFlop) {
+            const int a = t.c1[h], c = t.c2[h];
+            const uint64_t hm = (1ull << a) | (1ull << c);
+            const uint64_t hb = card_bits16(a) | card_bits16(c);
+            if (!(d0 & hm)) v0 = code16(score_packed(b0 | hb)) + 1u;
+            if (!(d1 & hm)) v1 = code16(score_packed(b1 | hb)) + 1u;
+        }
+        row[h] = v0 | (v1 << 16);
     }
 }
 
-static std::once_flag g_runout_once;
-static int upload_runouts() {
-    static int status = MTT_OK;
-    std::call_once(g_runout_once, [] {
-        uint8_t xs[kRunouts], ys[kRunouts];
-        int b = 0;
-        for (int x = 0; x < 49; ++x)
-            for (int y = x + 1; y < 49; ++y) { xs[b] = (uint8_t)x; ys[b] = (uint8_t)y; b++; }
-        hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_runout_x), xs, sizeof(xs));
-        if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_runout_y), ys, sizeof(ys));
-        if (e != hipSuccess) status = hip_fail(e, "upload runout table", __FILE__, __LINE__);
-    });
-    return status;
+typedef unsigned short fc_u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ fc_u16x2 fc_bits(uint32_t v) { return __builtin_bit_cast(fc_u16x2, v); }
+__device__ __forceinline__ uint32_t fc_word(fc_u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// One workgroup per (tile pair I <= J, flop): 128 hero x 128 villain alive
+// indices as four 64 x 64 wave tiles, 8 x 8 pairs per lane.  Wave tiles that
+// lie wholly below the diagonal or wholly in the padding past 1176 only help
+// stage the LDS chunks.
+__global__ __launch_bounds__(256) void k_fc_pairs(const FlopTables* __restrict__ tabs,
+                                                  const uint32_t* __restrict__ codes,
+                                                  int32_t* __restrict__ W, int32_t* __restrict__ T, int S) {
+    __shared__ uint4 lds_h[kChunk][kTile / 4];
+    __shared__ uint4 lds_v[kChunk][kTile / 4];
+    const int f = blockIdx.y;
+    const int I = c_tile_pairs.i[blockIdx.x], J = c_tile_pairs.j[blockIdx.x];
+    const int I0 = I * kTile, J0 = J * kTile;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int wr = w >> 1, wc = w & 1, lr = lane & 7, lc = lane >> 3;
+    const bool active = !(I == J && wr > wc) && (I0 + 64 * wr < kAliveFlop) && (J0 + 64 * wc < kAliveFlop);
+    const uint32_t* cf = codes + (size_t)f * kRunPairs * kHandPad;
+    const fc_u16x2 two = {2, 2};
+    const fc_u16x2 one = {1, 1};
+    fc_u16x2 s1[8][8], s2[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) { s1[a][b] = fc_u16x2{0, 0}; s2[a][b] = fc_u16x2{0, 0}; }
+
+    for (int c0 = 0; c0 < kRunPairs; c0 += kChunk) {
+        // kChunk runout pairs x (128 hero + 128 villain) dwords: 3 uint4 per thread
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int e = tid + 256 * k;        // 0..767
+            const int r = e >> 6, q = e & 63;
+            const uint4* src = reinterpret_cast<const uint4*>(cf + (size_t)(c0 + r) * kHandPad);
+            if (q < 32) {
+                // hero form v + 1 in each 16-bit half: a dead hand (0xFFFF) becomes 0
+                uint4 v = src[(I0 >> 2) + q];
+                v.x = fc_word(fc_bits(v.x) + one);
+                v.y = fc_word(fc_bits(v.y) + one);
+                v.z = fc_word(fc_bits(v.z) + one);
+                v.w = fc_word(fc_bits(v.w) + one);
+                lds_h[r][q] = v;
+            } else {
+                lds_v[r][q - 32] = src[(J0 >> 2) + (q - 32)];
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll 2
+            for (int r = 0; r < kChunk; ++r) {
+                const uint4 h0 = lds_h[r][wr * 16 + lr * 2], h1 = lds_h[r][wr * 16 + lr * 2 + 1];
+                const uint4 v0 = lds_v[r][wc * 16 + lc * 2], v1 = lds_v[r][wc * 16 + lc * 2 + 1];
+                const uint32_t hv[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+                const uint32_t vv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        const fc_u16x2 x = __builtin_elementwise_min(
+                            __builtin_elementwise_sub_sat(fc_bits(hv[a]), fc_bits(vv[b])), two);
+                        s1[a][b] += x;
+                        s2[a][b] += x * x;
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    const FlopTables& t = tabs[f];
+    int32_t* Wf = W + (size_t)f * S * S;
+    int32_t* Tf = T + (size_t)f * S * S;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int i = I0 + 64 * wr + 8 * lr + a;
+        if (i >= kAliveFlop) continue;
+        const uint64_t mi = (1ull << t.c1[i]) | (1ull << t.c2[i]);
+        const size_t oi = (size_t)t.out_idx[i];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int j = J0 + 64 * wc + 8 * lc + b;
+            if (j >= kAliveFlop || j < i) continue;
+            const size_t oj = (size_t)t.out_idx[j];
+            const uint64_t mj = (1ull << t.c1[j]) | (1ull << t.c2[j]);
+            int32_t wij = 0, tij = 0, wji = 0;
+            if (!(mi & mj)) {
+                const int x1 = (int)s1[a][b].x + (int)s1[a][b].y;     // 2W + T
+                const int x2 = (int)s2[a][b].x + (int)s2[a][b].y;     // 4W + T
+                wij = (x2 - x1) >> 1;
+                tij = 2 * x1 - x2;
+                wji = kFcLive - wij - tij;
+            }
+            Wf[oi * S + oj] = wij;
+            Tf[oi * S + oj] = tij;
+            Wf[oj * S + oi] = wji;
+            Tf[oj * S + oi] = tij;
+        }
+    }
+}
+
+// Rows and columns of output indices that are not alive combos are 0.
+__global__ void k_fc_border(const FlopTables* __restrict__ tabs, int32_t* __restrict__ W,
+                            int32_t* __restrict__ T, int S) {
+    const int f = blockIdx.y, oi = blockIdx.x;
+    const FlopTables& t = tabs[f];
+    int32_t* Wr = W + ((size_t)f * S + oi) * S;
+    int32_t* Tr = T + ((size_t)f * S + oi) * S;
+    const bool row_dead = t.alive_of_out[oi] < 0;
+    for (int oj = threadIdx.x; oj < S; oj += blockDim.x)
+        if (row_dead || t.alive_of_out[oj] < 0) { Wr[oj] = 0; Tr[oj] = 0; }
 }
 
 static bool valid_flop(const uint8_t* f) {
     return f[0] < 52 && f[1] < 52 && f[2] < 52 && f[0] != f[1] && f[0] != f[2] && f[1] != f[2];
+}
+
+static size_t fc_workspace_bytes(int n_flops) {
+    return (size_t)n_flops * (sizeof(FlopTables) + sizeof(uint32_t) * (size_t)kRunPairs * kHandPad);
+}
+
+static int fc_check(const char* fn, const uint8_t* flops, int n_flops, const int32_t* d_W, const int32_t* d_T,
+                    int layout) {
+    MTT_ARG(n_flops >= 0, "%s: negative n_flops", fn);
+    if (n_flops == 0) return MTT_OK;
+    MTT_ARG(flops && d_W && d_T, "%s: null pointer", fn);
+    MTT_ARG(layout == MTT_LAYOUT_GLOBAL1326 || layout == MTT_LAYOUT_ALIVE1216, "%s: unknown layout %d", fn, layout);
+    for (int f = 0; f < n_flops; ++f)
+        MTT_ARG(valid_flop(flops + 3 * f), "%s: flop %d has invalid or repeated cards", fn, f);
+    return MTT_OK;
+}
+
+// Launches the four kernels for `cnt` flops (cnt <= kFcBatch) on `stream`,
+// with the tables and codes in `ws` (fc_workspace_bytes(cnt) bytes).
+static hipError_t fc_launch(const uint8_t* flops, int cnt, int32_t* W, int32_t* T, int layout, void* ws,
+                            hipStream_t stream) {
+    const int S = layout == MTT_LAYOUT_GLOBAL1326 ? 1326 : kPadFlop;
+    FlopBatch fb{};
+    memcpy(fb.c, flops, 3 * (size_t)cnt);
+    FlopTables* tabs = reinterpret_cast<FlopTables*>(ws);
+    uint32_t* codes = reinterpret_cast<uint32_t*>(tabs + cnt);
+    hipLaunchKernelGGL(k_fc_tables, dim3(cnt), dim3(64), 0, stream, fb, layout, tabs);
+    hipLaunchKernelGGL(k_fc_codes, dim3(kRunPairs, cnt), dim3(256), 0, stream, tabs, fb, codes);
+    hipLaunchKernelGGL(k_fc_pairs, dim3(kTilePairs, cnt), dim3(256), 0, stream, tabs, codes, W, T, S);
+    hipLaunchKernelGGL(k_fc_border, dim3(S, cnt), dim3(256), 0, stream, tabs, W, T, S);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------- EV dump --
@@ -483,47 +580,24 @@ int mtt_combo_index(int a, int b) {
 
 int mtt_flop_counts(const uint8_t* flops, int n_flops, int32_t* d_W, int32_t* d_T, int layout,
                     hipStream_t stream) {
-    MTT_ARG(n_flops >= 0, "mtt_flop_counts: negative n_flops");
-    if (n_flops == 0) return MTT_OK;
-    MTT_ARG(flops && d_W && d_T, "mtt_flop_counts: null pointer");
-    MTT_ARG(layout == MTT_LAYOUT_GLOBAL1326 || layout == MTT_LAYOUT_ALIVE1216,
-            "mtt_flop_counts: unknown layout %d", layout);
-    for (int f = 0; f < n_flops; ++f)
-        MTT_ARG(valid_flop(flops + 3 * f), "mtt_flop_counts: flop %d has invalid or repeated cards", f);
-    int rc = upload_runouts();
-    if (rc) return rc;
-    const int S = layout == MTT_LAYOUT_GLOBAL1326 ? 1326 : kPadFlop;
-    // Work in batches so the score workspace stays bounded (6 MB per flop).
-    const int kBatch = 64;
-    int nb = n_flops < kBatch ? n_flops : kBatch;
-    std::vector<FlopTables> tabs(nb);
-    FlopTables* d_tabs = nullptr;
-    uint8_t* d_flops = nullptr;
-    uint32_t* d_sc = nullptr;
-    auto cleanup = [&]() {
-        hipFree(d_tabs); hipFree(d_flops); hipFree(d_sc);
-    };
-    hipError_t e = hipMalloc(&d_tabs, sizeof(FlopTables) * nb);
-    if (e == hipSuccess) e = hipMalloc(&d_flops, 3 * nb);
-    if (e == hipSuccess) e = hipMalloc(&d_sc, sizeof(uint32_t) * (size_t)nb * kRunPad * kHandPad);
-    if (e != hipSuccess) { cleanup(); return hip_fail(e, "mtt_flop_counts workspace", __FILE__, __LINE__); }
-    for (int f0 = 0; f0 < n_flops; f0 += nb) {
-        int cnt = n_flops - f0 < nb ? n_flops - f0 : nb;
-        for (int k = 0; k < cnt; ++k) build_flop_tables(flops + 3 * (f0 + k), layout, tabs[k]);
-        e = hipMemcpyAsync(d_tabs, tabs.data(), sizeof(FlopTables) * cnt, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(d_flops, flops + 3 * f0, 3 * cnt, hipMemcpyHostToDevice, stream);
-        if (e != hipSuccess) { cleanup(); return hip_fail(e, "mtt_flop_counts upload", __FILE__, __LINE__); }
-        int32_t* W = d_W + (size_t)f0 * S * S;
-        int32_t* T = d_T + (size_t)f0 * S * S;
-        hipLaunchKernelGGL(k_fc_scores, dim3(kRunPad, cnt), dim3(256), 0, stream, d_tabs, d_flops, d_sc);
-        hipLaunchKernelGGL(k_fc_compare, dim3(kHandPad / kTile, kHandPad / kTile, cnt), dim3(256), 0, stream,
-                           d_tabs, d_sc, W, S);
-        hipLaunchKernelGGL(k_fc_finish, dim3((S + 255) / 256, S, cnt), dim3(256), 0, stream, d_tabs, W, T, S);
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);  // tabs reused next batch
-        if (e != hipSuccess) { cleanup(); return hip_fail(e, "mtt_flop_counts kernels", __FILE__, __LINE__); }
+    int rc = fc_check("mtt_flop_counts", flops, n_flops, d_W, d_T, layout);
+    if (rc || n_flops == 0) return rc;
+    const size_t S = layout == MTT_LAYOUT_GLOBAL1326 ? 1326 : kPadFlop;
+    // Batches of up to 64 flops share one workspace (193 MB), reused in
+    // stream order; it is freed once the stream has drained.
+    const int nb = n_flops < kFcBatch ? n_flops : kFcBatch;
+    void* ws = nullptr;
+    hipError_t e = hipMalloc(&ws, fc_workspace_bytes(nb));
+    if (e != hipSuccess) return hip_fail(e, "mtt_flop_counts workspace", __FILE__, __LINE__);
+    for (int f0 = 0; f0 < n_flops && e == hipSuccess; f0 += nb) {
+        const int cnt = n_flops - f0 < nb ? n_flops - f0 : nb;
+        e = fc_launch(flops + 3 * (size_t)f0, cnt, d_W + (size_t)f0 * S * S, d_T + (size_t)f0 * S * S, layout, ws,
+                      stream);
     }
-    cleanup();
+    hipError_t e2 = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = e2;
+    hipFree(ws);
+    if (e != hipSuccess) return hip_fail(e, "mtt_flop_counts kernels", __FILE__, __LINE__);
     return MTT_OK;
 }
 

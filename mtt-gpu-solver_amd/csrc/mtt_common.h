@@ -109,6 +109,32 @@ MTT_HD uint32_t eval7(const uint8_t* c) {
     return score_packed(m);
 }
 
+// Order-preserving 16-bit code of a score: code16(a) < code16(b) exactly when
+// a < b, equal exactly when equal.  The preorder has few classes once
+// kickers are dropped: a category base plus the payload's position in its
+// category, high card and flush by their 13-bit rank masks (numeric order of
+// equal-size masks is the order of their descending rank lists).  Range
+// [0, 16781); the count builder compares these in 16-bit lanes.
+MTT_HD uint32_t code16(uint32_t s) {
+    uint32_t cat = s >> 24, p = s & 0xFFFFFFu;
+    switch (cat) {
+        case 0: return p;                                              // 7-rank mask
+        case 1: return 8192u + (p >> 12);                              // pair
+        case 2: return 8205u + (p >> 8) * 13u + ((p >> 4) & 15u);      // two pair
+        case 3: return 8374u + (p >> 8);                               // trips
+        case 4: return 8387u + p - 3u;                                 // straight top 3..12
+        case 5: {                                                      // flush: 5 rank nibbles
+            uint32_t m = 0;
+            for (int k = 0; k < 5; ++k) m |= 1u << ((p >> (4 * k)) & 15u);
+            return 8397u + m;
+        }
+        case 6: return 16589u + (p >> 4) * 13u + (p & 15u);           // full house
+        case 7: return 16758u + (p >> 4);                              // quads
+        default: return 16771u + p - 3u;                               // straight flush
+    }
+}
+constexpr uint32_t kCode16Range = 16781;
+
 // Lexicographic rank of (a, b), a < b, among the 1,326 two-card combos.
 MTT_HD int combo_index(int a, int b) { return a * 51 - a * (a - 1) / 2 + (b - a - 1); }
 

@@ -12,6 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmtt.so")
 
 MTT_OK = 0
+MTT_ERR_ARG, MTT_ERR_OOM, MTT_ERR_HIP, MTT_ERR_TREE, MTT_ERR_STATE = 1, 2, 3, 4, 5
 LAYOUT_GLOBAL1326 = 0
 LAYOUT_ALIVE1216 = 1
 N_ALIVE = 1176
