@@ -119,6 +119,14 @@ enum { MTT_LAYOUT_GLOBAL1326 = 0, MTT_LAYOUT_ALIVE1216 = 1 };
  * array of n_flops*3 cards.  Bit-exact against the reference evaluator. */
 int mtt_flop_counts(const uint8_t* flops, int n_flops, int32_t* d_W, int32_t* d_T,
                     int layout, mtt_stream_t stream);
+/* No counterpart in the reference.  mtt_flop_counts without any device
+ * allocation, for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc): same outputs,
+ * at most 16 flops per call (MTT_ERR_ARG beyond), workspace in a static
+ * 48,294,400-byte device array per GPU.  Asynchronous on `stream`; calls that
+ * overlap on one GPU race on that workspace. */
+int mtt_flop_counts_noalloc(const uint8_t* flops, int n_flops, int32_t* d_W, int32_t* d_T,
+                            int layout, mtt_stream_t stream);
 
 /* kernel_evaluate_flop_ev, mtt_ev_dumper.cu:85-116: subsampled per-combo EV
  * of one flop against reach_p1/reach_p0 (1216 floats).  d_combos 1216

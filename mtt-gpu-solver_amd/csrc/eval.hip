@@ -280,13 +280,41 @@ __global__ void k_fc_border(const FlopTables* __restrict__ tabs, int32_t* __rest
         if (row_dead || t.alive_of_out[oj] < 0) { Wr[oj] = 0; Tr[oj] = 0; }
 }
 
+// Workspace of the allocation-free entry point: kNoAllocFlops flops' tables
+// and codes in one file-scope device array (each GPU the library runs on
+// holds its own copy until the process ends).  Host code never uses the
+// array's name as an address: the per-device address comes from
+// hipGetSymbolAddress with that device current.
+constexpr int kNoAllocFlops = 16;
+constexpr size_t kFcFlopBytes = sizeof(FlopTables) + sizeof(uint32_t) * (size_t)kRunPairs * kHandPad;
+constexpr size_t kNoAllocBytes = kNoAllocFlops * kFcFlopBytes;   // 48,294,400 bytes
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+static int noalloc_scratch(void** out) {
+    int dev = -1;
+    MTT_HIP(hipGetDevice(&dev));
+    MTT_ARG(dev >= 0 && dev < kMaxDevices, "mtt_flop_counts_noalloc: device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "mtt_flop_counts_noalloc: scratch address", __FILE__, __LINE__);
+        if (!p) { set_error("mtt_flop_counts_noalloc: no scratch address on device %d", dev); return MTT_ERR_HIP; }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MTT_OK;
+}
+
 static bool valid_flop(const uint8_t* f) {
     return f[0] < 52 && f[1] < 52 && f[2] < 52 && f[0] != f[1] && f[0] != f[2] && f[1] != f[2];
 }
 
-static size_t fc_workspace_bytes(int n_flops) {
-    return (size_t)n_flops * (sizeof(FlopTables) + sizeof(uint32_t) * (size_t)kRunPairs * kHandPad);
-}
+static size_t fc_workspace_bytes(int n_flops) { return (size_t)n_flops * kFcFlopBytes; }
 
 static int fc_check(const char* fn, const uint8_t* flops, int n_flops, const int32_t* d_W, const int32_t* d_T,
                     int layout) {
@@ -598,6 +626,25 @@ int mtt_flop_counts(const uint8_t* flops, int n_flops, int32_t* d_W, int32_t* d_
     if (e == hipSuccess) e = e2;
     hipFree(ws);
     if (e != hipSuccess) return hip_fail(e, "mtt_flop_counts kernels", __FILE__, __LINE__);
+    return MTT_OK;
+}
+
+int mtt_flop_counts_noalloc(const uint8_t* flops, int n_flops, int32_t* d_W, int32_t* d_T, int layout,
+                            hipStream_t stream) {
+    int rc = fc_check("mtt_flop_counts_noalloc", flops, n_flops, d_W, d_T, layout);
+    if (rc || n_flops == 0) return rc;
+    MTT_ARG(n_flops <= kNoAllocFlops,
+            "mtt_flop_counts_noalloc: %d flops exceed the %d-flop workspace (%zu bytes per GPU); "
+            "split the call or use mtt_flop_counts", n_flops, kNoAllocFlops, kNoAllocBytes);
+    void* ws = nullptr;
+    rc = noalloc_scratch(&ws);
+    if (rc) return rc;
+    // Every workspace byte a kernel reads is written earlier in the same
+    // launch sequence, so nothing is cleared.  No synchronisation: the call
+    // is asynchronous on `stream`, and calls that overlap on one GPU race
+    // on the shared workspace.
+    hipError_t e = fc_launch(flops, n_flops, d_W, d_T, layout, ws, stream);
+    if (e != hipSuccess) return hip_fail(e, "mtt_flop_counts_noalloc kernels", __FILE__, __LINE__);
     return MTT_OK;
 }
 
