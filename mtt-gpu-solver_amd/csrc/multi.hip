@@ -126,6 +126,55 @@ __device__ __forceinline__ double wave_incl_scan(double v) {
 
 __device__ __forceinline__ double pay_net(double net, double bf) { return net >= 0.0 ? net : bf * net; }
 
+// Showdown payoff of player p (still in) when its active opponents, in seat
+// order, are weaker (digit 0), tied (1) or stronger (2) than p: outcome
+// s = d0 + 3 d1 + 9 d2.  The pot is split into layers at the distinct
+// commitments of the players still in (all seats' chips up to each level);
+// p takes each layer it is eligible for unless an eligible opponent is
+// stronger, shared with the eligible tied ones.  Net chips, scaled by the
+// bubble factor when negative.
+__device__ double k4_showdown_pay(const mtt_flat_node4& nd, int p, int s, double bpp) {
+    int opp[3] = {0, 0, 0}, na = 0;
+    for (int o = 0; o < P4; ++o)
+        if (o != p && (nd.active_mask & (1 << o))) opp[na++] = o;
+    int dig[3];
+    for (int t = 0; t < 3; ++t) { dig[t] = s % 3; s /= 3; }
+    // distinct commitments of the players still in, ascending
+    double lv[P4];
+    int k = 0;
+    for (int q = 0; q < P4; ++q) {
+        if (!(nd.active_mask & (1 << q))) continue;
+        const double c = nd.committed[q];
+        int at = k;
+        bool dup = false;
+        for (int u = 0; u < k; ++u) {
+            if (fabs(lv[u] - c) <= 1e-6) dup = true;
+            else if (lv[u] > c && at == k) at = u;
+        }
+        if (dup) continue;
+        for (int u = k; u > at; --u) lv[u] = lv[u - 1];
+        lv[at] = c;
+        k++;
+    }
+    double won = 0.0, prev = 0.0;
+    for (int l = 0; l < k; ++l) {
+        const double top = lv[l];
+        double chips = 0.0;
+        for (int q = 0; q < P4; ++q) chips += fmin(fmax((double)nd.committed[q] - prev, 0.0), top - prev);
+        prev = top;
+        if ((double)nd.committed[p] < top - 1e-6) continue;      // p is not in this layer
+        bool lost = false;
+        int tied = 0;
+        for (int t = 0; t < na; ++t) {
+            if ((double)nd.committed[opp[t]] < top - 1e-6) continue;
+            lost |= dig[t] == 2;
+            tied += dig[t] == 1;
+        }
+        if (!lost) won += chips / (double)(tied + 1);
+    }
+    return pay_net(won - (double)nd.committed[p], bpp);
+}
+
 __global__ __launch_bounds__(TT) void k4_terminals(const int* __restrict__ terms, int nterm,
                                                    const mtt_flat_node4* __restrict__ nodes,
                                                    const float* __restrict__ reach, const mtt_board4* __restrict__ boards,
@@ -138,6 +187,7 @@ __global__ __launch_bounds__(TT) void k4_terminals(const int* __restrict__ terms
     __shared__ uint16_t pc[NP];
     __shared__ uint16_t cl[52][48];
     __shared__ uint8_t ncard[52];
+    __shared__ double pay[P4][27];
     const int t = blockIdx.x;
     if (t >= nterm) return;
     const int f = blockIdx.y;
@@ -153,6 +203,11 @@ __global__ __launch_bounds__(TT) void k4_terminals(const int* __restrict__ terms
     for (int e = tid; e < NP; e += TT) pc[e] = B.pos_combo[e];
     for (int e = tid; e < 52 * 48; e += TT) cl[e / 48][e % 48] = B.card_list[e / 48][e % 48];
     if (tid < 52) ncard[tid] = B.n_card[tid];
+    if (nd.type == MTT_NODE_SHOWDOWN && tid < P4 * 27) {
+        const int p = tid / 27;
+        pay[p][tid % 27] = (nd.active_mask & (1 << p)) ? k4_showdown_pay(nd, p, tid % 27, (double)bubble[p * P4 + p])
+                                                      : 0.0;
+    }
     // hero data, packed: cards | lt counts, tie group, le counts
     uint32_t hcl[HPT], hgr[HPT], hle[HPT];
 #pragma unroll
@@ -239,29 +294,33 @@ __global__ __launch_bounds__(TT) void k4_terminals(const int* __restrict__ terms
                                             : -bpp * cp;
                     v = pay * opp;
                 } else {
-                    double inact = 1.0, win = 1.0, allG = 1.0, allW = 1.0;
-                    int om = 0;  // active opponents
+                    // outcome masses per active opponent (weaker, tied, stronger),
+                    // contracted with p's payoff table; absent digits are "weaker" with mass 1
+                    double inact = 1.0, m[3][3];
+                    int na = 0;
 #pragma unroll
                     for (int o = 0; o < P4; ++o) {
                         if (o == p) continue;
                         if (nd.active_mask & (1 << o)) {
-                            om |= 1 << o;
-                            win *= F[o][h];
-                            allG *= G[o][h];
-                            allW *= W[o][h];
+                            m[na][0] = F[o][h];
+                            m[na][1] = (double)G[o][h] - (double)F[o][h];
+                            m[na][2] = (double)W[o][h] - (double)G[o][h];
+                            na++;
                         } else {
                             inact *= W[o][h];
                         }
                     }
-                    double acc = ((double)nd.pot_size - cp) * win + (-bpp * cp) * (allW - allG);
-                    for (int m = om; m > 0; m = (m - 1) & om) {  // nonempty tied subsets of the active opponents
-                        double pr = 1.0;
+                    for (int u = na; u < 3; ++u) { m[u][0] = 1.0; m[u][1] = 0.0; m[u][2] = 0.0; }
+                    double acc = 0.0;
 #pragma unroll
-                        for (int o = 0; o < P4; ++o) {
-                            if (!(om & (1 << o))) continue;
-                            pr *= (m & (1 << o)) ? (double)G[o][h] - (double)F[o][h] : (double)F[o][h];
+                    for (int d2 = 0; d2 < 3; ++d2) {
+                        double a1 = 0.0;
+#pragma unroll
+                        for (int d1 = 0; d1 < 3; ++d1) {
+                            const double* row = &pay[p][3 * d1 + 9 * d2];
+                            a1 += m[1][d1] * (m[0][0] * row[0] + m[0][1] * row[1] + m[0][2] * row[2]);
                         }
-                        acc += pr * pay_net((double)nd.pot_size / (double)(__builtin_popcount(m) + 1) - cp, bpp);
+                        acc += m[2][d2] * a1;
                     }
                     v = acc * inact;
                 }

@@ -3,14 +3,19 @@
 //
 // Betting rules (project-defined):
 //   players act in seat order 0..3, starting with seat 0;
-//   unopened  : check, each pot-fraction bet below the cap, all-in (= cap)
-//               when the menu offers it;
-//   facing bet: fold, call, and - while fewer than raise_cap raises were
-//               made - a raise to raise_mult x the current bet (below the
-//               cap) and an all-in raise (to the cap) when offered;
-//   the cap is the smallest stack among the players still in the hand, so
-//   every call is possible without side pots;
-//   a player with no chips behind no longer acts;
+//   the actor's cap is the smaller of its own stack and the largest stack
+//   of another player still in the hand (chips nobody can call are never
+//   bet);
+//   unopened  : check, each pot-fraction bet (a size at or above the cap
+//               becomes a bet of the cap, i.e. all-in), all-in when the menu
+//               offers it;
+//   facing bet: fold, call (all-in for less when the stack is short of the
+//               bet: the excess of the others forms side pots), and - while
+//               fewer than raise_cap raises were made - a raise to
+//               raise_mult x the current bet (the cap when that is at or
+//               above it) and an all-in raise when offered;
+//   a player with no chips behind no longer acts, nor does one facing no
+//   bet when nobody else in the hand can still act;
 //   the street closes when every player still in the hand has acted since
 //   the last bet or raise: one player left is a fold terminal (active_mask
 //   = the winner's bit), otherwise a showdown among the players left
@@ -43,11 +48,12 @@ struct Builder4 {
     int n_info = 0;
     explicit Builder4(const mtt_4way_game& g_) : g(g_) {}
 
-    double cap(const S4& s) const {
-        double c = 1e300;
+    // Largest commitment the actor can make that someone else can match.
+    double cap(const S4& s, int a) const {
+        double other = 0.0;
         for (int p = 0; p < 4; ++p)
-            if (s.active & (1 << p)) c = std::fmin(c, (double)g.stack[p]);
-        return c;
+            if (p != a && (s.active & (1 << p))) other = std::fmax(other, (double)g.stack[p]);
+        return std::fmin((double)g.stack[a], other);
     }
     double level(const S4& s) const {
         double l = 0.0;
@@ -62,10 +68,18 @@ struct Builder4 {
     bool can_act(const S4& s, int p) const {
         return (s.active & (1 << p)) && (double)g.stack[p] - s.bet[p] > eps;
     }
+    // Someone other than p still has chips behind.
+    bool others_can_act(const S4& s, int p) const {
+        for (int o = 0; o < 4; ++o)
+            if (o != p && can_act(s, o)) return true;
+        return false;
+    }
     int next_actor(const S4& s, int from) const {
         for (int k = 1; k <= 4; ++k) {
             int p = (from + k) & 3;
-            if ((s.need & (1 << p)) && can_act(s, p)) return p;
+            if (!(s.need & (1 << p)) || !can_act(s, p)) continue;
+            if (level(s) <= s.bet[p] + eps && !others_can_act(s, p)) continue;  // nothing left to contest
+            return p;
         }
         return -1;
     }
@@ -75,29 +89,24 @@ struct Builder4 {
     std::vector<Act> actions(const S4& s) const {
         const mtt_street_menu& m = g.menu;
         const int a = s.actor;
-        const double L = level(s), C = cap(s), P = pot(s);
+        const double L = level(s), C = cap(s, a), P = pot(s);
         std::vector<Act> v;
         auto add_to = [&](double to) {
-            if (to <= L + eps || to > C + eps) return;
+            if (to > C) to = C;                       // at or above the cap: all-in
+            if (to <= L + eps) return;
             for (auto& x : v)
                 if (x.kind == 1 && std::fabs(x.to - to) < eps) return;
             v.push_back({1, to});
         };
         if (L <= s.bet[a] + eps) {
             v.push_back({0, 0});
-            for (int k = 0; k < m.n_sizes; ++k) {
-                double to = (double)m.sizes[k] * P;
-                if (to < C - eps) add_to(to);
-            }
+            for (int k = 0; k < m.n_sizes; ++k) add_to((double)m.sizes[k] * P);
             if (m.allin) add_to(C);
         } else {
             v.push_back({2, 0});
             v.push_back({3, 0});
-            if (s.raises < m.raise_cap) {
-                if (m.raise_mult > 0) {
-                    double to = (double)m.raise_mult * L;
-                    if (to < C - eps) add_to(to);
-                }
+            if (s.raises < m.raise_cap && C > L + eps) {
+                if (m.raise_mult > 0) add_to((double)m.raise_mult * L);
                 if (m.allin) add_to(C);
             }
         }
@@ -162,7 +171,7 @@ struct Builder4 {
                 if (x.kind == 2) {
                     c.active &= ~(1 << a);
                 } else if (x.kind == 3) {
-                    c.bet[a] = level(s);
+                    c.bet[a] = std::fmin(level(s), (double)g.stack[a]);   // all-in for less when short
                 } else if (x.kind == 1) {
                     if (level(s) > eps) c.raises++;  // a raise over a bet (the opening bet is not one)
                     c.bet[a] = x.to;

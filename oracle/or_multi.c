@@ -9,9 +9,10 @@
  * Restated from the reference (paths relative to its repository root):
  *   or_4way_forward / or_4way_backward   multiway_4way_engine.cu:41-83, 219-257
  *   or_4way_terminals                    multiway_4way_engine.cu:85-217, with
- *       card removal against the hero's hand and split pots for ties (the
- *       reference has neither); the strength-sorted exclusive prefix scan is
- *       kept, extended with per-card prefix lists for the card removal.
+ *       card removal against the hero's hand, split pots for ties and side
+ *       pots for unequal commitments (the reference has none of these); the
+ *       strength-sorted exclusive prefix scan is kept, extended with
+ *       per-card prefix lists for the card removal.
  *   or_dcfr_step                         preflop_grid_generator.cu:387-410,
  *       generalised to R <- (R + r) * (R + r > 0 ? w_pos : w_neg); w_neg = 0
  *       is the reference's floor-then-scale form.
@@ -147,6 +148,61 @@ void or_4way_board(const uint8_t flop[3], or_board4* B) {
 /* ------------------------------------------------------ terminal values -- */
 static double pay_net(double net, double bf) { return net >= 0.0 ? net : bf * net; }
 
+/* Pot layers of a showdown: the distinct commitments L1 < L2 < ... of the
+ * players still in; layer l holds every seat's chips between L(l-1) and Ll
+ * (folded seats' included) and is contested by the players still in whose
+ * commitment reaches Ll.  One layer when everyone in matched. */
+static int pot_layers(const or_node4* nd, double* amt, int* elig) {
+    double lv[P4];
+    int k = 0;
+    for (int p = 0; p < P4; ++p) {
+        if (!(nd->active_mask & (1 << p))) continue;
+        double c = nd->committed[p];
+        int dup = 0;
+        for (int t = 0; t < k; ++t) if (fabs(lv[t] - c) <= 1e-6) dup = 1;
+        if (!dup) lv[k++] = c;
+    }
+    for (int a = 1; a < k; ++a)             /* insertion sort, ascending */
+        for (int b = a; b > 0 && lv[b] < lv[b - 1]; --b) { double t = lv[b]; lv[b] = lv[b - 1]; lv[b - 1] = t; }
+    double prev = 0.0;
+    for (int l = 0; l < k; ++l) {
+        amt[l] = 0.0;
+        elig[l] = 0;
+        for (int q = 0; q < P4; ++q) {
+            double x = (double)nd->committed[q] - prev, cap = lv[l] - prev;
+            amt[l] += x <= 0.0 ? 0.0 : (x < cap ? x : cap);
+            if ((nd->active_mask & (1 << q)) && nd->committed[q] >= lv[l] - 1e-6) elig[l] |= 1 << q;
+        }
+        prev = lv[l];
+    }
+    return k;
+}
+
+/* Payoff of player p (still in) at a showdown when each active opponent
+ * opp[t] is weaker (d_t = 0), tied (1) or stronger (2) than p, outcome
+ * s = sum d_t 3^t: per layer p is eligible for, p wins it when no eligible
+ * opponent is stronger, shared with the eligible tied ones; the net result
+ * is scaled by the bubble factor when negative. */
+static double showdown_pay(const or_node4* nd, int p, const int* opp, int na, int s, double bpp) {
+    double amt[P4];
+    int elig[P4];
+    int nl = pot_layers(nd, amt, elig);
+    int d[3] = {0, 0, 0};
+    for (int t = 0; t < na; ++t) { d[t] = s % 3; s /= 3; }
+    double won = 0.0;
+    for (int l = 0; l < nl; ++l) {
+        if (!(elig[l] & (1 << p))) continue;
+        int lost = 0, tied = 0;
+        for (int t = 0; t < na; ++t) {
+            if (!(elig[l] & (1 << opp[t]))) continue;
+            if (d[t] == 2) lost = 1;
+            if (d[t] == 1) tied++;
+        }
+        if (!lost) won += amt[l] / (double)(tied + 1);
+    }
+    return pay_net(won - (double)nd->committed[p], bpp);
+}
+
 /* Values of one terminal for all players and combos, in double.
  * reach[p*NP4 + i]; out[p*NP4 + i]. */
 void or_4way_terminal(const or_board4* B, const or_node4* nd, const float* reach, const double* bf, double* out) {
@@ -174,6 +230,13 @@ void or_4way_terminal(const or_board4* B, const or_node4* nd, const float* reach
         for (int p = 0; p < P4; ++p)
             if (nd->active_mask & (1 << p)) { winner = p; break; }
     for (int p = 0; p < P4; ++p) {
+        int opp_act[P4], na = 0;
+        for (int o = 0; o < P4; ++o)
+            if (o != p && (nd->active_mask & (1 << o))) opp_act[na++] = o;
+        double pay[27];
+        const int ns = na == 3 ? 27 : (na == 2 ? 9 : 3);
+        if (nd->type == MTT_NODE_SHOWDOWN && (nd->active_mask & (1 << p)))
+            for (int s = 0; s < ns; ++s) pay[s] = showdown_pay(nd, p, opp_act, na, s, bf[p * P4 + p]);
         for (int i = 0; i < NP4; ++i) {
             double v = 0.0;
             if (i < NA4) {
@@ -181,32 +244,30 @@ void or_4way_terminal(const or_board4* B, const or_node4* nd, const float* reach
                 if (nd->type == MTT_NODE_FOLD) {
                     double opp = 1.0;
                     for (int o = 0; o < P4; ++o) if (o != p) opp *= W[o][i];
-                    const double pay = p == winner ? nd->pot_size - cp : -bf[p * P4 + winner] * cp;
-                    v = pay * opp;
+                    const double py = p == winner ? nd->pot_size - cp : -bf[p * P4 + winner] * cp;
+                    v = py * opp;
                 } else if (!(nd->active_mask & (1 << p))) {
                     double opp = 1.0;
                     for (int o = 0; o < P4; ++o) if (o != p) opp *= W[o][i];
                     v = -bf[p * P4 + p] * cp * opp;
                 } else {
-                    int opp_act[P4], na = 0;
                     double inact = 1.0;
-                    for (int o = 0; o < P4; ++o) {
-                        if (o == p) continue;
-                        if (nd->active_mask & (1 << o)) opp_act[na++] = o;
-                        else inact *= W[o][i];
+                    for (int o = 0; o < P4; ++o)
+                        if (o != p && !(nd->active_mask & (1 << o))) inact *= W[o][i];
+                    /* outcome masses per active opponent: weaker, tied, stronger */
+                    double m[3][3];
+                    for (int t = 0; t < na; ++t) {
+                        int o = opp_act[t];
+                        m[t][0] = F[o][i];
+                        m[t][1] = G[o][i] - F[o][i];
+                        m[t][2] = W[o][i] - G[o][i];
                     }
-                    double win = 1.0, allG = 1.0, allW = 1.0;
-                    for (int t = 0; t < na; ++t) { win *= F[opp_act[t]][i]; allG *= G[opp_act[t]][i]; allW *= W[opp_act[t]][i]; }
-                    const double bpp = bf[p * P4 + p];
-                    double acc = (nd->pot_size - cp) * win + (-bpp * cp) * (allW - allG);
-                    for (int m = 1; m < (1 << na); ++m) {  /* tied subsets */
+                    double acc = 0.0;
+                    for (int s = 0; s < ns; ++s) {
                         double pr = 1.0;
-                        int k = 0;
-                        for (int t = 0; t < na; ++t) {
-                            if (m & (1 << t)) { pr *= G[opp_act[t]][i] - F[opp_act[t]][i]; k++; }
-                            else pr *= F[opp_act[t]][i];
-                        }
-                        acc += pr * pay_net(nd->pot_size / (double)(k + 1) - cp, bpp);
+                        int x = s;
+                        for (int t = 0; t < na; ++t) { pr *= m[t][x % 3]; x /= 3; }
+                        acc += pr * pay[s];
                     }
                     v = acc * inact;
                 }
@@ -255,23 +316,46 @@ void or_4way_terminal_brute(const or_board4* B, const or_node4* nd, const float*
                             } else if (!(nd->active_mask & (1 << p))) {
                                 pay = -bf[p * P4 + p] * cp;
                             } else {
+                                /* chips layer by layer: each pot layer goes to the strongest of the
+                                 * players still in whose commitment reaches it, split on ties */
                                 int js[3] = {j0, j1, j2};
-                                uint32_t best = 0;
-                                int any = 0;
-                                for (int t = 0; t < 3; ++t)
-                                    if (nd->active_mask & (1 << opp[t])) {
-                                        if (!any || score[js[t]] > best) best = score[js[t]];
-                                        any = 1;
-                                    }
-                                const uint32_t s = score[i];
-                                if (!any || s > best) pay = nd->pot_size - cp;
-                                else if (s < best) pay = -bf[p * P4 + p] * cp;
-                                else {
-                                    int tied = 0;
-                                    for (int t = 0; t < 3; ++t)
-                                        if ((nd->active_mask & (1 << opp[t])) && score[js[t]] == s) tied++;
-                                    pay = pay_net(nd->pot_size / (double)(tied + 1) - cp, bf[p * P4 + p]);
+                                uint32_t sc[P4];
+                                sc[p] = score[i];
+                                for (int t = 0; t < 3; ++t) sc[opp[t]] = score[js[t]];
+                                double lv[P4];
+                                int nl = 0;
+                                for (int q = 0; q < P4; ++q) {
+                                    if (!(nd->active_mask & (1 << q))) continue;
+                                    int seen = 0;
+                                    for (int u = 0; u < nl; ++u) if (fabs(lv[u] - nd->committed[q]) <= 1e-6) seen = 1;
+                                    if (!seen) lv[nl++] = nd->committed[q];
                                 }
+                                double won = 0.0, prev = 0.0;
+                                for (int round = 0; round < nl; ++round) {
+                                    double lo = 1e300;             /* next level above prev */
+                                    for (int u = 0; u < nl; ++u) if (lv[u] > prev + 1e-6 && lv[u] < lo) lo = lv[u];
+                                    double chips = 0.0;
+                                    for (int q = 0; q < P4; ++q) {
+                                        double c = nd->committed[q];
+                                        double hi = c < lo ? c : lo;
+                                        if (hi > prev) chips += hi - prev;
+                                    }
+                                    uint32_t best = 0;
+                                    int any = 0, nwin = 0, hero_in = 0;
+                                    for (int q = 0; q < P4; ++q)
+                                        if ((nd->active_mask & (1 << q)) && nd->committed[q] >= lo - 1e-6) {
+                                            if (!any || sc[q] > best) best = sc[q];
+                                            any = 1;
+                                        }
+                                    for (int q = 0; q < P4; ++q)
+                                        if ((nd->active_mask & (1 << q)) && nd->committed[q] >= lo - 1e-6 && sc[q] == best) {
+                                            nwin++;
+                                            if (q == p) hero_in = 1;
+                                        }
+                                    if (hero_in) won += chips / (double)nwin;
+                                    prev = lo;
+                                }
+                                pay = pay_net(won - cp, bf[p * P4 + p]);
                             }
                             v += w * pay;
                         }

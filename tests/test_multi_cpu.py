@@ -23,38 +23,65 @@ def _game(M, stacks=(30.0, 45.0, 12.0, 60.0)):
 
 
 # ------------------------------------------------------------------ tree --
-def test_tree4_invariants(M):
-    nodes, ni = M.tree4(_game(M))
-    assert len(nodes) == 428 and ni == 195
+SEED0_BEHIND = (50.272126549109075, 22.734003532290274, 5.573014295214602, 3.739572664639682)  # config 5, seed 0
+
+
+def _check_tree4(nodes, ni, stacks):
     dec = nodes[nodes["type"] == 0]
     assert dec["num_actions"].min() >= 2 and dec["num_actions"].max() <= 3
     assert np.array_equal(np.sort(dec["infoset_offset"]), np.arange(ni))
     seen = np.zeros(len(nodes), int)
     for n in np.nonzero(nodes["type"] == 0)[0]:
         x = nodes[n]
+        p = int(x["player"])
+        assert x["committed"][p] - 2.5 < stacks[p] - 1e-6          # the actor has chips behind
         for a in range(x["num_actions"]):
             c = x["children_offset"] + a
             seen[c] += 1
             ch = nodes[c]
             # pot accounting: the pot is the sum of the commitments
             assert abs(ch["pot_size"] - ch["committed"].sum()) < 1e-4
-            # only the actor's commitment changes
+            # only the actor's commitment changes, and never beyond its stack
             d = ch["committed"] - x["committed"]
-            assert np.all(np.abs(np.delete(d, x["player"])) < 1e-6)
-            assert d[x["player"]] >= -1e-6
+            assert np.all(np.abs(np.delete(d, p)) < 1e-6)
+            assert d[p] >= -1e-6 and ch["committed"][p] - 2.5 <= stacks[p] + 1e-4
     assert seen[0] == 0 and np.all(seen[1:] == 1)  # a tree: every node but the root has one parent
     for x in nodes[nodes["type"] == 1]:
         assert bin(int(x["active_mask"])).count("1") == 1
+    n_side = 0
     for x in nodes[nodes["type"] == 2]:
         m = int(x["active_mask"])
         assert bin(m).count("1") >= 2
         live = [p for p in range(4) if m >> p & 1]
-        assert len({round(float(x["committed"][p]), 4) for p in live}) == 1  # everyone still in matched
-    # bets never exceed the smallest stack still in the hand (no side pots)
+        top = max(float(x["committed"][p]) for p in live)
+        # everyone still in matched the largest commitment or is all-in for less
+        for p in live:
+            c = float(x["committed"][p])
+            assert abs(c - top) < 1e-4 or abs(c - 2.5 - stacks[p]) < 1e-4
+        n_side += len({round(float(x["committed"][p]), 4) for p in live}) > 1
+        # no chips above what a player still in can win back
+        assert max(float(v) for v in x["committed"]) <= top + 1e-4
+    return n_side
+
+
+def test_tree4_invariants(M):
     stacks = np.array([30.0, 45.0, 12.0, 60.0])
-    for x in nodes[nodes["type"] != 1]:
-        live = [p for p in range(4) if int(x["active_mask"]) >> p & 1]
-        assert max(x["committed"][p] for p in live) - 2.5 <= stacks[live].min() + 1e-4
+    nodes, ni = M.tree4(_game(M, stacks))
+    assert len(nodes) == 677 and ni == 308
+    assert _check_tree4(nodes, ni, stacks) > 0          # the 12bb stack is called all-in for less
+
+
+def test_tree4_config5_seed0_side_pots(M):
+    """Config 5's seed-0 stacks (5.57 and 3.74bb behind for seats 2 and 3):
+    bets the short stacks cannot match are called all-in for less, leaving
+    side pots; sizes at or above a short actor's stack collapse into one
+    all-in, so the tree has 470 nodes (677 with deep stacks)."""
+    stacks = np.array(SEED0_BEHIND)
+    nodes, ni = M.tree4(_game(M, stacks))
+    assert len(nodes) == 470 and ni == 211
+    assert _check_tree4(nodes, ni, stacks) == 120
+    deep, _ = M.tree4(_game(M, (200.0,) * 4))
+    assert len(deep) == 677
 
 
 def test_tree4_check_around_and_fold_out(M):
@@ -110,6 +137,77 @@ def test_terminal_prefix_scan_matches_brute_force(M, oracle, seed):
         a = oracle.terminal4(board, nodes[n:n + 1], reach, bf)
         b = oracle.terminal4_brute(board, nodes[n:n + 1], reach, bf, score)
         assert np.allclose(a, b, rtol=1e-9, atol=1e-12), key
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_terminal_side_pots_match_brute_force(M, oracle, seed):
+    """Showdowns with unequal commitments (short stacks all-in for less):
+    the layered payoff tables against chips resolved layer by layer for
+    every tuple of opponent hands."""
+    rng = np.random.default_rng(seed + 10)
+    nodes, _ = M.tree4(_game(M, SEED0_BEHIND))
+    flop = np.sort(rng.choice(52, 3, replace=False))
+    board = oracle.board4(flop)
+    t = board[0]
+    score = np.array([oracle.evaln([*flop, t["c1"][i], t["c2"][i]]) for i in range(1176)], np.uint32)
+    bf = rng.uniform(1.0, 1.6, (4, 4))
+    reach = _sparse_reach(rng, 8)
+    picks = {}
+    for n in np.nonzero(nodes["type"] == 2)[0]:
+        x = nodes[n]
+        live = [p for p in range(4) if int(x["active_mask"]) >> p & 1]
+        levels = len({round(float(x["committed"][p]), 4) for p in live})
+        if levels > 1:
+            picks.setdefault((len(live), levels), n)
+    assert {k[0] for k in picks} >= {2, 3, 4} and max(k[1] for k in picks) >= 3
+    for key, n in sorted(picks.items()):
+        a = oracle.terminal4(board, nodes[n:n + 1], reach, bf)
+        b = oracle.terminal4_brute(board, nodes[n:n + 1], reach, bf, score)
+        assert np.allclose(a, b, rtol=1e-9, atol=1e-12), key
+
+
+def test_terminal_side_pot_hand_cases(M, oracle):
+    """A short stack all-in for 5 against two players at 20, a fourth seat
+    folded after putting in 8: main pot 4 x 5 = 20 for all three, side pot
+    15 + 15 + 3 = 33 for the two deep players."""
+    flop = [card("2c"), card("7d"), card("Kh")]
+    board = oracle.board4(flop)
+    t = board[0]
+    idx = {(int(t["c1"][i]), int(t["c2"][i])): i for i in range(1176)}
+    def combo(a, b):
+        a, b = sorted((card(a), card(b)))
+        return idx[(a, b)]
+    node = np.zeros(1, [("type", "u1"), ("player", "u1"), ("num_actions", "u1"), ("active_mask", "u1"),
+                        ("children_offset", "<i4"), ("infoset_offset", "<i4"), ("pot_size", "<f4"),
+                        ("committed", "<f4", (4,))])
+    node["type"] = 2
+    node["active_mask"] = 0b0111
+    node["pot_size"] = 53.0
+    node["committed"] = [5.0, 20.0, 20.0, 8.0]
+    bf = np.full((4, 4), 1.5)
+    reach = np.zeros((4, 1216), np.float32)
+    short, mid, weak = combo("Kc", "Kd"), combo("Ac", "Ad"), combo("3s", "4s")
+    reach[0, short] = reach[1, mid] = reach[2, weak] = 1.0
+    reach[3, combo("5h", "6h")] = 1.0
+    v = oracle.terminal4(board, node, reach, bf)
+    # the short stack's set of kings wins the main pot: 20 - 5
+    assert abs(v[0, short] - 15.0) < 1e-9
+    # aces lose the main pot but win the side pot: 33 - 20
+    assert abs(v[1, mid] - 13.0) < 1e-9
+    # the weakest hand loses both: -20, scaled by its bubble factor
+    assert abs(v[2, weak] - (-1.5 * 20.0)) < 1e-9
+    # aces against a weaker short stack win both pots: 53 - 20
+    reach[0] = 0
+    reach[0, combo("3c", "4c")] = 1.0
+    v = oracle.terminal4(board, node, reach, bf)
+    assert abs(v[1, mid] - 33.0) < 1e-9
+    # a tie for the side pot between the deep players, short stack best
+    reach[0] = 0
+    reach[0, short] = 1.0
+    reach[2] = 0
+    reach[2, combo("Ah", "As")] = 1.0
+    v = oracle.terminal4(board, node, reach, bf)
+    assert abs(v[1, mid] - (16.5 - 20.0) * 1.5) < 1e-9
 
 
 def test_terminal_ties_split_and_card_removal(M, oracle):

@@ -20,9 +20,14 @@ def M(mtt):
     return _load("mtt_multi", os.path.join(PKG, "multi.py"))
 
 
-def _setup(M, seed=0):
+STACKS = {"mixed": [30.0, 45.0, 12.0, 60.0],     # 677 nodes, the 12bb stack called all-in for less
+          "config5_seed0": [50.272126549109075, 22.734003532290274, 5.573014295214602,
+                            3.739572664639682]}     # 470 nodes, 120 of 221 showdowns with side pots
+
+
+def _setup(M, seed=0, stacks="mixed"):
     rng = np.random.default_rng(seed)
-    game = M.game4([2.5] * 4, [30.0, 45.0, 12.0, 60.0], M.config5_menu())
+    game = M.game4([2.5] * 4, STACKS[stacks], M.config5_menu())
     nodes, ni = M.tree4(game)
     flop = np.sort(rng.choice(52, 3, replace=False)).astype(np.uint8)
     ranges = np.zeros((4, 1216), np.float32)
@@ -43,10 +48,13 @@ def _depths(nodes):
     return [d for d in dec if len(d)]
 
 
-def test_4way_terminals_match_oracle(M, mtt, oracle, torch_dev):
+@pytest.mark.parametrize("stacks", ["mixed", "config5_seed0"])
+def test_4way_terminals_match_oracle(M, mtt, oracle, torch_dev, stacks):
+    """Every terminal of the tree (folds, showdowns of 2-4 players, side pots
+    where short stacks called all-in for less) against the oracle."""
     import torch
-    game, nodes, ni, flop, ranges, bf = _setup(M, 1)
-    print("4-way terminals, seed 1, flop", flop.tolist(), flush=True)
+    game, nodes, ni, flop, ranges, bf = _setup(M, 1, stacks)
+    print(f"4-way terminals, stacks {stacks}, seed 1, flop", flop.tolist(), flush=True)
     rng = np.random.default_rng(7)
     nn = len(nodes)
     reach = rng.uniform(0, 1e-3, (nn, 4, 1216)).astype(np.float32)
@@ -165,18 +173,22 @@ def test_dcfr_step_gpu_matches_oracle(mtt, oracle, torch_dev):
         assert np.allclose(g.cpu().numpy(), c, rtol=1e-5, atol=1e-6)
 
 
-def test_config5_full_size(M, oracle, torch_dev):
+@pytest.mark.parametrize("seed,n_nodes", [(0, 470), (1, 677)])
+def test_config5_full_size(M, oracle, torch_dev, seed, n_nodes):
     """The 4-way benchmark configuration at full size: 100 seeded flops,
     DCFR(1.5, 0, 2), 200 iterations, bubble factors from FP64 ICM of eight
-    stacks.  Flops 0 and 57 match the oracle after 10 iterations; after 200
-    the root strategy sums equal sum_t t^2 * range (every iteration reaches
-    the root infoset with the range itself)."""
-    game, flops, ranges, bf, st8, pay = M.config5(0, 100)
+    stacks.  Seed 0 draws two short stacks (5.6 and 3.7bb behind), whose
+    all-in calls leave side pots (470 nodes); seed 1's tree has 677 nodes.
+    Flops 0 and 57 match the oracle after 10 iterations; after 200 the root
+    strategy sums equal sum_t t^2 * range (every iteration reaches the root
+    infoset with the range itself)."""
+    game, flops, ranges, bf, st8, pay = M.config5(seed, 100)
     assert np.all(bf >= 1.0)
-    print("config5: 100 flops, DCFR 200 iterations", flush=True)
     mm = M.Multi(game, flops, ranges, bf)
-    mm.run(10)
     nodes, ni = mm.nodes, mm.n_info
+    print(f"config5 seed {seed}: {len(nodes)} nodes, {ni} infosets, 100 flops, DCFR 200 iterations", flush=True)
+    assert len(nodes) == n_nodes
+    mm.run(10)
     for f in (0, 57):
         h = oracle.FourWay(nodes, ni, flops[f], ranges[f], bf.astype(np.float64))
         h.iterate(10, scheme=1, alpha=1.5, beta=0.0, gamma=2.0)
@@ -188,7 +200,7 @@ def test_config5_full_size(M, oracle, torch_dev):
     mm.run(190)
     kt = mm.kernel_times()
     ms = sum(v[0] for v in kt.values()) / 190
-    print(f"config5 iteration: {ms:.3f} ms over 100 flops ({1e3 / ms:.1f} iterations/s); "
+    print(f"config5 seed {seed} iteration: {ms:.3f} ms over 100 flops ({1e3 / ms:.1f} iterations/s); "
           + ", ".join(f"{k} {v[0] / 190:.3f} ms" for k, v in kt.items()), flush=True)
     w = sum(float(t) ** 2 for t in range(1, 201))
     root = nodes[0]
