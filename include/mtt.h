@@ -263,7 +263,9 @@ typedef struct mtt_solver_cfg {
 } mtt_solver_cfg;
 typedef struct mtt_solver mtt_solver;
 
-/* ranges: 1326 floats each (global combo order), blocked combos ignored. */
+/* ranges: 1326 floats each (global combo order), blocked combos ignored;
+ * weights finite and >= 0, each range totalling less than 32768 (card
+ * sums are signed 2^48 fixed point), else MTT_ERR_ARG. */
 int mtt_solver_create(const mtt_hu_game* game, const float* range0, const float* range1,
                       const mtt_solver_cfg* cfg, mtt_solver** out);
 int mtt_solver_run(mtt_solver* h, int iters);
@@ -326,7 +328,8 @@ int mtt_solver_destroy(mtt_solver* h);
  * reference has no batch driver).  Every instance
  * is a game with n_board = 3, root_chance = 0, last_street = 0 (flop
  * betting, showdown over the 990 turn/river runouts) and its own board,
- * pot, stack and menu[0].  ranges: [n][2][1326] (global combo order).
+ * pot, stack and menu[0].  ranges: [n][2][1326] (global combo order;
+ * finite weights >= 0, each range totalling less than 32768).
  * Iterations run the per-instance street sweeps over all instances at once
  * and the showdowns as a batched GEMM of exact W - L counts (2 x BF16)
  * against BF16 hi+lo reach on MFMA; after the first call, iterations are
@@ -340,6 +343,14 @@ int mtt_batch_tree_info(mtt_batch* h, int instance, int* n_nodes, int* n_infoset
 /* Raw regrets / strategy sums of one instance in that instance's tree
  * layout [infoset][MTT_MAX_ACTIONS][1216] (flop-alive combo order). */
 int mtt_batch_get_state(mtt_batch* h, int instance, float* R_out, float* S_out, size_t n_floats);
+/* Average strategy of one instance in the reference layout (strategy sums
+ * normalised per decision and combo, uniform where they sum to 0). */
+int mtt_batch_get_strategy(mtt_batch* h, int instance, float* out, size_t n_floats);
+/* FP64 best-response exploitability of one instance's average strategy
+ * (new: the reference has no best response), out as for
+ * mtt_solver_exploitability_fp64: BR0, BR1, the profile's values, pair
+ * mass, exploitability in % of the instance's starting pot. */
+int mtt_batch_exploitability_fp64(mtt_batch* h, int instance, double* out);
 /* 0: launch kernels one by one (also while timing is on), 1: graph replay. */
 int mtt_batch_set_graph(mtt_batch* h, int on);
 /* Autotuner's choice for the showdown GEMM (hero tiles per workgroup) and

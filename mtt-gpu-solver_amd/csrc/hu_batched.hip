@@ -1305,9 +1305,11 @@ int upload_level(DeviceArena& A, hipStream_t st, LevelHost& H, LevelDev& L, int 
     if (L.ncolp == 0) L.ncolp = 32;
     L.maxterm = 0;
     L.maxcolt = 0;
+    L.maxslot = 0;
     for (int t = 0; t < L.ntmpl; ++t) {
         L.maxterm = std::max(L.maxterm, term0[t + 1] - term0[t]);
         L.maxcolt = std::max(L.maxcolt, col0[t + 1] - col0[t]);
+        L.maxslot = std::max(L.maxslot, slot0[t + 1] - slot0[t]);
         MTT_ARG(H.tnode0[t + 1] - H.tnode0[t] <= MAXN, "template has more than %d nodes", MAXN);
         MTT_ARG(H.tev0[t + 1] - H.tev0[t] <= MAXE, "template has more than %d events", MAXE);
         MTT_ARG(slot0[t + 1] - slot0[t] <= MAXSLOT, "template has more than %d action slots", MAXSLOT);
@@ -2176,6 +2178,7 @@ constexpr int FLOP_NT = 640;   // sweep workgroup size for flop instances: two c
 
 struct BatchImpl {
     std::vector<mtt_hu_game> games;
+    std::vector<float> ranges_h;     // [instance][2][1326], as given (best responses rebuild the games)
     mtt_solver_cfg cfg;
     std::vector<HuTree> trees;
     std::vector<int> ncol;           // showdown columns per instance
@@ -2308,6 +2311,19 @@ int batch_create(const mtt_hu_game* games, int n, const float* ranges, const mtt
     BatchImpl& B = *h;
     B.n = n;
     B.games.assign(games, games + n);
+    B.ranges_h.assign(ranges, ranges + (size_t)n * 2 * 1326);
+    for (int k = 0; k < n; ++k)
+        for (int p = 0; p < 2; ++p) {
+            double tot = 0.0;
+            for (int i = 0; i < 1326; ++i) {
+                const float v = ranges[((size_t)k * 2 + p) * 1326 + i];
+                MTT_ARG(std::isfinite(v) && v >= 0, "mtt_batch_create: instance %d range %d has a negative or "
+                        "non-finite weight", k, p);
+                tot += v;
+            }
+            MTT_ARG(tot < kMaxRangeTotal, "mtt_batch_create: instance %d range %d totals %g, at least %g", k, p, tot,
+                    kMaxRangeTotal);
+        }
     if (cfg) B.cfg = *cfg;
     else { memset(&B.cfg, 0, sizeof(B.cfg)); B.cfg.scheme = MTT_SCHEME_CFR_PLUS; }
     memset(&B.D, 0, sizeof(B.D));
@@ -2469,6 +2485,29 @@ int batch_state(BatchImpl& B, int k, float* R, float* S, size_t nf) {
     return MTT_OK;
 }
 
+// Average strategy of instance k in the reference layout: strategy sums
+// normalised over each decision's actions, uniform where they sum to 0.
+int batch_strategy(BatchImpl& B, int k, float* out, size_t nf) {
+    const HuTree& tr = B.trees[k];
+    const size_t need = (size_t)tr.n_infosets * MTT_MAX_ACTIONS * NPF;
+    MTT_ARG(nf >= need, "mtt_batch_get_strategy: buffer of %zu floats, need %zu", nf, need);
+    std::vector<float> S(need);
+    int rc = batch_state(B, k, nullptr, S.data(), need);
+    if (rc) return rc;
+    memset(out, 0, need * 4);
+    for (const mtt_flat_node& x : tr.flat) {
+        if (x.type != MTT_NODE_DECISION) continue;
+        const size_t base = (size_t)x.infoset_offset * MTT_MAX_ACTIONS * NPF;
+        for (int i = 0; i < NPF; ++i) {
+            float sum = 0.0f;
+            for (int a = 0; a < x.num_actions; ++a) sum += S[base + (size_t)a * NPF + i];
+            for (int a = 0; a < x.num_actions; ++a)
+                out[base + (size_t)a * NPF + i] = sum > 0.0f ? S[base + (size_t)a * NPF + i] / sum : 1.0f / x.num_actions;
+        }
+    }
+    return MTT_OK;
+}
+
 void batch_destroy(BatchImpl* B) {
     if (!B) return;
     (void)hipStreamSynchronize(B->stream);
@@ -2566,6 +2605,25 @@ int mtt_batch_get_state(mtt_batch* h, int instance, float* R_out, float* S_out, 
     using namespace mtt;
     MTT_ARG(h && instance >= 0 && instance < h->impl->n, "mtt_batch_get_state: bad arguments");
     return batch_state(*h->impl, instance, R_out, S_out, n_floats);
+}
+
+int mtt_batch_get_strategy(mtt_batch* h, int instance, float* out, size_t n_floats) {
+    using namespace mtt;
+    MTT_ARG(h && out && instance >= 0 && instance < h->impl->n, "mtt_batch_get_strategy: bad arguments");
+    return batch_strategy(*h->impl, instance, out, n_floats);
+}
+
+int mtt_batch_exploitability_fp64(mtt_batch* h, int instance, double* out) {
+    using namespace mtt;
+    MTT_ARG(h && out && instance >= 0 && instance < h->impl->n, "mtt_batch_exploitability_fp64: bad arguments");
+    BatchImpl& B = *h->impl;
+    const HuTree& tr = B.trees[instance];
+    const size_t need = (size_t)tr.n_infosets * MTT_MAX_ACTIONS * NPF;
+    std::vector<float> st(need);
+    int rc = batch_strategy(B, instance, st.data(), need);
+    if (rc) return rc;
+    const float* r = B.ranges_h.data() + (size_t)instance * 2 * 1326;
+    return solver_exploitability_of(B.games[instance], r, r + 1326, st.data(), need, tr.n_infosets, out);
 }
 
 int mtt_batch_set_graph(mtt_batch* h, int on) {

@@ -33,6 +33,9 @@ constexpr int kAliveFlop = 1176;   // combos avoiding a flop
 constexpr int kPadFlop = 1216;     // reference padding, 19 x 64
 constexpr int kAliveTurn = 1128;   // combos avoiding 4 board cards
 constexpr int kAliveRiver = 1081;  // combos avoiding 5 board cards
+// A player's range may total at most this much: per-card reach sums are
+// kept in signed 64-bit fixed point with 48 fraction bits.
+constexpr double kMaxRangeTotal = 32768.0;
 
 MTT_HD int hibit(uint32_t x) { return 31 - __builtin_clz(x); }
 

@@ -113,4 +113,9 @@ void batched_destroy(BatchedState* b);
 // (normalised, reference layout) in d_strat.
 int br_exploitability(SolverImpl& s, const float* d_strat, double* out6);
 
+// The same for a strategy held in host memory (reference layout, normalised),
+// on a plain-mode handle created for the strategy's game (solver.hip).
+int solver_exploitability_of(const mtt_hu_game& game, const float* r0_1326, const float* r1_1326,
+                             const float* host_strat, size_t n_floats, int n_infosets, double* out6);
+
 }  // namespace mtt

@@ -302,11 +302,16 @@ static int solver_create(const mtt_hu_game* game, const float* r0, const float* 
     for (int p = 0; p < 2; ++p) {
         const float* src = p == 0 ? r0 : r1;
         s.range[p].assign(s.n_alive, 0.0f);
+        double tot = 0.0;
         for (int i = 0; i < s.n_alive; ++i) {
             float v = src[s.gidx[i]];
             MTT_ARG(std::isfinite(v) && v >= 0, "mtt_solver_create: range %d has a negative or non-finite weight", p);
             s.range[p][i] = v;
+            tot += v;
         }
+        // the batched path sums reaches per card in signed 2^48 fixed point
+        MTT_ARG(tot < kMaxRangeTotal, "mtt_solver_create: range %d totals %g, at least %g (normalise the range)", p,
+                tot, kMaxRangeTotal);
     }
     // pair mass M = sum_i r0(i) * (tot1 - R1[c1] - R1[c2] + r1(i))
     double tot1 = 0, card1[52] = {0};
@@ -357,6 +362,38 @@ static int solver_create(const mtt_hu_game* game, const float* r0, const float* 
 }  // namespace mtt
 
 using namespace mtt;
+
+namespace mtt {
+
+int solver_exploitability_of(const mtt_hu_game& game, const float* r0_1326, const float* r1_1326,
+                             const float* host_strat, size_t n_floats, int n_infosets, double* out6) {
+    mtt_solver_cfg cfg;
+    memset(&cfg, 0, sizeof(cfg));
+    cfg.mode = MTT_MODE_PLAIN_FP32;
+    cfg.scheme = MTT_SCHEME_CFR_PLUS;
+    mtt_solver* h = nullptr;
+    int rc = mtt_solver_create(&game, r0_1326, r1_1326, &cfg, &h);
+    if (rc) return rc;
+    SolverImpl& si = h->impl;
+    const size_t need = (size_t)si.tree.n_infosets * AS * NP;
+    if (si.tree.n_infosets != n_infosets || n_floats < need) {
+        mtt_solver_destroy(h);
+        set_error("exploitability: the strategy has %d infosets, the game's tree %d", n_infosets, si.tree.n_infosets);
+        return MTT_ERR_ARG;
+    }
+    DeviceArena tmp;
+    float* d = nullptr;
+    rc = tmp.alloc(need, &d);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(d, host_strat, need * sizeof(float), hipMemcpyHostToDevice, si.stream);
+        if (e != hipSuccess) rc = hip_fail(e, "exploitability: strategy upload", __FILE__, __LINE__);
+    }
+    rc = rc ? rc : br_exploitability(si, d, out6);
+    mtt_solver_destroy(h);
+    return rc;
+}
+
+}  // namespace mtt
 
 extern "C" {
 

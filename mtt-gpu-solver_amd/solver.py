@@ -263,6 +263,8 @@ for name, args in {
     "mtt_batch_iterations": [_P],
     "mtt_batch_tree_info": [_P, _I, _P, _P],
     "mtt_batch_get_state": [_P, _I, _P, _P, ctypes.c_size_t],
+    "mtt_batch_get_strategy": [_P, _I, _P, ctypes.c_size_t],
+    "mtt_batch_exploitability_fp64": [_P, _I, _P],
     "mtt_batch_set_graph": [_P, _I],
     "mtt_batch_tuning": [_P, _P, _P],
     "mtt_batch_timing": [_P, _I],
@@ -326,6 +328,25 @@ class Batch:
         S = np.zeros(n, np.float32)
         mtt.call("mtt_batch_get_state", self.h, k, R.ctypes.data, S.ctypes.data, n)
         return R, S
+
+    @property
+    def iterations(self):
+        return mtt.lib().mtt_batch_iterations(self.h)
+
+    def strategy(self, k):
+        """Average strategy of instance k, reference layout [infoset][4][1216]."""
+        _, ni = self.tree_info(k)
+        n = ni * mtt.MAX_ACTIONS * 1216
+        out = np.zeros(n, np.float32)
+        mtt.call("mtt_batch_get_strategy", self.h, k, out.ctypes.data, n)
+        return out
+
+    def exploitability(self, k):
+        """FP64 best response of instance k: BR0, BR1, profile values, pair
+        mass, exploitability in % of the instance's starting pot."""
+        out = np.zeros(6, np.float64)
+        mtt.call("mtt_batch_exploitability_fp64", self.h, k, out.ctypes.data)
+        return out
 
     def tuning(self):
         ht = ctypes.c_int()
