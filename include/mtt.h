@@ -217,30 +217,36 @@ int mtt_seeded_inputs(uint64_t seed, int n_board, int dist, float alpha, uint8_t
                       float* range0_out, float* range1_out);
 
 /* ------------------------------------------- HU tree-sweep kernels -- */
-/* Per-depth kernels with the reference's argument meaning, for a host loop
- * that drives the sweeps itself.  Device pointers; layouts:
- * regrets/strategy/strategy sums [infoset][MTT_MAX_ACTIONS][1216],
- * reach/cfv [node][2][1216], matrices [n][1216][1216] (E[j][h] = equity of
- * hero h against villain j, 0 where blocked).  n_alive = live combos of the
- * root board (<= 1216).  stream may be NULL (the null stream). */
+/* Per-depth kernels with the reference's argument meaning and order, for a
+ * host loop that drives the sweeps itself; arguments the reference's
+ * kernels do not take follow its list.  Device pointers; layouts:
+ * regrets/strategy/strategy sums [infoset][action_stride][1216] (the
+ * reference's MAX_ACTIONS = 3, honest_multistreet_solver.cu:12; 4 for bet
+ * menus with four actions; decisions with more actions than the stride are
+ * left untouched), reach/cfv [node][2][1216], matrices [n][1216][1216]
+ * (E[j][h] = equity of hero h against villain j, 0 where blocked).
+ * n_alive = live combos of the root board (<= 1216).  stream may be NULL
+ * (the null stream). */
 /* Regret matching + reach propagation for the decision nodes of one depth;
  * replaces kernel_forward_sweep, honest_multistreet_solver.cu:88-126. */
 int mtt_hu_forward_sweep(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node* d_nodes,
                          const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_cumulative_regrets,
-                         float* d_current_strategy, float* d_reach_probs, int n_alive, hipStream_t stream);
+                         float* d_current_strategy, float* d_reach_probs, int n_alive, int action_stride,
+                         hipStream_t stream);
 /* Showdown and fold values of terminal nodes (card removal, exact chance
  * normalisation); replaces kernel_evaluate_terminals,
- * honest_multistreet_solver.cu:128-196. */
+ * honest_multistreet_solver.cu:128-196 (same first six arguments). */
 int mtt_hu_evaluate_terminals(const int* d_terminal_node_indices, int num_terminals, const mtt_flat_node* d_nodes,
-                              const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_reach_probs,
-                              const float* d_turn_matrices, float* d_cfvs, int n_alive, hipStream_t stream);
+                              const float* d_reach_probs, const float* d_turn_matrices, float* d_cfvs,
+                              const uint8_t* d_deck, const mtt_alive_combo* d_combos, int n_alive, hipStream_t stream);
 /* CFR+ backward pass for one depth: node values, regrets floored at 0,
  * strategy sums weighted by iter_weight; replaces kernel_backward_sweep,
  * honest_multistreet_solver.cu:198-249. */
 int mtt_hu_backward_sweep(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node* d_nodes,
                           const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_current_strategy,
                           const float* d_reach_probs, float* d_cfvs, float* d_cumulative_regrets,
-                          float* d_cumulative_strategy_sums, float iter_weight, int n_alive, hipStream_t stream);
+                          float* d_cumulative_strategy_sums, float iter_weight, int n_alive, int action_stride,
+                          hipStream_t stream);
 /* Same pass with DCFR discounting fused: R <- (R + r) * (R + r > 0 ? w_pos :
  * w_neg), S += w_strat * reach * sigma; the reference's DCFR step is
  * kernel_dcfr_step, preflop_grid_generator.cu:387-410. */
@@ -248,7 +254,7 @@ int mtt_hu_backward_sweep_dcfr(const int* d_depth_node_indices, int num_nodes_at
                                const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_current_strategy,
                                const float* d_reach_probs, float* d_cfvs, float* d_cumulative_regrets,
                                float* d_cumulative_strategy_sums, float w_pos, float w_neg, float w_strat, int n_alive,
-                               hipStream_t stream);
+                               int action_stride, hipStream_t stream);
 
 /* -------------------------------------------------------- HU solver -- */
 /* Solve entry points (new: the reference's HU main is a stub,
@@ -381,8 +387,9 @@ typedef struct mtt_4way_game {
     float stack[4];       /* chips behind at the start of the street */
     mtt_street_menu menu;
 } mtt_4way_game;
-/* Betting tree (seat order, bets capped at the smallest stack in the hand,
- * breadth-first layout, one infoset per decision node). */
+/* Betting tree (seat order; bets capped at the largest stack that can call
+ * them, sizes at or above that cap become all-in, calls all-in for less
+ * leave side pots; breadth-first layout, one infoset per decision node). */
 int mtt_4way_tree_size(const mtt_4way_game* game, int* n_nodes, int* n_infosets);
 int mtt_4way_tree_build(const mtt_4way_game* game, mtt_flat_node4* nodes, int capacity, int* n_nodes);
 
@@ -407,18 +414,31 @@ int mtt_4way_forward_sweep(const int* d_depth_node_indices, int num_nodes_at_dep
                            const float* d_cumulative_regrets, float* d_current_strategy, float* d_reach_probs,
                            hipStream_t stream);
 /* kernel_4way_evaluate_terminals, multiway_4way_engine.cu:85-217, with
- * card removal against the hero's hand and split pots for ties:
- * fold: pay * prod_{o != p} W_o; showdown, active p:
- *   inact * (g * prod F_o + sum_{tied T} share_T * prod_T E_o prod_rest F_o
- *            + l * (prod W_o - prod G_o)),
- * F/G/E/W = opponent reach strictly weaker / weaker-or-tied / tied / total,
- * each without combos that share a card with the hero;
- * g = pot - c_p, l = -bf[p][p] c_p, share = pot/(|T|+1) - c_p (scaled by
- * bf[p][p] when negative); inactive p: -bf[p][p] c_p prod W_o.
- * d_board: device copy of mtt_board4; d_bubble_factors 4x4 floats. */
+ * card removal against the hero's hand, split pots for ties and side pots
+ * for unequal commitments:
+ * fold: pay * prod_{o != p} W_o; showdown, player p still in: the pot split
+ * into layers at the distinct commitments of the players still in, p taking
+ * each layer it reaches unless an opponent reaching it is stronger (shared
+ * with the tied ones), net chips scaled by bf[p][p] when negative, summed
+ * over the 3^k weaker/tied/stronger outcomes of its k opponents still in,
+ * each weighted by prod F_o / (G_o - F_o) / (W_o - G_o), times the folded
+ * seats' prod W_o; folded p: -bf[p][p] c_p prod W_o.
+ * F/G/W = opponent reach strictly weaker / weaker-or-tied / total, each
+ * without combos that share a card with the hero.
+ * Same first seven arguments as the reference: d_combo_rank_pos (device,
+ * 1216 uint16) is the flop's ascending strength order, checked against the
+ * evaluator's (MTT_ERR_ARG if it is not one); the board tables are built
+ * from `flop` (host, 3 cards).  d_bubble_factors 4x4 floats.  Synchronous. */
 int mtt_4way_evaluate_terminals(const int* d_terminal_node_indices, int num_terminals, const mtt_flat_node4* d_nodes,
-                                const float* d_reach_probs, const mtt_board4* d_board, const float* d_bubble_factors,
-                                float* d_cfvs, hipStream_t stream);
+                                const float* d_reach_probs, const uint16_t* d_combo_rank_pos,
+                                const float* d_bubble_factors, float* d_cfvs, const uint8_t* flop,
+                                hipStream_t stream);
+/* The same with the board tables already on the device (d_board: a device
+ * copy of mtt_board4_build's output); asynchronous on `stream`. */
+int mtt_4way_evaluate_terminals_board(const int* d_terminal_node_indices, int num_terminals,
+                                      const mtt_flat_node4* d_nodes, const float* d_reach_probs,
+                                      const mtt_board4* d_board, const float* d_bubble_factors, float* d_cfvs,
+                                      hipStream_t stream);
 /* kernel_4way_backward_cfr_plus, multiway_4way_engine.cu:219-257. */
 int mtt_4way_backward_cfr_plus(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node4* d_nodes,
                                const float* d_current_strategy, const float* d_reach_probs, float* d_cfvs,

@@ -59,10 +59,11 @@ struct PlainTerminalPlan {
 
 int launch_hu_forward(const int* d_nodes_at_depth, int num, const mtt_flat_node* d_nodes, const uint8_t* d_deck,
                       const mtt_alive_combo* d_combos, const float* d_R, float* d_sigma, float* d_reach, int n_alive,
-                      hipStream_t s);
+                      hipStream_t s, int action_stride = MTT_MAX_ACTIONS);
 int launch_hu_backward(const int* d_nodes_at_depth, int num, const mtt_flat_node* d_nodes, const uint8_t* d_deck,
                        const mtt_alive_combo* d_combos, const float* d_sigma, const float* d_reach, float* d_cfv,
-                       float* d_R, float* d_S, float w_pos, float w_neg, float w_strat, int n_alive, hipStream_t s);
+                       float* d_R, float* d_S, float w_pos, float w_neg, float w_strat, int n_alive, hipStream_t s,
+                       int action_stride = MTT_MAX_ACTIONS);
 int launch_hu_terminals_ref(const int* d_terms, int num, const mtt_flat_node* d_nodes, const uint8_t* d_deck,
                             const mtt_alive_combo* d_combos, const float* d_reach, const float* d_mats, float* d_cfv,
                             int n_alive, hipStream_t s);

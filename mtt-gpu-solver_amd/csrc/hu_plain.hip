@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void k_hu_forward(const int* __restrict__ dept
                                                     const uint8_t* __restrict__ deck,
                                                     const mtt_alive_combo* __restrict__ combos,
                                                     const float* __restrict__ R, float* __restrict__ sigma,
-                                                    float* __restrict__ reach, int n_alive) {
+                                                    float* __restrict__ reach, int n_alive, int as) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= NP) return;
     uint64_t mask = tid < n_alive ? combos[tid].mask : ~0ull;
@@ -54,10 +54,10 @@ __global__ __launch_bounds__(256) void k_hu_forward(const int* __restrict__ dept
                 reach[(ch * 2 + 0) * NP + tid] = dead ? 0.0f : r0;
                 reach[(ch * 2 + 1) * NP + tid] = dead ? 0.0f : r1;
             }
-        } else if (n.type == MTT_NODE_DECISION) {
+        } else if (n.type == MTT_NODE_DECISION && n.num_actions <= as) {
             int na = n.num_actions;
-            const float* Rn = R + (size_t)n.infoset_offset * AS * NP + tid;
-            float* Sg = sigma + (size_t)n.infoset_offset * AS * NP + tid;
+            const float* Rn = R + (size_t)n.infoset_offset * as * NP + tid;
+            float* Sg = sigma + (size_t)n.infoset_offset * as * NP + tid;
             float pos[AS], sum = 0.0f;
 #pragma unroll
             for (int a = 0; a < AS; ++a) {
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256) void k_hu_backward(const int* __restrict__ dep
                                                      const float* __restrict__ sigma,
                                                      const float* __restrict__ reach, float* __restrict__ cfv,
                                                      float* __restrict__ R, float* __restrict__ S, float w_pos,
-                                                     float w_neg, float w_strat, int n_alive) {
+                                                     float w_neg, float w_strat, int n_alive, int as) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= NP) return;
     uint64_t mask = tid < n_alive ? combos[tid].mask : ~0ull;
@@ -351,9 +351,9 @@ __global__ __launch_bounds__(256) void k_hu_backward(const int* __restrict__ dep
             bool live = tid < n_alive;
             cfv[((size_t)id * 2 + 0) * NP + tid] = live ? s0 * inv : 0.0f;
             cfv[((size_t)id * 2 + 1) * NP + tid] = live ? s1 * inv : 0.0f;
-        } else if (n.type == MTT_NODE_DECISION) {
+        } else if (n.type == MTT_NODE_DECISION && n.num_actions <= as) {
             int na = n.num_actions, p = n.player, o = 1 - p;
-            size_t base = (size_t)n.infoset_offset * AS * NP + tid;
+            size_t base = (size_t)n.infoset_offset * as * NP + tid;
             float av[AS], node_v = 0.0f, opp = 0.0f;
 #pragma unroll
             for (int a = 0; a < AS; ++a) {
@@ -385,20 +385,21 @@ static inline dim3 node_grid(int num) { return dim3((NP + 255) / 256, num < 6553
 
 int launch_hu_forward(const int* d_nodes_at_depth, int num, const mtt_flat_node* d_nodes, const uint8_t* d_deck,
                       const mtt_alive_combo* d_combos, const float* d_R, float* d_sigma, float* d_reach, int n_alive,
-                      hipStream_t s) {
+                      hipStream_t s, int as) {
     if (num <= 0) return MTT_OK;
     hipLaunchKernelGGL(k_hu_forward, node_grid(num), dim3(256), 0, s, d_nodes_at_depth, num, d_nodes, d_deck,
-                       d_combos, d_R, d_sigma, d_reach, n_alive);
+                       d_combos, d_R, d_sigma, d_reach, n_alive, as);
     MTT_HIP(hipGetLastError());
     return MTT_OK;
 }
 
 int launch_hu_backward(const int* d_nodes_at_depth, int num, const mtt_flat_node* d_nodes, const uint8_t* d_deck,
                        const mtt_alive_combo* d_combos, const float* d_sigma, const float* d_reach, float* d_cfv,
-                       float* d_R, float* d_S, float w_pos, float w_neg, float w_strat, int n_alive, hipStream_t s) {
+                       float* d_R, float* d_S, float w_pos, float w_neg, float w_strat, int n_alive, hipStream_t s,
+                       int as) {
     if (num <= 0) return MTT_OK;
     hipLaunchKernelGGL(k_hu_backward, node_grid(num), dim3(256), 0, s, d_nodes_at_depth, num, d_nodes, d_deck,
-                       d_combos, d_sigma, d_reach, d_cfv, d_R, d_S, w_pos, w_neg, w_strat, n_alive);
+                       d_combos, d_sigma, d_reach, d_cfv, d_R, d_S, w_pos, w_neg, w_strat, n_alive, as);
     MTT_HIP(hipGetLastError());
     return MTT_OK;
 }
@@ -486,19 +487,22 @@ extern "C" {
 
 int mtt_hu_forward_sweep(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node* d_nodes,
                          const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_cumulative_regrets,
-                         float* d_current_strategy, float* d_reach_probs, int n_alive, hipStream_t stream) {
+                         float* d_current_strategy, float* d_reach_probs, int n_alive, int action_stride,
+                         hipStream_t stream) {
     MTT_ARG(num_nodes_at_depth >= 0, "mtt_hu_forward_sweep: negative node count");
     MTT_ARG(n_alive > 0 && n_alive <= NP, "mtt_hu_forward_sweep: n_alive %d", n_alive);
+    MTT_ARG(action_stride >= 2 && action_stride <= AS, "mtt_hu_forward_sweep: action stride %d outside 2..%d",
+            action_stride, AS);
     if (!num_nodes_at_depth) return MTT_OK;
     MTT_ARG(d_depth_node_indices && d_nodes && d_deck && d_combos && d_cumulative_regrets && d_current_strategy &&
                 d_reach_probs, "mtt_hu_forward_sweep: null pointer");
     return launch_hu_forward(d_depth_node_indices, num_nodes_at_depth, d_nodes, d_deck, d_combos,
-                             d_cumulative_regrets, d_current_strategy, d_reach_probs, n_alive, stream);
+                             d_cumulative_regrets, d_current_strategy, d_reach_probs, n_alive, stream, action_stride);
 }
 
 int mtt_hu_evaluate_terminals(const int* d_terminal_node_indices, int num_terminals, const mtt_flat_node* d_nodes,
-                              const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_reach_probs,
-                              const float* d_turn_matrices, float* d_cfvs, int n_alive, hipStream_t stream) {
+                              const float* d_reach_probs, const float* d_turn_matrices, float* d_cfvs,
+                              const uint8_t* d_deck, const mtt_alive_combo* d_combos, int n_alive, hipStream_t stream) {
     MTT_ARG(num_terminals >= 0, "mtt_hu_evaluate_terminals: negative count");
     MTT_ARG(n_alive > 0 && n_alive <= NP, "mtt_hu_evaluate_terminals: n_alive %d", n_alive);
     if (!num_terminals) return MTT_OK;
@@ -511,30 +515,35 @@ int mtt_hu_evaluate_terminals(const int* d_terminal_node_indices, int num_termin
 int mtt_hu_backward_sweep(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node* d_nodes,
                           const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_current_strategy,
                           const float* d_reach_probs, float* d_cfvs, float* d_cumulative_regrets,
-                          float* d_cumulative_strategy_sums, float iter_weight, int n_alive, hipStream_t stream) {
+                          float* d_cumulative_strategy_sums, float iter_weight, int n_alive, int action_stride,
+                          hipStream_t stream) {
     MTT_ARG(num_nodes_at_depth >= 0, "mtt_hu_backward_sweep: negative node count");
     MTT_ARG(n_alive > 0 && n_alive <= NP, "mtt_hu_backward_sweep: n_alive %d", n_alive);
+    MTT_ARG(action_stride >= 2 && action_stride <= AS, "mtt_hu_backward_sweep: action stride %d outside 2..%d",
+            action_stride, AS);
     if (!num_nodes_at_depth) return MTT_OK;
     MTT_ARG(d_depth_node_indices && d_nodes && d_deck && d_combos && d_current_strategy && d_reach_probs && d_cfvs &&
                 d_cumulative_regrets && d_cumulative_strategy_sums, "mtt_hu_backward_sweep: null pointer");
     return launch_hu_backward(d_depth_node_indices, num_nodes_at_depth, d_nodes, d_deck, d_combos, d_current_strategy,
                               d_reach_probs, d_cfvs, d_cumulative_regrets, d_cumulative_strategy_sums, 1.0f, 0.0f,
-                              iter_weight, n_alive, stream);
+                              iter_weight, n_alive, stream, action_stride);
 }
 
 int mtt_hu_backward_sweep_dcfr(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node* d_nodes,
                                const uint8_t* d_deck, const mtt_alive_combo* d_combos, const float* d_current_strategy,
                                const float* d_reach_probs, float* d_cfvs, float* d_cumulative_regrets,
                                float* d_cumulative_strategy_sums, float w_pos, float w_neg, float w_strat, int n_alive,
-                               hipStream_t stream) {
+                               int action_stride, hipStream_t stream) {
     MTT_ARG(num_nodes_at_depth >= 0, "mtt_hu_backward_sweep_dcfr: negative node count");
     MTT_ARG(n_alive > 0 && n_alive <= NP, "mtt_hu_backward_sweep_dcfr: n_alive %d", n_alive);
+    MTT_ARG(action_stride >= 2 && action_stride <= AS, "mtt_hu_backward_sweep_dcfr: action stride %d outside 2..%d",
+            action_stride, AS);
     if (!num_nodes_at_depth) return MTT_OK;
     MTT_ARG(d_depth_node_indices && d_nodes && d_deck && d_combos && d_current_strategy && d_reach_probs && d_cfvs &&
                 d_cumulative_regrets && d_cumulative_strategy_sums, "mtt_hu_backward_sweep_dcfr: null pointer");
     return launch_hu_backward(d_depth_node_indices, num_nodes_at_depth, d_nodes, d_deck, d_combos, d_current_strategy,
                               d_reach_probs, d_cfvs, d_cumulative_regrets, d_cumulative_strategy_sums, w_pos, w_neg,
-                              w_strat, n_alive, stream);
+                              w_strat, n_alive, stream, action_stride);
 }
 
 }  // extern "C"

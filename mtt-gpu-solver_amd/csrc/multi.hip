@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "solver.h"
@@ -543,15 +544,55 @@ int mtt_4way_forward_sweep(const int* d_depth_node_indices, int num_nodes_at_dep
                                d_current_strategy, d_reach_probs, 1, 0, 0, stream);
 }
 
-int mtt_4way_evaluate_terminals(const int* d_terminal_node_indices, int num_terminals, const mtt_flat_node4* d_nodes,
-                                const float* d_reach_probs, const mtt_board4* d_board, const float* d_bubble_factors,
-                                float* d_cfvs, hipStream_t stream) {
-    MTT_ARG(num_terminals >= 0, "mtt_4way_evaluate_terminals: negative count");
+int mtt_4way_evaluate_terminals_board(const int* d_terminal_node_indices, int num_terminals,
+                                      const mtt_flat_node4* d_nodes, const float* d_reach_probs,
+                                      const mtt_board4* d_board, const float* d_bubble_factors, float* d_cfvs,
+                                      hipStream_t stream) {
+    MTT_ARG(num_terminals >= 0, "mtt_4way_evaluate_terminals_board: negative count");
     if (!num_terminals) return MTT_OK;
     MTT_ARG(d_terminal_node_indices && d_nodes && d_reach_probs && d_board && d_bubble_factors && d_cfvs,
-            "mtt_4way_evaluate_terminals: null pointer");
+            "mtt_4way_evaluate_terminals_board: null pointer");
     return launch_4way_terminals(d_terminal_node_indices, num_terminals, d_nodes, d_reach_probs, d_board,
                                  d_bubble_factors, d_cfvs, 1, 0, stream);
+}
+
+int mtt_4way_evaluate_terminals(const int* d_terminal_node_indices, int num_terminals, const mtt_flat_node4* d_nodes,
+                                const float* d_reach_probs, const uint16_t* d_combo_rank_pos,
+                                const float* d_bubble_factors, float* d_cfvs, const uint8_t* flop,
+                                hipStream_t stream) {
+    MTT_ARG(num_terminals >= 0, "mtt_4way_evaluate_terminals: negative count");
+    if (!num_terminals) return MTT_OK;
+    MTT_ARG(d_terminal_node_indices && d_nodes && d_reach_probs && d_combo_rank_pos && d_bubble_factors && d_cfvs &&
+                flop, "mtt_4way_evaluate_terminals: null pointer");
+    std::unique_ptr<mtt_board4> B(new (std::nothrow) mtt_board4);
+    MTT_ARG(B, "mtt_4way_evaluate_terminals: out of host memory");
+    int rc = mtt_board4_build(flop, B.get());
+    if (rc) return rc;
+    // The caller's strength order must agree with the flop's: ascending
+    // positions, combos of equal strength (one tie group) adjacent.
+    uint16_t pos[NP];
+    MTT_HIP(hipMemcpyAsync(pos, d_combo_rank_pos, sizeof(pos), hipMemcpyDeviceToHost, stream));
+    MTT_HIP(hipStreamSynchronize(stream));
+    std::vector<uint8_t> seen(NA, 0);
+    for (int i = 0; i < NA; ++i) {
+        MTT_ARG(pos[i] < NA && !seen[pos[i]], "mtt_4way_evaluate_terminals: combo_rank_pos is not a permutation of "
+                "0..%d (combo %d)", NA - 1, i);
+        seen[pos[i]] = 1;
+        MTT_ARG(pos[i] >= B->grp_lo[i] && pos[i] < B->grp_hi[i],
+                "mtt_4way_evaluate_terminals: combo_rank_pos of combo %d (%d) is not the flop's strength order "
+                "(positions %d..%d)", i, (int)pos[i], (int)B->grp_lo[i], (int)B->grp_hi[i] - 1);
+    }
+    mtt_board4* d_board = nullptr;
+    MTT_HIP(hipMalloc(&d_board, sizeof(mtt_board4)));
+    hipError_t e = hipMemcpyAsync(d_board, B.get(), sizeof(mtt_board4), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+        rc = launch_4way_terminals(d_terminal_node_indices, num_terminals, d_nodes, d_reach_probs, d_board,
+                                   d_bubble_factors, d_cfvs, 1, 0, stream);
+    hipError_t e2 = hipStreamSynchronize(stream);   // the board buffer is freed below
+    hipFree(d_board);
+    if (e != hipSuccess) return hip_fail(e, "mtt_4way_evaluate_terminals: board upload", __FILE__, __LINE__);
+    if (e2 != hipSuccess) return hip_fail(e2, "mtt_4way_evaluate_terminals", __FILE__, __LINE__);
+    return rc;
 }
 
 int mtt_4way_backward_cfr_plus(const int* d_depth_node_indices, int num_nodes_at_depth, const mtt_flat_node4* d_nodes,

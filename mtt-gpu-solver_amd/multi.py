@@ -37,7 +37,8 @@ for name, args in {
     "mtt_4way_tree_build": [_P, _P, _I, _P],
     "mtt_board4_build": [_P, _P],
     "mtt_4way_forward_sweep": [_P, _I, _P, _P, _P, _P, _P],
-    "mtt_4way_evaluate_terminals": [_P, _I, _P, _P, _P, _P, _P, _P],
+    "mtt_4way_evaluate_terminals": [_P, _I, _P, _P, _P, _P, _P, _P, _P],
+    "mtt_4way_evaluate_terminals_board": [_P, _I, _P, _P, _P, _P, _P, _P],
     "mtt_4way_backward_cfr_plus": [_P, _I, _P, _P, _P, _P, _P, _P, _F, _P],
     "mtt_4way_backward_dcfr": [_P, _I, _P, _P, _P, _P, _P, _P, _F, _F, _F, _P],
     "mtt_multi_create": [_P, _I, _P, _P, _P, _P, _P],

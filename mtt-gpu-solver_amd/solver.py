@@ -62,11 +62,11 @@ for name, args in {
     "mtt_solver_timing": [_P, _I],
     "mtt_solver_kernel_times": [_P, _I, ctypes.c_char_p, _I, _P, _P],
     "mtt_solver_destroy": [_P],
-    "mtt_hu_forward_sweep": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P],
+    "mtt_hu_forward_sweep": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P],
     "mtt_hu_evaluate_terminals": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P],
-    "mtt_hu_backward_sweep": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_float, _I, _P],
+    "mtt_hu_backward_sweep": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_float, _I, _I, _P],
     "mtt_hu_backward_sweep_dcfr": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float,
-                                   ctypes.c_float, _I, _P],
+                                   ctypes.c_float, _I, _I, _P],
 }.items():
     mtt.register(name, args)
 

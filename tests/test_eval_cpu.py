@@ -177,3 +177,20 @@ def test_flop_matrix_entry_quarter_tie(oracle):
     # aces against a hand drawing dead to nothing but runner-runner: between 0 and 1
     e = oracle.flop_matrix_entry(flop, (c("Ac"), c("Ad")), (c("3c"), c("4d")))
     assert 0.5 < e < 1.0
+
+
+def test_flop_checksum_fixture_matches_oracle(oracle):
+    """The committed per-flop checksums (tests/golden/flop_checksums.npz)
+    are the oracle's: recomputed here for two flops."""
+    import importlib.util
+    from conftest import GOLDEN
+    spec = importlib.util.spec_from_file_location("make_flop_checksums", os.path.join(GOLDEN, "make_flop_checksums.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    fx = np.load(os.path.join(GOLDEN, "flop_checksums.npz"))
+    assert fx["checksums"].shape == (1755, 4)
+    assert np.array_equal(fx["flops"], oracle.canonical_flops()[:, :3])
+    w1, w2 = mk.splitmix_weights(1), mk.splitmix_weights(2)
+    for f in (0, 1333):
+        W, T = oracle.flop_counts(fx["flops"][f])
+        assert np.array_equal(mk.checksums(W, T, w1, w2), fx["checksums"][f]), f

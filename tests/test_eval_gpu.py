@@ -57,6 +57,59 @@ def test_gpu_flop_counts_bitexact(mtt, oracle, torch_dev, flop_idx):
     assert np.array_equal(T, T0)
 
 
+def test_gpu_flop_counts_all_1755(mtt, torch_dev):
+    """Config 2 at full size: exhaustive W/T counts of all 1,755 canonical
+    flops (global 1326 layout) against the oracle's per-flop checksums
+    (sum W, sum T and splitmix-weighted sums, exact int64 arithmetic:
+    tests/golden/flop_checksums.npz), plus the invariants on every entry:
+    W + W^T + T = 990 on unblocked pairs of alive combos, T symmetric,
+    zero elsewhere."""
+    import importlib.util
+    import time
+    import torch
+    spec = importlib.util.spec_from_file_location("make_flop_checksums", os.path.join(GOLDEN, "make_flop_checksums.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    fx = np.load(os.path.join(GOLDEN, "flop_checksums.npz"))
+    flops, want = np.ascontiguousarray(fx["flops"]), fx["checksums"]
+    assert flops.shape == (1755, 3)
+    canon = np.zeros((1755, 4), np.uint8)
+    assert mtt.lib().mtt_canonical_flops(canon.ctypes.data) == 1755
+    assert np.array_equal(canon[:, :3], flops)
+    w1 = torch.from_numpy(mk.splitmix_weights(1)).to(torch_dev)
+    w2 = torch.from_numpy(mk.splitmix_weights(2)).to(torch_dev)
+    pairs = np.array(list(itertools.combinations(range(52), 2)), np.int64)
+    cm = torch.from_numpy((np.int64(1) << pairs[:, 0]) | (np.int64(1) << pairs[:, 1])).to(torch_dev)
+    disjoint = (cm[:, None] & cm[None, :]) == 0
+    B = 64
+    W = torch.empty((B, 1326, 1326), dtype=torch.int32, device=torch_dev)
+    T = torch.empty_like(W)
+    t_kernels = 0.0
+    for f0 in range(0, 1755, B):
+        n = min(B, 1755 - f0)
+        print(f"config2 flops {f0}..{f0 + n - 1}", flush=True)
+        W.fill_(-7)
+        T.fill_(-7)
+        torch.cuda.synchronize()
+        t = time.time()
+        mtt.call("mtt_flop_counts", flops[f0:f0 + n].ctypes.data, n, W.data_ptr(), T.data_ptr(),
+                 mtt.LAYOUT_GLOBAL1326, None)
+        torch.cuda.synchronize()
+        t_kernels += time.time() - t
+        Wn, Tn = W[:n].long(), T[:n].long()
+        got = torch.stack([Wn.sum((1, 2)), Tn.sum((1, 2)), (Wn * w1).sum((1, 2)), (Tn * w2).sum((1, 2))], 1)
+        bad = np.nonzero((got.cpu().numpy() != want[f0:f0 + n]).any(1))[0]
+        assert len(bad) == 0, f"flops {(f0 + bad).tolist()[:10]} differ from the oracle checksums"
+        fm = torch.from_numpy((np.int64(1) << flops[f0:f0 + n].astype(np.int64)).sum(1)).to(torch_dev)
+        alive = (cm[None, :] & fm[:, None]) == 0
+        live = alive[:, :, None] & alive[:, None, :] & disjoint[None]
+        assert torch.equal(Tn, Tn.transpose(1, 2))
+        assert bool(((Wn + Wn.transpose(1, 2) + Tn) == 990)[live].all())
+        assert not bool(Wn[~live].any()) and not bool(Tn[~live].any())
+        del Wn, Tn, got, live, alive
+    print(f"config2: 1755 flops, {t_kernels:.3f} s in mtt_flop_counts (batches of {B})", flush=True)
+
+
 def test_gpu_flop_counts_batch_and_alive_layout(mtt, oracle, torch_dev):
     import torch
     flops = oracle.canonical_flops()

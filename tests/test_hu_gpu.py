@@ -99,17 +99,24 @@ def test_plain_multistreet_matches_oracle(solver_api, oracle, torch_dev):
     _multistreet_check(solver_api, oracle, solver_api.MODE_PLAIN_FP32, STATE_REL_FP32)
 
 
-def test_dropin_kernels_manual_iteration(solver_api, oracle, torch_dev, mtt):
+@pytest.mark.parametrize("stride", [4, 3])
+def test_dropin_kernels_manual_iteration(solver_api, oracle, torch_dev, mtt, stride):
     """A host loop over the reference-style kernel entry points (forward per
     depth, per-terminal evaluation, backward per depth) reproduces the
-    oracle's first iterations."""
+    oracle's first iterations; with per-action buffers of the bet menu's
+    four actions, and with the reference's stride of 3
+    (honest_multistreet_solver.cu:12) on a menu of at most three actions
+    (one bet size and all-in; fold, call, all-in raise)."""
     import torch
     g, r0, r1 = solver_api.config1(2)
+    if stride == 3:
+        g.menu[2] = solver_api.menu((0.5,), 0.0, 1, 1)
     nodes, ni, _ = solver_api.tree(g)
+    assert nodes["num_actions"][nodes["type"] == 0].max() == stride
     board = list(g.board)[:5]
     gi, cards = solver_api.alive_index(board)
     n_alive = len(gi)
-    NP, A = 1216, 4
+    NP, A = 1216, stride
     dev = torch_dev
     combos = np.zeros(NP, dtype=[("c1", "u1"), ("c2", "u1"), ("pad", "V6"), ("mask", "<u8")])
     combos["c1"][:n_alive] = cards[:, 0]
@@ -166,18 +173,20 @@ def test_dropin_kernels_manual_iteration(solver_api, oracle, torch_dev, mtt):
         for lv in levels:
             mtt.check(L.mtt_hu_forward_sweep(lv.data_ptr(), lv.numel(), d_nodes.data_ptr(), d_deck.data_ptr(),
                                              d_combos.data_ptr(), R.data_ptr(), sig.data_ptr(), reach.data_ptr(),
-                                             n_alive, None), "forward")
+                                             n_alive, A, None), "forward")
         mtt.check(L.mtt_hu_evaluate_terminals(d_terms.data_ptr(), d_terms.numel(), d_nodes.data_ptr(),
-                                              d_deck.data_ptr(), d_combos.data_ptr(), reach.data_ptr(),
-                                              mats.data_ptr(), cfv.data_ptr(), n_alive, None), "terminals")
+                                              reach.data_ptr(), mats.data_ptr(), cfv.data_ptr(), d_deck.data_ptr(),
+                                              d_combos.data_ptr(), n_alive, None), "terminals")
         for lv in reversed(levels):
             mtt.check(L.mtt_hu_backward_sweep(lv.data_ptr(), lv.numel(), d_nodes.data_ptr(), d_deck.data_ptr(),
                                               d_combos.data_ptr(), sig.data_ptr(), reach.data_ptr(), cfv.data_ptr(),
-                                              R.data_ptr(), S.data_ptr(), ctypes.c_float(float(t)), n_alive, None),
+                                              R.data_ptr(), S.data_ptr(), ctypes.c_float(float(t)), n_alive, A, None),
                       "backward")
         torch.cuda.synchronize()
         h.iterate(1)
-        Rc, Sc = h.state()
+        Rc, Sc = h.state()                      # the oracle's layout: stride 4
+        Rc = Rc.reshape(ni, 4, NP)[:, :A].ravel()
+        Sc = Sc.reshape(ni, 4, NP)[:, :A].ravel()
         Rg = R.cpu().numpy()
         Sg = S.cpu().numpy()
         scale = max(1.0, np.abs(Rc).max())

@@ -67,10 +67,28 @@ def test_4way_terminals_match_oracle(M, mtt, oracle, torch_dev, stacks):
     d_bf = torch.from_numpy(bf.astype(np.float32).ravel()).to(torch_dev)
     d_terms = torch.from_numpy(terms).to(torch_dev)
     d_cfv = torch.zeros(nn * 4 * 1216, device=torch_dev)
-    mtt.call("mtt_4way_evaluate_terminals", d_terms.data_ptr(), len(terms), d_nodes.data_ptr(), d_reach.data_ptr(),
-             d_board.data_ptr(), d_bf.data_ptr(), d_cfv.data_ptr(), None)
+    mtt.call("mtt_4way_evaluate_terminals_board", d_terms.data_ptr(), len(terms), d_nodes.data_ptr(),
+             d_reach.data_ptr(), d_board.data_ptr(), d_bf.data_ptr(), d_cfv.data_ptr(), None)
     torch.cuda.synchronize()
     cfv = d_cfv.cpu().numpy().reshape(nn, 4, 1216)
+    # the reference's argument list: combo_rank_pos instead of the board tables
+    d_rank = torch.from_numpy(board["rank_pos"][0].astype(np.int16)).to(torch_dev)
+    d_cfv2 = torch.zeros_like(d_cfv)
+    fl = np.ascontiguousarray(flop, np.uint8)
+    mtt.call("mtt_4way_evaluate_terminals", d_terms.data_ptr(), len(terms), d_nodes.data_ptr(), d_reach.data_ptr(),
+             d_rank.data_ptr(), d_bf.data_ptr(), d_cfv2.data_ptr(), fl.ctypes.data, None)
+    torch.cuda.synchronize()
+    assert torch.equal(d_cfv, d_cfv2)
+    # a strength order that is not the flop's is refused
+    bad = board["rank_pos"][0].astype(np.int16).copy()
+    lo = int(np.argmin(board["rank_pos"][0][:1176]))
+    hi = int(np.argmax(board["rank_pos"][0][:1176]))
+    bad[lo], bad[hi] = bad[hi], bad[lo]
+    d_bad = torch.from_numpy(bad).to(torch_dev)
+    rc = mtt.lib().mtt_4way_evaluate_terminals(d_terms.data_ptr(), len(terms), d_nodes.data_ptr(), d_reach.data_ptr(),
+                                               d_bad.data_ptr(), d_bf.data_ptr(), d_cfv2.data_ptr(), fl.ctypes.data,
+                                               None)
+    assert rc == mtt.MTT_ERR_ARG and "strength order" in mtt.last_error()
     ob = oracle.board4(flop)
     bf32 = bf.astype(np.float32).astype(np.float64)
     for n in terms:
@@ -103,8 +121,8 @@ def test_4way_dropin_sweeps_match_oracle(M, mtt, oracle, torch_dev):
         for lv in levels:
             mtt.call("mtt_4way_forward_sweep", lv.data_ptr(), lv.numel(), d_nodes.data_ptr(), R.data_ptr(),
                      sig.data_ptr(), reach.data_ptr(), None)
-        mtt.call("mtt_4way_evaluate_terminals", terms.data_ptr(), terms.numel(), d_nodes.data_ptr(), reach.data_ptr(),
-                 d_board.data_ptr(), d_bf.data_ptr(), cfv.data_ptr(), None)
+        mtt.call("mtt_4way_evaluate_terminals_board", terms.data_ptr(), terms.numel(), d_nodes.data_ptr(),
+                 reach.data_ptr(), d_board.data_ptr(), d_bf.data_ptr(), cfv.data_ptr(), None)
         for lv in reversed(levels):
             mtt.call("mtt_4way_backward_cfr_plus", lv.data_ptr(), lv.numel(), d_nodes.data_ptr(), sig.data_ptr(),
                      reach.data_ptr(), cfv.data_ptr(), R.data_ptr(), S.data_ptr(), ctypes.c_float(float(t)), None)
