@@ -34,6 +34,11 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 METRIC = "CFR+ iter/s at ≤0.5% pot exploitability: batched BF16-MFMA vs plain FP32"
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+# INT32 VALU: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane operations per
+# second; the count builder spends 2 (packed 16-bit) operations per
+# pair-runout comparison, which counts both W and T of an unordered pair
+VALU_LANE_OPS = 256 * 64 * 2.4e9
+PAIR_RUNOUTS_PER_FLOP = 635628 * 990          # unordered unblocked pairs x runouts avoiding both hands
 
 
 def load(name, path):
@@ -206,6 +211,87 @@ def strategy_linf(a, b, S_ref, ni, n_alive, chunk=8192):
             "frac_mass_1e-5_off_by_1e-2": n_off / max(n_live, 1)}
 
 
+def run_flop_counts(args):
+    """Exhaustive win/tie counts of all 1,755 canonical flops (global 1326
+    layout, int32 W and T), `--warmup` untimed passes then `--steps` timed
+    passes; a step is one pass over all flops in batches of 128."""
+    import ctypes
+    import torch
+    mtt = load("mtt", os.path.join(ROOT, "mtt-gpu-solver_amd", "mtt.py"))
+    canon = np.zeros((1755, 4), np.uint8)
+    assert mtt.lib().mtt_canonical_flops(canon.ctypes.data) == 1755
+    flops = np.ascontiguousarray(canon[:, :3])
+    B = 128
+    W = torch.empty((B, 1326, 1326), dtype=torch.int32, device="cuda:0")
+    T = torch.empty_like(W)
+
+    def one_pass():
+        for f0 in range(0, 1755, B):
+            n = min(B, 1755 - f0)
+            mtt.call("mtt_flop_counts", flops[f0:f0 + n].ctypes.data, n, W.data_ptr(), T.data_ptr(),
+                     mtt.LAYOUT_GLOBAL1326, None)
+
+    for _ in range(args.warmup):
+        one_pass()
+    torch.cuda.synchronize()
+    mtt.call("mtt_flop_counts_timing", 1)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record()
+    t = time.time()
+    for _ in range(args.steps):
+        one_pass()
+    ev1.record()
+    torch.cuda.synchronize()
+    dt = time.time() - t
+    ktimes = {}
+    k = 0
+    while True:
+        name = ctypes.create_string_buffer(32)
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        if mtt.lib().mtt_flop_counts_kernel_times(k, name, 32, ctypes.byref(ms), ctypes.byref(n)) != 0:
+            break
+        ktimes[name.value.decode()] = (ms.value, n.value)
+        k += 1
+    mtt.call("mtt_flop_counts_timing", 0)
+    launches = ktimes["fc_pairs"][1]
+    avg_ms = ktimes["fc_pairs"][0] / launches
+    flops_per_launch = 1755 * args.steps / launches
+    peak = VALU_LANE_OPS / 2 / 1e12
+    achieved = PAIR_RUNOUTS_PER_FLOP * flops_per_launch / (avg_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        try:
+            O = load("oracle_ctypes", os.path.join(ROOT, "oracle", "oracle_ctypes.py"))
+            O.flop_counts(flops[0])
+            n_s, t0 = 0, time.time()
+            while n_s < 40 and (n_s < 2 or time.time() - t0 < 15.0):
+                O.flop_counts(flops[(n_s * 97) % 1755])
+                n_s += 1
+            model, ncpu, avail = cpu_info()
+            cpu = {"value": n_s / (time.time() - t0), "unit": "flops/s", "cores": O.max_threads(), "kind": "port",
+                   "cpu_model": model, "nproc": ncpu,
+                   "sample": f"{n_s} canonical flops' exhaustive counts on the C oracle (OpenMP over runouts and rows)"}
+        except Exception as ex:
+            cpu = {"error": str(ex)}
+    out = {
+        "metric": "exhaustive flop W/T count builds per second (all 1,755 canonical flops, bit-exact)",
+        "value": 1755 * args.steps / dt, "unit": "flops/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic (the 1,755 canonical flops)",
+        "config": {"workload": "flop_counts_1755", "layout": "global1326", "batch_flops": B,
+                   "pair_runouts_per_flop": PAIR_RUNOUTS_PER_FLOP},
+        "event_ms_per_step": ev0.elapsed_time(ev1) / args.steps,
+        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in ktimes.items()},
+        "roofline": {"kernel": "k_fc_pairs", "bound": "valu", "achieved": achieved, "peak": peak,
+                     "unit": "Tcompare/s", "frac": achieved / peak, "traffic": None, "avg_ms": avg_ms,
+                     "note": "algorithmic pair-runout comparisons (unordered unblocked pairs x 990 runouts) per "
+                             "second; peak = INT32 VALU lane rate / 2 operations per comparison"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +305,9 @@ def main():
                          "exploitability every 50 iterations (0: stop after the timed window)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workload", default="hu_turn_river", choices=["hu_turn_river", "flop_counts"],
+                    help="hu_turn_river: the metric's CFR+ game (default); flop_counts: exhaustive W/T counts of "
+                         "all 1,755 canonical flops")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (3x): the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -238,6 +327,9 @@ def main():
         print(json.dumps({"error": "this solver runs on one GPU; sharding across GPUs is designed but not built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "flop_counts":
+        run_flop_counts(args)
+        return
     S = load("mtt_solver", os.path.join(ROOT, "mtt-gpu-solver_amd", "solver.py"))
     game, r0, r1 = S.config3(args.seed)
     total = args.warmup + args.steps

@@ -127,6 +127,11 @@ int mtt_flop_counts(const uint8_t* flops, int n_flops, int32_t* d_W, int32_t* d_
  * overlap on one GPU race on that workspace. */
 int mtt_flop_counts_noalloc(const uint8_t* flops, int n_flops, int32_t* d_W, int32_t* d_T,
                             int layout, mtt_stream_t stream);
+/* HIP-event timing of the count builder's kernels (fc_zero, fc_tables,
+ * fc_codes, fc_pairs) on the stream they run on: enable (1) or disable
+ * (0) and reset; then read entry idx (total ms, launches), -1 past the end. */
+int mtt_flop_counts_timing(int enable);
+int mtt_flop_counts_kernel_times(int idx, char* name, int name_cap, double* total_ms, int64_t* launches);
 
 /* kernel_evaluate_flop_ev, mtt_ev_dumper.cu:85-116: subsampled per-combo EV
  * of one flop against reach_p1/reach_p0 (1216 floats).  d_combos 1216

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "mtt_common.h"
+#include "solver.h"
 
 namespace mtt {
 
@@ -63,8 +64,8 @@ __global__ void k_eval7_batch(const uint8_t* __restrict__ cards, int64_t n, uint
 //      so s1 = 2W + T and s2 = 4W + T give W[i][j] and T[i][j] from one pass;
 //      W[j][i] = 990 - W[i][j] - T[i][j] (every unblocked pair has exactly
 //      C(45,2) = 990 live runouts).  Two VALU ops per pair-runout.
-//   3. k_fc_border: zeroes the rows and columns of the output that are not
-//      alive combos of the flop.
+//   0. k_fc_zero clears the outputs first, so the entries of output indices
+//      that are not alive combos of the flop stay 0.
 constexpr int kRunouts = 1176;
 constexpr int kRunPairs = 588;
 constexpr int kHandPad = 1280;    // 10 tiles of 128
@@ -113,33 +114,49 @@ constexpr TilePairTable make_tile_pairs() {
 }
 __constant__ TilePairTable c_tile_pairs = make_tile_pairs();
 
-// One workgroup per flop; the lexicographic walk is one lane's (1326 steps).
-__global__ void k_fc_tables(FlopBatch fb, int layout, FlopTables* __restrict__ tabs) {
-    const int f = blockIdx.x;
+// One wave per flop: lane a (< 52) owns the combos {a, b > a}; exclusive
+// wave scans of the per-lane counts place the deck and the alive combos (in
+// lexicographic order).
+__device__ __forceinline__ int fc_excl_scan(int v) {
+    const int lane = threadIdx.x;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(x, o, 64);
+        if (lane >= o) x += t;
+    }
+    return x - v;
+}
+
+__global__ __launch_bounds__(64) void k_fc_tables(FlopBatch fb, int layout, FlopTables* __restrict__ tabs) {
+    const int f = blockIdx.x, a = threadIdx.x;
     FlopTables& t = tabs[f];
     const uint64_t fm = (1ull << fb.c[3 * f]) | (1ull << fb.c[3 * f + 1]) | (1ull << fb.c[3 * f + 2]);
-    for (int o = threadIdx.x; o < 1328; o += blockDim.x)
-        t.alive_of_out[o] = (layout == MTT_LAYOUT_ALIVE1216 && o < kAliveFlop) ? (int16_t)o : (int16_t)-1;
-    for (int h = kAliveFlop + threadIdx.x; h < kHandPad; h += blockDim.x) {
-        t.c1[h] = 0; t.c2[h] = 0; t.out_idx[h] = -1;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    int nd = 0;
-    for (int c = 0; c < 52; ++c)
-        if (!(fm & (1ull << c))) t.deck[nd++] = (uint8_t)c;
-    for (int c = nd; c < 52; ++c) t.deck[c] = 0;
-    int n = 0, g = 0;
-    for (int a = 0; a < 52; ++a)
-        for (int b = a + 1; b < 52; ++b, ++g) {
-            if (fm & ((1ull << a) | (1ull << b))) continue;
+    const bool global = layout == MTT_LAYOUT_GLOBAL1326;
+    for (int o = a; o < 1328; o += 64) t.alive_of_out[o] = (!global && o < kAliveFlop) ? (int16_t)o : (int16_t)-1;
+    for (int h = kAliveFlop + a; h < kHandPad; h += 64) { t.c1[h] = 0; t.c2[h] = 0; t.out_idx[h] = -1; }
+    const bool card_live = a < 52 && !(fm & (1ull << a));
+    // deck: cards not on the flop, ascending
+    const int dpos = fc_excl_scan(card_live ? 1 : 0);
+    if (card_live) t.deck[dpos] = (uint8_t)a;
+    if (a >= 49 && a < 52) t.deck[a] = 0;
+    // combos {a, b > a}: alive if neither card is on the flop
+    const uint64_t above = a < 52 ? (~0ull << (a + 1)) & ((1ull << 52) - 1) : 0ull;
+    const int n_alive_a = card_live ? __builtin_popcountll(above & ~fm) : 0;
+    int n = fc_excl_scan(n_alive_a);
+    int g = a < 52 ? a * 51 - a * (a - 1) / 2 : 0;          // global index of {a, a+1}
+    for (int b = a + 1; b < 52; ++b, ++g) {
+        if (card_live && !(fm & (1ull << b))) {
             t.c1[n] = (uint8_t)a;
             t.c2[n] = (uint8_t)b;
-            int o = layout == MTT_LAYOUT_GLOBAL1326 ? g : n;
-            t.out_idx[n] = (int16_t)o;
-            if (layout == MTT_LAYOUT_GLOBAL1326) t.alive_of_out[o] = (int16_t)n;
+            t.out_idx[n] = (int16_t)(global ? g : n);
             n++;
         }
+    }
+    __syncthreads();
+    // inverse map (global layout) after every lane has placed its combos
+    if (global)
+        for (int h = a; h < kAliveFlop; h += 64) t.alive_of_out[t.out_idx[h]] = (int16_t)h;
 }
 
 // codes[f][rp][h] = v(runout 2rp) | v(runout 2rp+1) << 16.
@@ -268,16 +285,17 @@ __global__ __launch_bounds__(256) void k_fc_pairs(const FlopTables* __restrict__
     }
 }
 
-// Rows and columns of output indices that are not alive combos are 0.
-__global__ void k_fc_border(const FlopTables* __restrict__ tabs, int32_t* __restrict__ W,
-                            int32_t* __restrict__ T, int S) {
-    const int f = blockIdx.y, oi = blockIdx.x;
-    const FlopTables& t = tabs[f];
-    int32_t* Wr = W + ((size_t)f * S + oi) * S;
-    int32_t* Tr = T + ((size_t)f * S + oi) * S;
-    const bool row_dead = t.alive_of_out[oi] < 0;
-    for (int oj = threadIdx.x; oj < S; oj += blockDim.x)
-        if (row_dead || t.alive_of_out[oj] < 0) { Wr[oj] = 0; Tr[oj] = 0; }
+// The outputs of a batch start at zero (16-byte stores, grid-stride): the
+// pair kernel then writes every alive pair, and the entries of output
+// indices that are not alive combos stay 0.  A fill kernel rather than a
+// memset node, so that the launch sequence can be graph-captured.
+template <typename V>
+__global__ __launch_bounds__(256) void k_fc_zero(V* __restrict__ W, V* __restrict__ T, size_t n) {
+    const V z{};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        W[i] = z;
+        T[i] = z;
+    }
 }
 
 // Workspace of the allocation-free entry point: kNoAllocFlops flops' tables
@@ -327,6 +345,10 @@ static int fc_check(const char* fn, const uint8_t* flops, int n_flops, const int
     return MTT_OK;
 }
 
+// Per-kernel HIP-event timing of the count builder (off unless enabled).
+static std::mutex g_fc_timer_mu;
+static KernelTimer g_fc_timer;
+
 // Launches the four kernels for `cnt` flops (cnt <= kFcBatch) on `stream`,
 // with the tables and codes in `ws` (fc_workspace_bytes(cnt) bytes).
 static hipError_t fc_launch(const uint8_t* flops, int cnt, int32_t* W, int32_t* T, int layout, void* ws,
@@ -336,10 +358,25 @@ static hipError_t fc_launch(const uint8_t* flops, int cnt, int32_t* W, int32_t* 
     memcpy(fb.c, flops, 3 * (size_t)cnt);
     FlopTables* tabs = reinterpret_cast<FlopTables*>(ws);
     uint32_t* codes = reinterpret_cast<uint32_t*>(tabs + cnt);
+    std::lock_guard<std::mutex> lock(g_fc_timer_mu);
+    const bool timed = g_fc_timer.enabled;
+    KernelTimer::Mark m;
+    if (timed) g_fc_timer.begin("fc_zero", stream, m);
+    // cnt * S * S int32 per output, a multiple of 4 (S is even): 16-byte stores when both outputs are
+    // 16-byte aligned, else 4-byte stores
+    const size_t n32 = (size_t)cnt * S * S;
+    if ((((uintptr_t)W | (uintptr_t)T) & 15u) == 0)
+        hipLaunchKernelGGL(k_fc_zero<uint4>, dim3(2048), dim3(256), 0, stream, reinterpret_cast<uint4*>(W),
+                           reinterpret_cast<uint4*>(T), n32 / 4);
+    else
+        hipLaunchKernelGGL(k_fc_zero<int32_t>, dim3(2048), dim3(256), 0, stream, W, T, n32);
+    if (timed) { g_fc_timer.end(m, stream); g_fc_timer.begin("fc_tables", stream, m); }
     hipLaunchKernelGGL(k_fc_tables, dim3(cnt), dim3(64), 0, stream, fb, layout, tabs);
+    if (timed) { g_fc_timer.end(m, stream); g_fc_timer.begin("fc_codes", stream, m); }
     hipLaunchKernelGGL(k_fc_codes, dim3(kRunPairs, cnt), dim3(256), 0, stream, tabs, fb, codes);
+    if (timed) { g_fc_timer.end(m, stream); g_fc_timer.begin("fc_pairs", stream, m); }
     hipLaunchKernelGGL(k_fc_pairs, dim3(kTilePairs, cnt), dim3(256), 0, stream, tabs, codes, W, T, S);
-    hipLaunchKernelGGL(k_fc_border, dim3(S, cnt), dim3(256), 0, stream, tabs, W, T, S);
+    if (timed) g_fc_timer.end(m, stream);
     return hipGetLastError();
 }
 
@@ -645,6 +682,27 @@ int mtt_flop_counts_noalloc(const uint8_t* flops, int n_flops, int32_t* d_W, int
     // on the shared workspace.
     hipError_t e = fc_launch(flops, n_flops, d_W, d_T, layout, ws, stream);
     if (e != hipSuccess) return hip_fail(e, "mtt_flop_counts_noalloc kernels", __FILE__, __LINE__);
+    return MTT_OK;
+}
+
+int mtt_flop_counts_timing(int enable) {
+    std::lock_guard<std::mutex> lock(g_fc_timer_mu);
+    g_fc_timer.collect();
+    g_fc_timer.enabled = enable != 0;
+    g_fc_timer.names.clear();
+    g_fc_timer.ms.clear();
+    g_fc_timer.count.clear();
+    return MTT_OK;
+}
+
+int mtt_flop_counts_kernel_times(int idx, char* name, int name_cap, double* total_ms, int64_t* launches) {
+    MTT_ARG(name && total_ms && launches && name_cap > 0, "mtt_flop_counts_kernel_times: bad arguments");
+    std::lock_guard<std::mutex> lock(g_fc_timer_mu);
+    g_fc_timer.collect();
+    if (idx < 0 || idx >= (int)g_fc_timer.names.size()) return -1;
+    snprintf(name, name_cap, "%s", g_fc_timer.names[idx].c_str());
+    *total_ms = g_fc_timer.ms[idx];
+    *launches = g_fc_timer.count[idx];
     return MTT_OK;
 }
 

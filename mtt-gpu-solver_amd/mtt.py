@@ -73,6 +73,8 @@ _SIGS = {
     "mtt_combo_index": [_I, _I],
     "mtt_flop_counts": [_P, _I, _P, _P, _I, _P],
     "mtt_flop_counts_noalloc": [_P, _I, _P, _P, _I, _P],
+    "mtt_flop_counts_timing": [_I],
+    "mtt_flop_counts_kernel_times": [_I, ctypes.c_char_p, _I, _P, _P],
     "mtt_evaluate_flop_ev": [_P, _P, _P, _P, _P, _P, _P, _P],
     "mtt_ev_dump_tensor": [_I, _I, _P],
     "mtt_ev_dump": [_I, ctypes.c_char_p],

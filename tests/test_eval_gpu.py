@@ -110,6 +110,26 @@ def test_gpu_flop_counts_all_1755(mtt, torch_dev):
     print(f"config2: 1755 flops, {t_kernels:.3f} s in mtt_flop_counts (batches of {B})", flush=True)
 
 
+def test_gpu_flop_counts_unaligned_outputs(mtt, torch_dev):
+    """Output buffers that are not 16-byte aligned (offset by one int32) give
+    the same counts as aligned ones."""
+    import torch
+    canon = np.zeros((1755, 4), np.uint8)
+    mtt.lib().mtt_canonical_flops(canon.ctypes.data)
+    fl = np.ascontiguousarray(canon[[7, 901], :3])
+    S = 1326
+    n = 2 * S * S
+    Wa = torch.full((n,), -7, dtype=torch.int32, device=torch_dev)
+    Ta = torch.full((n,), -7, dtype=torch.int32, device=torch_dev)
+    Wu = torch.full((n + 1,), -7, dtype=torch.int32, device=torch_dev)
+    Tu = torch.full((n + 1,), -7, dtype=torch.int32, device=torch_dev)
+    mtt.call("mtt_flop_counts", fl.ctypes.data, 2, Wa.data_ptr(), Ta.data_ptr(), mtt.LAYOUT_GLOBAL1326, None)
+    mtt.call("mtt_flop_counts", fl.ctypes.data, 2, Wu.data_ptr() + 4, Tu.data_ptr() + 4, mtt.LAYOUT_GLOBAL1326, None)
+    torch.cuda.synchronize()
+    assert torch.equal(Wa, Wu[1:]) and torch.equal(Ta, Tu[1:])
+    assert int(Wu[0]) == -7 and int(Tu[0]) == -7
+
+
 def test_gpu_flop_counts_batch_and_alive_layout(mtt, oracle, torch_dev):
     import torch
     flops = oracle.canonical_flops()
