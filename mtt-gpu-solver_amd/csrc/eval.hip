@@ -35,6 +35,17 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
     return e == hipErrorOutOfMemory ? MTT_ERR_OOM : MTT_ERR_HIP;
 }
 
+// Owned device buffer, freed when it goes out of scope (after the blocking
+// copies that end every use).
+template <typename T>
+struct DeviceBuf {
+    T* p = nullptr;
+    hipError_t alloc(size_t n) { return hipMalloc(&p, sizeof(T) * (n ? n : 1)); }
+    ~DeviceBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
 // ------------------------------------------------- device batch evaluator --
 __global__ void k_eval7_batch(const uint8_t* __restrict__ cards, int64_t n, uint32_t* __restrict__ out) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -392,10 +403,10 @@ struct EvGrid {
     }
 };
 
-__global__ void k_ev_scores(const mtt_alive_combo* __restrict__ combos, const uint8_t* __restrict__ deck,
-                            const uint8_t* __restrict__ flop, const uint8_t* __restrict__ g1,
-                            const uint8_t* __restrict__ g2, uint32_t* __restrict__ sc) {
-    int p = blockIdx.x;
+__device__ __forceinline__ void k_ev_scores_body(const mtt_alive_combo* __restrict__ combos,
+                                                 const uint8_t* __restrict__ deck, const uint8_t* __restrict__ flop,
+                                                 const uint8_t* __restrict__ g1, const uint8_t* __restrict__ g2,
+                                                 uint32_t* __restrict__ sc, int p) {
     int x = deck[g1[p]], y = deck[g2[p]];
     uint64_t board = card_bits16(flop[0]) | card_bits16(flop[1]) | card_bits16(flop[2]) |
                      card_bits16(x) | card_bits16(y);
@@ -405,14 +416,21 @@ __global__ void k_ev_scores(const mtt_alive_combo* __restrict__ combos, const ui
     }
 }
 
+__global__ void k_ev_scores(const mtt_alive_combo* __restrict__ combos, const uint8_t* __restrict__ deck,
+                            const uint8_t* __restrict__ flop, const uint8_t* __restrict__ g1,
+                            const uint8_t* __restrict__ g2, uint32_t* __restrict__ sc) {
+    k_ev_scores_body(combos, deck, flop, g1, g2, sc, blockIdx.x);
+}
+
 // Per hero: villains j = 0,4,..; counts over the grid skipping dead cards;
 // eq = (2W+T)/(2n) and fused multiply-adds as in mtt_ev_dumper.cu:104-113.
-__global__ void k_ev_flop(const mtt_alive_combo* __restrict__ combos, const uint8_t* __restrict__ deck,
-                          const uint8_t* __restrict__ g1, const uint8_t* __restrict__ g2, int npts,
-                          const uint32_t* __restrict__ sc, const float* __restrict__ r0,
-                          const float* __restrict__ r1, float* __restrict__ ev0, float* __restrict__ ev1) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kAliveFlop) return;
+// Reaches are per-villain arrays, or the scalars u0 / u1 when those are null.
+__device__ __forceinline__ void k_ev_flop_body(const mtt_alive_combo* __restrict__ combos,
+                                               const uint8_t* __restrict__ deck, const uint8_t* __restrict__ g1,
+                                               const uint8_t* __restrict__ g2, int npts,
+                                               const uint32_t* __restrict__ sc, float u0, float u1,
+                                               const float* __restrict__ r0, const float* __restrict__ r1, int i,
+                                               float& out0, float& out1) {
     mtt_alive_combo hero = combos[i];
     float t0 = 0.0f, t1 = 0.0f;
     for (int j = 0; j < kAliveFlop; j += 4) {
@@ -429,12 +447,51 @@ __global__ void k_ev_flop(const mtt_alive_combo* __restrict__ combos, const uint
         }
         if (total > 0) {
             float eq = (float)(wins + ties) / (float)total;
-            t0 = __fmaf_rn(r1[j], __fmaf_rn(eq, 2.0f, -1.0f), t0);
-            t1 = __fmaf_rn(r0[j], __fmaf_rn(1.0f - eq, 2.0f, -1.0f), t1);
+            const float w1 = r1 ? r1[j] : u1, w0 = r0 ? r0[j] : u0;
+            t0 = __fmaf_rn(w1, __fmaf_rn(eq, 2.0f, -1.0f), t0);
+            t1 = __fmaf_rn(w0, __fmaf_rn(1.0f - eq, 2.0f, -1.0f), t1);
         }
     }
+    out0 = t0;
+    out1 = t1;
+}
+
+__global__ void k_ev_flop(const mtt_alive_combo* __restrict__ combos, const uint8_t* __restrict__ deck,
+                          const uint8_t* __restrict__ g1, const uint8_t* __restrict__ g2, int npts,
+                          const uint32_t* __restrict__ sc, const float* __restrict__ r0,
+                          const float* __restrict__ r1, float* __restrict__ ev0, float* __restrict__ ev1) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kAliveFlop) return;
+    float t0, t1;
+    k_ev_flop_body(combos, deck, g1, g2, npts, sc, 0.0f, 0.0f, r0, r1, i, t0, t1);
     ev0[i] = t0;
     ev1[i] = t1;
+}
+
+// Batched forms for the EV dump: blockIdx.y = flop of the batch, each with
+// its own combos [f][1216], deck [f][52], flop [f][4] and score rows
+// [f][npts][1216]; the arithmetic per flop is that of k_ev_scores / k_ev_flop.
+__global__ void k_ev_scores_b(const mtt_alive_combo* __restrict__ combos, const uint8_t* __restrict__ deck,
+                              const uint8_t* __restrict__ flop, const uint8_t* __restrict__ g1,
+                              const uint8_t* __restrict__ g2, int npts, uint32_t* __restrict__ sc) {
+    const int f = blockIdx.y;
+    k_ev_scores_body(combos + (size_t)f * kPadFlop, deck + (size_t)f * 52, flop + (size_t)f * 4, g1, g2,
+                     sc + (size_t)f * npts * kPadFlop, blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void k_ev_flop_b(const mtt_alive_combo* __restrict__ combos,
+                                                  const uint8_t* __restrict__ deck, const uint8_t* __restrict__ g1,
+                                                  const uint8_t* __restrict__ g2, int npts,
+                                                  const uint32_t* __restrict__ sc, float reach,
+                                                  float* __restrict__ ev) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kAliveFlop) return;
+    float t0, t1;
+    k_ev_flop_body(combos + (size_t)f * kPadFlop, deck + (size_t)f * 52, g1, g2, npts,
+                   sc + (size_t)f * npts * kPadFlop, reach, reach, nullptr, nullptr, i, t0, t1);
+    ev[((size_t)f * 2) * kPadFlop + i] = t0;
+    ev[((size_t)f * 2 + 1) * kPadFlop + i] = t1;
 }
 
 // -------------------------------------------------- turn / river matrices --
@@ -733,57 +790,80 @@ int mtt_evaluate_flop_ev(const mtt_alive_combo* d_combos, const uint8_t* d_deck,
 #pragma clang fp contract(off)
 int mtt_ev_dump_tensor(int first, int count, float* out) {
     MTT_ARG(out && first >= 0 && count >= 0 && first + count <= 1755, "mtt_ev_dump_tensor: bad range");
+    if (count == 0) return MTT_OK;
+    static const EvGrid grid;
     uint8_t flops[1755 * 4];
     mtt_canonical_flops(flops);
     memset(out, 0, sizeof(float) * (size_t)count * 1352);
-    mtt_alive_combo* d_combos = nullptr;
-    uint8_t *d_deck = nullptr, *d_flop = nullptr;
-    float *d_r = nullptr, *d_ev = nullptr;
-    hipError_t e = hipMalloc(&d_combos, sizeof(mtt_alive_combo) * kPadFlop);
-    if (e == hipSuccess) e = hipMalloc(&d_deck, 52);
-    if (e == hipSuccess) e = hipMalloc(&d_flop, 4);
-    if (e == hipSuccess) e = hipMalloc(&d_r, sizeof(float) * kPadFlop);
-    if (e == hipSuccess) e = hipMalloc(&d_ev, sizeof(float) * 2 * kPadFlop);
-    std::vector<float> reach(kPadFlop, 1.0f / (float)kAliveFlop);
-    if (e == hipSuccess) e = hipMemcpy(d_r, reach.data(), sizeof(float) * kPadFlop, hipMemcpyHostToDevice);
-    int rc = MTT_OK;
-    if (e != hipSuccess) rc = hip_fail(e, "mtt_ev_dump_tensor setup", __FILE__, __LINE__);
-    std::vector<mtt_alive_combo> alive(kPadFlop);
-    std::vector<float> ev(2 * kPadFlop);
-    for (int k = 0; k < count && rc == MTT_OK; ++k) {
-        const uint8_t* fl = flops + 4 * (first + k);
-        uint64_t fm = (1ull << fl[0]) | (1ull << fl[1]) | (1ull << fl[2]);
-        uint8_t deck[52];
-        int nd = 0;
-        for (int c = 0; c < 52; ++c)
-            if (!(fm & (1ull << c))) deck[nd++] = (uint8_t)c;
-        uint8_t c1s[1326], c2s[1326];
-        int na = alive_combos(fm, c1s, c2s);
-        memset(alive.data(), 0, sizeof(mtt_alive_combo) * kPadFlop);
-        for (int i = 0; i < na; ++i) alive[i] = {c1s[i], c2s[i], (1ull << c1s[i]) | (1ull << c2s[i])};
-        e = hipMemcpy(d_flop, fl, 3, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(d_deck, deck, nd, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(d_combos, alive.data(), sizeof(mtt_alive_combo) * kPadFlop, hipMemcpyHostToDevice);
-        if (e != hipSuccess) { rc = hip_fail(e, "mtt_ev_dump_tensor upload", __FILE__, __LINE__); break; }
-        rc = mtt_evaluate_flop_ev(d_combos, d_deck, d_flop, d_r, d_r, d_ev, d_ev + kPadFlop, 0);
-        if (rc) break;
-        e = hipMemcpy(ev.data(), d_ev, sizeof(float) * 2 * kPadFlop, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) { rc = hip_fail(e, "mtt_ev_dump_tensor download", __FILE__, __LINE__); break; }
+    // Flops go in batches of up to 256 per launch pair (scores 0.9 MB per flop).
+    const int B = count < 256 ? count : 256;
+    DeviceBuf<mtt_alive_combo> d_combos;
+    DeviceBuf<uint8_t> d_deck, d_flop, d_g;
+    DeviceBuf<uint32_t> d_sc;
+    DeviceBuf<float> d_ev;
+    hipError_t e = d_combos.alloc((size_t)B * kPadFlop);
+    if (e == hipSuccess) e = d_deck.alloc((size_t)B * 52);
+    if (e == hipSuccess) e = d_flop.alloc((size_t)B * 4);
+    if (e == hipSuccess) e = d_g.alloc(800);
+    if (e == hipSuccess) e = d_sc.alloc((size_t)B * grid.n * kPadFlop);
+    if (e == hipSuccess) e = d_ev.alloc((size_t)B * 2 * kPadFlop);
+    if (e == hipSuccess) e = hipMemcpy(d_g.p, grid.r1, 400, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_g.p + 400, grid.r2, 400, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "mtt_ev_dump_tensor setup", __FILE__, __LINE__);
+    std::vector<mtt_alive_combo> alive((size_t)B * kPadFlop);
+    std::vector<uint8_t> decks((size_t)B * 52), fls((size_t)B * 4);
+    std::vector<float> ev((size_t)B * 2 * kPadFlop);
+    std::vector<uint8_t> c1s((size_t)B * 1326), c2s((size_t)B * 1326);
+    std::vector<int> nas(B);
+    const float reach = 1.0f / (float)kAliveFlop;
+    for (int k0 = 0; k0 < count; k0 += B) {
+        const int n = count - k0 < B ? count - k0 : B;
+        memset(alive.data(), 0, sizeof(mtt_alive_combo) * alive.size());
+        memset(decks.data(), 0, decks.size());
+        for (int k = 0; k < n; ++k) {
+            const uint8_t* fl = flops + 4 * (first + k0 + k);
+            const uint64_t fm = (1ull << fl[0]) | (1ull << fl[1]) | (1ull << fl[2]);
+            int nd = 0;
+            for (int c = 0; c < 52; ++c)
+                if (!(fm & (1ull << c))) decks[(size_t)k * 52 + nd++] = (uint8_t)c;
+            for (int q = 0; q < 4; ++q) fls[(size_t)k * 4 + q] = q < 3 ? fl[q] : 0;
+            uint8_t* a1 = &c1s[(size_t)k * 1326];
+            uint8_t* a2 = &c2s[(size_t)k * 1326];
+            nas[k] = alive_combos(fm, a1, a2);
+            for (int i = 0; i < nas[k]; ++i)
+                alive[(size_t)k * kPadFlop + i] = {a1[i], a2[i], (1ull << a1[i]) | (1ull << a2[i])};
+        }
+        e = hipMemcpy(d_combos.p, alive.data(), sizeof(mtt_alive_combo) * (size_t)n * kPadFlop, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_deck.p, decks.data(), (size_t)n * 52, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_flop.p, fls.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "mtt_ev_dump_tensor upload", __FILE__, __LINE__);
+        hipLaunchKernelGGL(k_ev_scores_b, dim3(grid.n, n), dim3(256), 0, 0, d_combos.p, d_deck.p, d_flop.p, d_g.p,
+                           d_g.p + 400, grid.n, d_sc.p);
+        hipLaunchKernelGGL(k_ev_flop_b, dim3((kAliveFlop + 63) / 64, n), dim3(64), 0, 0, d_combos.p, d_deck.p, d_g.p,
+                           d_g.p + 400, grid.n, d_sc.p, reach, d_ev.p);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpy(ev.data(), d_ev.p, sizeof(float) * (size_t)n * 2 * kPadFlop,
+                                           hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "mtt_ev_dump_tensor kernels", __FILE__, __LINE__);
         // Host accumulation in combo order, multiply then add (mtt_ev_dumper.cu:160-168).
-        for (int s = 0; s < 4; ++s) {
-            float* base = out + ((size_t)k * 4 + s) * 2 * 169;
-            float mult = 1.0f + 0.1f * (float)s;
-            for (int c = 0; c < na; ++c) {
-                int cls = mtt_canonical_class(c1s[c], c2s[c]);
-                float a = ev[c] * mult;
-                float b = ev[kPadFlop + c] * mult;
-                base[cls] = base[cls] + a;
-                base[169 + cls] = base[169 + cls] + b;
+        for (int k = 0; k < n; ++k) {
+            const float* evk = &ev[(size_t)k * 2 * kPadFlop];
+            const uint8_t* a1 = &c1s[(size_t)k * 1326];
+            const uint8_t* a2 = &c2s[(size_t)k * 1326];
+            for (int s = 0; s < 4; ++s) {
+                float* base = out + ((size_t)(k0 + k) * 4 + s) * 2 * 169;
+                float mult = 1.0f + 0.1f * (float)s;
+                for (int c = 0; c < nas[k]; ++c) {
+                    int cls = mtt_canonical_class(a1[c], a2[c]);
+                    float x = evk[c] * mult;
+                    float y = evk[kPadFlop + c] * mult;
+                    base[cls] = base[cls] + x;
+                    base[169 + cls] = base[169 + cls] + y;
+                }
             }
         }
     }
-    hipFree(d_combos); hipFree(d_deck); hipFree(d_flop); hipFree(d_r); hipFree(d_ev);
-    return rc;
+    return MTT_OK;
 }
 #pragma clang fp contract(on)
 

@@ -160,6 +160,33 @@ def test_gpu_ev_dump_matches_reference(mtt, torch_dev):
         assert np.array_equal(out[0], vals[k]), f
 
 
+def test_gpu_ev_dump_file_all_1755(mtt, torch_dev, tmp_path):
+    """bin/ev_dumper end to end (mtt_ev_dumper.cu:118-176): mtt_ev_dump
+    writes the whole float32[1755][4][2][169] tensor to a file of the
+    reference's size; every player-0 scenario-0 value equals the reference's
+    data/mtt_ev_dump_1755.bin bit for bit (tests/golden/ev_dump_p0s0.npz),
+    player 1 is -player 0 to FP32 rounding as in the reference, and the
+    scenarios are the 1.1/1.2/1.3 rescalings of the same per-combo values."""
+    import time
+    ref = np.load(os.path.join(GOLDEN, "ev_dump_p0s0.npz"))["values"]
+    path = str(tmp_path / "mtt_ev_dump_1755.bin")
+    t = time.time()
+    mtt.call("mtt_ev_dump", 1755, path.encode())
+    dt = time.time() - t
+    raw = open(path, "rb").read()
+    assert len(raw) == 1755 * 4 * 2 * 169 * 4
+    dump = np.frombuffer(raw, "<f4").reshape(1755, 4, 2, 169)
+    d = np.abs(dump[:, 0, 0, :].astype(np.float64) - ref)
+    exact = float(np.mean(dump[:, 0, 0, :] == ref))
+    print(f"EV dump of 1755 flops in {dt:.2f} s; player 0 scenario 0 vs reference: max |diff| {d.max():.3g}, "
+          f"{exact:.4f} of the values bit-equal", flush=True)
+    assert np.array_equal(dump[:, 0, 0, :], ref)
+    assert np.abs(dump[:, :, 1, :].astype(np.float64) + dump[:, :, 0, :]).max() <= 1e-6
+    for sc in (1, 2, 3):
+        want = dump[:, 0, 0, :].astype(np.float64) * (1 + 0.1 * sc)
+        assert np.abs(dump[:, sc, 0, :] - want).max() <= 1e-5
+
+
 def test_gpu_turn_matrices(mtt, oracle, torch_dev):
     import torch
     flop = np.array([50, 33, 2], np.uint8)
