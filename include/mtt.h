@@ -150,6 +150,20 @@ int mtt_ev_dump_tensor(int first, int count, float* out);
  * "mtt_ev_dump_1755.bin" in the working directory; NULL means that). */
 int mtt_ev_dump(int limit, const char* path);
 
+/* kernel_gen_preflop_matrix, preflop_grid_generator.cu:95-127: for every
+ * (hero class, villain class) of the 169, the all-in equity of their
+ * representative hands over `samples` Monte-Carlo boards (3000 in the
+ * reference), with the reference's sample stream and its repeated board
+ * cards; d_eq[169*169] = (wins + ties/2) / samples, 0.5 on the diagonal;
+ * d_wins / d_ties (may be NULL) receive the integer counts.  Representative
+ * hands as mtt_class_cards.  Asynchronous on `stream`. */
+int mtt_gen_preflop_matrix(float* d_eq, int32_t* d_wins, int32_t* d_ties, int samples, mtt_stream_t stream);
+/* class_to_cards, preflop_grid_generator.cu:79-92: the representative hand
+ * of class cls (pairs suits 0/1, suited both suit 0, offsuit suits 0/1),
+ * with the rank-block walk corrected (the reference's gives the last class
+ * of each rank block two cards of one rank and never ends for 32s/32o). */
+int mtt_class_cards(int cls, uint8_t* c1, uint8_t* c2);
+
 /* kernel_gen_49_turn_matrices, honest_multistreet_solver.cu:56-86:
  * d_out[49][1216][1216] FP32, E[t][j][h] = (W + T/2)/n over the n rivers
  * avoiding both hands and the turn; blocked and padded entries 0. */

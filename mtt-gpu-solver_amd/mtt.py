@@ -81,6 +81,8 @@ _SIGS = {
     "mtt_gen_turn_matrices": [_P, _P, _P, _P, _P],
     "mtt_gen_river_matrices": [_P, _P, _P, _P, _I, _P, _P],
     "mtt_gen_flop_matrices": [_P, _I, _P, _P, _P],
+    "mtt_gen_preflop_matrix": [_P, _P, _P, _I, _P],
+    "mtt_class_cards": [_I, _P, _P],
 }
 _RESTYPES = {"mtt_last_error": ctypes.c_char_p, "mtt_version": ctypes.c_char_p,
              "mtt_eval7": ctypes.c_uint32}

@@ -103,6 +103,12 @@ void or_4way_destroy(or_4way* h);
 void or_4way_iterate(or_4way* h, int iters, int t0, float alpha, float beta, float gamma, int scheme);
 void or_4way_state(const or_4way* h, float* R, float* S, float* root_cfv);
 
+/* preflop pieces (or_preflop.c) */
+void or_class_to_cards(int cls, uint8_t* h1, uint8_t* h2);
+void or_preflop_cell(int h, int v, int samples, int32_t* wins, int32_t* ties);
+void or_preflop_matrix(int samples, int32_t* wins, int32_t* ties, float* eq);
+void or_preflop_exact(int h, int v, double* mean, double* var);
+
 #ifdef __cplusplus
 }
 #endif

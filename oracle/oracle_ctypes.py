@@ -56,6 +56,35 @@ def max_threads():
     return int(lib().or_max_threads())
 
 
+def class_to_cards(cls):
+    a, b = ctypes.c_uint8(), ctypes.c_uint8()
+    lib().or_class_to_cards(int(cls), ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def preflop_cell(h, v, samples):
+    w, t = ctypes.c_int32(), ctypes.c_int32()
+    lib().or_preflop_cell(int(h), int(v), int(samples), ctypes.byref(w), ctypes.byref(t))
+    return w.value, t.value
+
+
+def preflop_matrix(samples=3000):
+    """(wins, ties, equity) [169][169] of the Monte-Carlo all-in matrix."""
+    w = np.zeros((169, 169), np.int32)
+    t = np.zeros((169, 169), np.int32)
+    e = np.zeros((169, 169), np.float32)
+    lib().or_preflop_matrix(int(samples), _p(w), _p(t), _p(e))
+    return w, t, e
+
+
+def preflop_exact(h, v):
+    """Exact mean and variance of one sample's score (1 win, 1/2 tie) under
+    the Monte-Carlo kernel's board-sampling process."""
+    m, s = ctypes.c_double(), ctypes.c_double()
+    lib().or_preflop_exact(int(h), int(v), ctypes.byref(m), ctypes.byref(s))
+    return m.value, s.value
+
+
 def category_counts():
     out = np.zeros(9, np.uint64)
     lib().or_category_counts(_p(out))

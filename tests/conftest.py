@@ -25,6 +25,12 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def mtt():
+    # PyTorch ships its own HIP runtime; importing it before libmtt.so is
+    # loaded makes the library bind to that same runtime in this process
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     return _load("mtt", os.path.join(PKG, "mtt.py"))
 
 
