@@ -4,7 +4,8 @@ Run in the build container (the reference tree is not on the GPU box):
     python tests/golden/make_fixtures.py
 Fixtures are data only: slices of the reference's committed
 data/mtt_ev_dump_1755.bin (written by src/tools/mtt_ev_dumper.cu:172-173),
-its size and sha256, the contents of data/mtt_subset.bin, and the
+its size and sha256, the contents of data/mtt_subset.bin and
+data/mtt_subset_weights.txt, and the
 player-0 scenario-0 plane of the dump for all 1,755 flops
 (ev_dump_p0s0.npz; in the reference file player 1 is -player 0 to FP32
 rounding, at most 4.8e-7 apart, checked here, and scenarios 1-3 are host
@@ -31,11 +32,13 @@ def main():
     assert p1_dev <= 1e-6
     np.savez_compressed(os.path.join(HERE, "ev_dump_p0s0.npz"), values=dump[:, 0, 0, :].copy())
     sub = open(os.path.join(REF, "mtt_subset.bin"), "rb").read()
+    sub_txt = open(os.path.join(REF, "mtt_subset_weights.txt")).read()
     meta = {
         "ev_dump_bytes": len(raw),
         "ev_dump_sha256": hashlib.sha256(raw).hexdigest(),
         "ev_dump_flop_index": idx,
         "subset_bin_hex": sub.hex(),
+        "subset_weights_txt": sub_txt,
         "ev_dump_max_abs_p0_plus_p1": p1_dev,
     }
     with open(os.path.join(HERE, "ev_dump_meta.json"), "w") as f:
