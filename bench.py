@@ -292,6 +292,118 @@ def run_flop_counts(args):
     print(json.dumps(out))
 
 
+def run_preflop(args):
+    """Preflop DCFR co-solver (preflop_grid_generator.cu:426-614) on the
+    reference's 100-flop subset (its data/mtt_subset.bin, a committed
+    fixture): the 3000-sample all-in matrix and the subset's class matrices
+    once (timed, untimed warmup first), then `--warmup` untimed and `--steps`
+    timed calls of the 300-iteration DCFR loop (the README's figure: 45.4 ms
+    for 300 iterations on a T4)."""
+    import ctypes
+    import torch
+    mtt = load("mtt", os.path.join(ROOT, "mtt-gpu-solver_amd", "mtt.py"))
+    pf = load("mtt_preflop", os.path.join(ROOT, "mtt-gpu-solver_amd", "preflop.py"))
+    meta = json.load(open(os.path.join(ROOT, "tests", "golden", "ev_dump_meta.json")))
+    cards, w = pf.read_subset(bytes.fromhex(meta["subset_bin_hex"]))
+    n = len(cards)
+    iters = 300
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.time()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.time() - t
+
+    pf.allin_matrix(3000)
+    A, t_allin = timed(lambda: pf.allin_matrix(3000))
+    pf.class_matrices(cards)
+    mtt.call("mtt_preflop_timing", 1)
+    M, t_class = timed(lambda: pf.class_matrices(cards))
+    out = torch.empty((4, 3, 169), dtype=torch.float32, device="cuda:0")
+
+    def solve():
+        mtt.call("mtt_preflop_solve", M.data_ptr(), w.ctypes.data, n, A.data_ptr(), iters, 20.0, 1.35, 2.5,
+                 out.data_ptr(), None)
+
+    # the timed loop runs without per-kernel events; one more call measures the kernels
+    ct = {}
+    k = 0
+    while True:
+        name = ctypes.create_string_buffer(32)
+        ms, cnt = ctypes.c_double(), ctypes.c_int64()
+        if mtt.lib().mtt_preflop_kernel_times(k, name, 32, ctypes.byref(ms), ctypes.byref(cnt)) != 0:
+            break
+        ct[name.value.decode()] = ms.value
+        k += 1
+    mtt.call("mtt_preflop_timing", 0)
+    for _ in range(args.warmup):
+        solve()
+    _, dt = timed(lambda: [solve() for _ in range(args.steps)])
+    mtt.call("mtt_preflop_timing", 1)
+    solve()
+    ktimes = {}
+    k = 0
+    while True:
+        name = ctypes.create_string_buffer(32)
+        ms, cnt = ctypes.c_double(), ctypes.c_int64()
+        if mtt.lib().mtt_preflop_kernel_times(k, name, 32, ctypes.byref(ms), ctypes.byref(cnt)) != 0:
+            break
+        ktimes[name.value.decode()] = (ms.value, cnt.value)
+        k += 1
+    mtt.call("mtt_preflop_timing", 0)
+    ms_solve = 1e3 * dt / args.steps
+    # class-equity kernel: villain-runout comparisons (hero combos off the flop x 1,326 villain lanes x 192
+    # sampled runouts) at 3 packed 16-bit VALU operations per comparison
+    heroes = sum(sum(1 for a in range(52) for b in range(a + 1, 52) if a not in fl and b not in fl)
+                 for fl in (set(int(x) for x in c) for c in cards))
+    compares = heroes * 1326 * 192
+    ce_ms, ce_n = ct.get("pf_class_eq", 0.0), 2
+    ce_peak = VALU_LANE_OPS / 3 / 1e12
+    ce_ach = compares / (ce_ms * 1e-3) / 1e12 if ce_ms else None
+    # the DCFR loop's dominant kernel: one workgroup per iteration
+    tr_ms, tr_n = ktimes.get("pf_tree", (0.0, 1))
+    tree_flops = 169 * (n * 2 + 5 * 169 * 2 + 60)           # flop sum, five 169-long FMA chains, tree + DCFR
+    cpu = None
+    if not args.no_cpu:
+        try:
+            O = load("oracle_ctypes", os.path.join(ROOT, "oracle", "oracle_ctypes.py"))
+            Mh, Ah = M.cpu().numpy(), A.cpu().numpy().reshape(169, 169)
+            O.preflop_solve(Mh, w, Ah, iters=5)
+            t0 = time.time()
+            O.preflop_solve(Mh, w, Ah, iters=iters)
+            t_loop = time.time() - t0
+            model, ncpu, avail = cpu_info()
+            cpu = {"value": iters / t_loop, "unit": "DCFR iter/s", "cores": 1, "kind": "port",
+                   "cpu_model": model, "nproc": ncpu, "ms_per_300_iters": 1e3 * t_loop,
+                   "sample": f"the C oracle's {iters}-iteration loop on the same 100 class matrices, one thread"}
+        except Exception as ex:
+            cpu = {"error": str(ex)}
+    value = iters / (ms_solve * 1e-3)
+    baseline = 300 / 45.4e-3
+    print(json.dumps({
+        "metric": "preflop DCFR co-solver iterations/s (100 subset flops, 300-iteration loop)",
+        "value": value, "unit": "DCFR iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_solve, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": value / baseline, "baseline": "README.md:8 (45.4 ms for 300 iterations, Tesla T4)",
+        "dtype": "fp32", "data": "the reference's 100-flop subset (fixture); synthetic Monte-Carlo all-in matrix",
+        "config": {"workload": "preflop_cosolve_100", "flops": n, "iterations": iters, "stack_bb": 20.0,
+                   "bubble_factor": 1.35, "dead_pot": 2.5, "allin_samples": 3000},
+        "ms_allin_matrix": 1e3 * t_allin, "ms_class_matrices": 1e3 * t_class,
+        "kernel_ms_class_matrices": ct,
+        "kernel_ms_per_step": {k: v[0] for k, v in ktimes.items()},
+        "roofline": {"kernel": "k_pf_tree", "bound": "latency (one workgroup, dependent FP32 chains)",
+                     "achieved": tree_flops / (tr_ms / tr_n * 1e-3) / 1e12 if tr_ms else None,
+                     "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": tree_flops / (tr_ms / tr_n * 1e-3) / 157.3e12 if tr_ms else None, "traffic": None,
+                     "avg_ms": tr_ms / tr_n if tr_ms else None},
+        "roofline_class_eq": {"kernel": "k_pf_class_eq", "bound": "valu", "achieved": ce_ach, "peak": ce_peak,
+                              "unit": "Tcompare/s", "frac": ce_ach / ce_peak if ce_ach else None,
+                              "total_ms": ce_ms, "launches": ce_n, "compares": compares},
+        "cpu_baseline": cpu,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -305,9 +417,9 @@ def main():
                          "exploitability every 50 iterations (0: stop after the timed window)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", default="hu_turn_river", choices=["hu_turn_river", "flop_counts"],
+    ap.add_argument("--workload", default="hu_turn_river", choices=["hu_turn_river", "flop_counts", "preflop"],
                     help="hu_turn_river: the metric's CFR+ game (default); flop_counts: exhaustive W/T counts of "
-                         "all 1,755 canonical flops")
+                         "all 1,755 canonical flops; preflop: the preflop DCFR co-solver on the 100-flop subset")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (3x): the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -329,6 +441,9 @@ def main():
         sys.exit(2)
     if args.workload == "flop_counts":
         run_flop_counts(args)
+        return
+    if args.workload == "preflop":
+        run_preflop(args)
         return
     S = load("mtt_solver", os.path.join(ROOT, "mtt-gpu-solver_amd", "solver.py"))
     game, r0, r1 = S.config3(args.seed)

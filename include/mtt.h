@@ -191,6 +191,44 @@ typedef struct mtt_subset_flop {
 int mtt_gen_flop_matrices(const mtt_subset_flop* d_flops, int n_flops, const mtt_alive_combo* d_combos,
                           float* d_flop_matrices, mtt_stream_t stream);
 
+/* ------------------------------------------- preflop DCFR co-solver -- */
+/* kernel_gen_100_flop_matrices + kernel_reduce_combo_to_class_matrices,
+ * preflop_grid_generator.cu:130-223, over all 1,326 combos: for each flop
+ * (host array flops[n_flops][3], any card order) the class equity matrix
+ * d_class_matrices[(f*169 + v)*169 + h] = mean, over the member combos of
+ * hero class h and villain class v that avoid the flop and each other (hero
+ * members outer, villain inner, lexicographic, FP32 sum), of the flop-matrix
+ * entry (2W + T/2)/(2n) on the 192 runouts sampled from the hero's deck;
+ * 0.5 when no pair is valid.  The reference keeps only the first 1,176
+ * combos (dropping every combo whose first card is 35 or higher), averages
+ * only entries > 0 and reads unwritten memory for heroes on the flop
+ * (:486, :214, :146); here every combo counts and a valid pair with equity
+ * 0 counts.  Returns after the stream has finished. */
+int mtt_flop_class_matrices(const uint8_t* flops, int n_flops, float* d_class_matrices, mtt_stream_t stream);
+/* The preflop DCFR loop, preflop_grid_generator.cu:588-614 over :228-421:
+ * `iters` iterations (500 in the reference) of reaches into the flop ->
+ * flop co-solve over the class matrices (subset weights flop_weights, host
+ * [n_flops]) -> closed four-node tree values with the all-in matrix d_allin
+ * [169*169] (hero class row) -> DCFR on the four nodes (w_pos = t^1.5 /
+ * (t^1.5 + 1), w_strat = t^2; root reach = class combos / 1326).  Output
+ * d_strategy_sums[4][3][169]: nodes button root (fold, min-raise, jam), big
+ * blind facing the min-raise (fold, call, 3-bet jam), button facing the
+ * 3-bet jam (fold, call), big blind facing the open jam (fold, call); rows of
+ * 2-action nodes end with zeros.  The reference's stack 20bb, bubble factor
+ * 1.35, dead pot 2.5bb (:427-430).  Returns after the stream has finished. */
+int mtt_preflop_solve(const float* d_class_matrices, const float* flop_weights, int n_flops, const float* d_allin,
+                      int iters, float stack_bb, float bubble_factor, float dead_pot, float* d_strategy_sums,
+                      mtt_stream_t stream);
+/* Class matrices of a subset (host mtt_subset_flop records, as read from
+ * mtt_subset.bin) followed by mtt_preflop_solve with the subset weights:
+ * the reference program's steps 2-4 (preflop_grid_generator.cu:445-614). */
+int mtt_preflop_cosolve(const mtt_subset_flop* subset, int n_flops, const float* d_allin, int iters, float stack_bb,
+                        float bubble_factor, float dead_pot, float* d_strategy_sums, mtt_stream_t stream);
+/* Per-kernel HIP-event timing of the two calls above (kernels pf_codes,
+ * pf_class_eq, pf_cosolve, pf_tree); enabling clears the totals. */
+int mtt_preflop_timing(int enable);
+int mtt_preflop_kernel_times(int idx, char* name, int name_cap, double* total_ms, int64_t* launches);
+
 /* ------------------------------------------------------- HU game trees -- */
 /* Bet menu of one street.  The reference has no bet-menu config in code
  * (menus exist only as README.md:10 prose and MAX_ACTIONS); this struct

@@ -83,6 +83,11 @@ _SIGS = {
     "mtt_gen_flop_matrices": [_P, _I, _P, _P, _P],
     "mtt_gen_preflop_matrix": [_P, _P, _P, _I, _P],
     "mtt_class_cards": [_I, _P, _P],
+    "mtt_flop_class_matrices": [_P, _I, _P, _P],
+    "mtt_preflop_solve": [_P, _P, _I, _P, _I, _F, _F, _F, _P, _P],
+    "mtt_preflop_cosolve": [_P, _I, _P, _I, _F, _F, _F, _P, _P],
+    "mtt_preflop_timing": [_I],
+    "mtt_preflop_kernel_times": [_I, ctypes.c_char_p, _I, _P, _P],
 }
 _RESTYPES = {"mtt_last_error": ctypes.c_char_p, "mtt_version": ctypes.c_char_p,
              "mtt_eval7": ctypes.c_uint32}

@@ -38,6 +38,7 @@
  */
 #include "oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -137,4 +138,212 @@ void or_preflop_exact(int h, int v, double* mean, double* var) {
     }
     *mean = sw / n;
     *var = sw2 / n - (sw / n) * (sw / n);
+}
+
+/* --------------------------------------------------- preflop co-solver -- */
+/* Members of each of the 169 classes: lexicographic (c1 < c2) combos of the
+ * whole deck (all 1,326: the reference keeps only the first 1,176 of them,
+ * preflop_grid_generator.cu:486, which drops every combo whose first card is
+ * 35 or higher). */
+static uint8_t g_mem1[169][12], g_mem2[169][12];
+static int g_nmem[169];
+static int g_mem_ready;
+
+static void class_members_init(void) {
+    if (g_mem_ready) return;
+#pragma omp critical(or_class_members)
+    {
+        if (!g_mem_ready) {
+            memset(g_nmem, 0, sizeof(g_nmem));
+            for (int a = 0; a < 52; ++a)
+                for (int b = a + 1; b < 52; ++b) {
+                    int c = or_canonical_class(a, b);
+                    g_mem1[c][g_nmem[c]] = (uint8_t)a;
+                    g_mem2[c][g_nmem[c]] = (uint8_t)b;
+                    g_nmem[c]++;
+                }
+            g_mem_ready = 1;
+        }
+    }
+}
+
+/* kernel_reduce_combo_to_class_matrices, preflop_grid_generator.cu:189-223,
+ * over the flop equities of kernel_gen_100_flop_matrices (:130-183, here
+ * or_flop_matrix_entry): M[v * 169 + h] = mean equity of hero class h
+ * against villain class v over the member pairs that avoid the flop and
+ * each other (hero members outer, villain members inner, FP32 sum), 0.5
+ * when there is none.  The reference instead averages the entries > 0 of a
+ * matrix that holds only 1,176 combos and leaves heroes holding a flop card
+ * unwritten (its :146, :214); a pair with equity exactly 0 is a valid pair
+ * here. */
+void or_flop_class_matrix(const uint8_t flop[3], float* M) {
+    class_members_init();
+    const uint64_t fm = (1ull << flop[0]) | (1ull << flop[1]) | (1ull << flop[2]);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int h = 0; h < 169; ++h)
+        for (int v = 0; v < 169; ++v) {
+            float sum = 0.0f;
+            int cnt = 0;
+            for (int i = 0; i < g_nmem[h]; ++i) {
+                const uint64_t hm = (1ull << g_mem1[h][i]) | (1ull << g_mem2[h][i]);
+                if (hm & fm) continue;
+                for (int j = 0; j < g_nmem[v]; ++j) {
+                    const uint64_t vm = (1ull << g_mem1[v][j]) | (1ull << g_mem2[v][j]);
+                    if ((vm & fm) || (vm & hm)) continue;
+                    sum += or_flop_matrix_entry(flop, g_mem1[h][i], g_mem2[h][i], g_mem1[v][j], g_mem2[v][j]);
+                    cnt++;
+                }
+            }
+            M[v * 169 + h] = cnt ? sum / (float)cnt : 0.5f;
+        }
+}
+
+/* The reference's preflop DCFR loop (preflop_grid_generator.cu:588-614) over
+ * its closed tree, FP32:
+ *   reaches into the flop (kernel_update_dynamic_reaches, :412-421);
+ *   flop co-solve (kernel_cosolve_100_board_matrices, :228-285): per class,
+ *     the subset-weighted mean over flops of the net EV of taking the class
+ *     to the flop against the opponent's calling range, with realisation
+ *     factors 1.06 (button) / 0.94 (big blind), the flop pot dead_pot + 3.1
+ *     and the 2.1bb investment, losses scaled by the bubble factor;
+ *   closed tree values (kernel_build_closed_preflop_cfvs, :290-385):
+ *     button root fold / min-raise / jam, big blind fold / call / 3-bet jam,
+ *     button fold / call facing the jam, big blind fold / call facing an
+ *     open jam, all-ins by the Monte-Carlo all-in matrix;
+ *   DCFR (kernel_dcfr_step, :387-410) on the four nodes, w_pos = t^1.5 /
+ *     (t^1.5 + 1), w_strat = t^2, negatives floored, strategy sums weighted
+ *     by the root reach.
+ * Root reach: class combo count / 1326 (6 pairs, 4 suited, 12 offsuit).
+ * Outputs the strategy sums of the four nodes, sums[node][3][169]
+ * (nodes: button root, big blind defence, button facing the 3-bet jam,
+ * big blind facing the open jam; 2-action nodes leave row 2 zero). */
+static void dcfr_node(int na, float* R, float* sg, float* S, const float* cfv, const float* reach, float wp,
+                      float ws) {
+    for (int c = 0; c < 169; ++c) {
+        float node_v = 0.0f;
+        for (int a = 0; a < na; ++a) node_v = fmaf(sg[a * 169 + c], cfv[a * 169 + c], node_v);
+        float pos = 0.0f, nx[3] = {0, 0, 0};
+        for (int a = 0; a < na; ++a) {
+            const int k = a * 169 + c;
+            const float q = R[k] + (cfv[k] - node_v);
+            const float nr = wp * (q > 0.0f ? q : 0.0f);
+            R[k] = nr;
+            pos += nr;
+            nx[a] = nr;
+            S[k] = fmaf(ws * reach[c], sg[k], S[k]);
+        }
+        for (int a = 0; a < na; ++a) sg[a * 169 + c] = pos > 1e-7f ? nx[a] / pos : 1.0f / (float)na;
+    }
+}
+
+void or_preflop_solve(int n_flops, const float* flop_w, const float* M, const float* allin, int iters,
+                      float stack_bb, float bf, float dead_pot, float* sums_out) {
+    enum { C = 169 };
+    float R[4][3 * C], sg[4][3 * C], S[4][3 * C], cfv[4][3 * C];
+    float root[C], rbc[C], rbb[C], ev_btn[C], ev_bb[C];
+    memset(R, 0, sizeof(R));
+    memset(S, 0, sizeof(S));
+    memset(cfv, 0, sizeof(cfv));
+    memset(ev_btn, 0, sizeof(ev_btn));
+    memset(ev_bb, 0, sizeof(ev_bb));
+    const int na[4] = {3, 3, 2, 2};
+    for (int n = 0; n < 4; ++n)
+        for (int k = 0; k < 3 * C; ++k) sg[n][k] = k < na[n] * C ? 1.0f / (float)na[n] : 0.0f;
+    for (int c = 0; c < C; ++c) root[c] = (c < 13 ? 6.0f : (c < 91 ? 4.0f : 12.0f)) / 1326.0f;
+    const float pot_flop = dead_pot + 3.1f;
+    const float jam_pot = 2.0f * stack_bb + dead_pot;
+    for (int t = 1; t <= iters; ++t) {
+        const float tf = (float)t;
+        const float t15 = powf(tf, 1.5f);
+        const float wp = t15 / (t15 + 1.0f), ws = tf * tf;
+        for (int c = 0; c < C; ++c) {
+            rbc[c] = root[c] * sg[0][1 * C + c];
+            rbb[c] = root[c] * sg[1][1 * C + c];
+        }
+        /* flop co-solve */
+        for (int c = 0; c < C; ++c) {
+            float acc_btn = 0.0f, acc_bb = 0.0f, tw = 0.0f;
+            for (int f = 0; f < n_flops; ++f) {
+                const float w = flop_w[f];
+                tw += w;
+                const float* E = M + (size_t)f * C * C;
+                float s1 = 0.0f, e0 = 0.0f;
+                for (int o = 0; o < C; ++o) { s1 += rbb[o]; e0 = fmaf(rbb[o], E[o * C + c], e0); }
+                if (s1 > 1e-7f) e0 /= s1;
+                float s0 = 0.0f, e1 = 0.0f;
+                for (int o = 0; o < C; ++o) { s0 += rbc[o]; e1 = fmaf(rbc[o], 1.0f - E[c * C + o], e1); }
+                if (s0 > 1e-7f) e1 /= s0;
+                const float sh_btn = e0 * 1.06f * pot_flop, sh_bb = e1 * 0.94f * pot_flop;
+                const float net_btn = sh_btn >= 2.1f ? sh_btn - 2.1f : -((2.1f - sh_btn) * bf);
+                const float net_bb = sh_bb >= 2.1f ? sh_bb - 2.1f : -((2.1f - sh_bb) * bf);
+                acc_btn = fmaf(w, net_btn, acc_btn);
+                acc_bb = fmaf(w, net_bb, acc_bb);
+            }
+            if (tw > 0.0f) { ev_btn[c] = acc_btn / tw; ev_bb[c] = acc_bb / tw; }
+        }
+        /* closed tree values */
+        for (int c = 0; c < C; ++c) {
+            /* node 2: button facing the 3-bet jam */
+            cfv[2][0 * C + c] = -2.1f * bf;
+            float m3 = 0.0f, eq_call = 0.0f;
+            for (int o = 0; o < C; ++o) {
+                const float r = root[o] * sg[1][2 * C + o];
+                m3 += r;
+                eq_call = fmaf(r, allin[c * C + o], eq_call);
+            }
+            if (m3 > 1e-7f) eq_call /= m3;
+            cfv[2][1 * C + c] = eq_call * (jam_pot - stack_bb) - (1.0f - eq_call) * bf * stack_bb;
+            const float v3b = sg[2][0 * C + c] * cfv[2][0 * C + c] + sg[2][1 * C + c] * cfv[2][1 * C + c];
+            /* node 1: big blind facing the min-raise */
+            cfv[1][0 * C + c] = -1.0f * bf;
+            cfv[1][1 * C + c] = ev_bb[c];
+            float bf3 = 0.0f, bc3 = 0.0f, eq_jam = 0.0f;
+            for (int o = 0; o < C; ++o) {
+                const float ro = root[o] * sg[0][1 * C + o];
+                bf3 = fmaf(ro, sg[2][0 * C + o], bf3);
+                bc3 = fmaf(ro, sg[2][1 * C + o], bc3);
+                eq_jam = fmaf(ro * sg[2][1 * C + o], 1.0f - allin[o * C + c], eq_jam);
+            }
+            if (bc3 > 1e-7f) eq_jam /= bc3;
+            const float tot3 = bf3 + bc3 + 1e-9f;
+            const float win_unc = dead_pot + 2.1f;
+            const float ev_sh = eq_jam * (jam_pot - stack_bb) - (1.0f - eq_jam) * bf * stack_bb;
+            cfv[1][2 * C + c] = (bf3 * win_unc + bc3 * ev_sh) / tot3;
+            /* node 3: big blind facing the open jam */
+            cfv[3][0 * C + c] = -1.0f * bf;
+            float mj = 0.0f, eq_oj = 0.0f;
+            for (int o = 0; o < C; ++o) {
+                const float r = root[o] * sg[0][2 * C + o];
+                mj += r;
+                eq_oj = fmaf(r, 1.0f - allin[o * C + c], eq_oj);
+            }
+            if (mj > 1e-7f) eq_oj /= mj;
+            cfv[3][1 * C + c] = eq_oj * (jam_pot - stack_bb) - (1.0f - eq_oj) * bf * stack_bb;
+            /* node 0: button root */
+            cfv[0][0 * C + c] = 0.0f;
+            float bff = 0.0f, bfc = 0.0f, bfj = 0.0f;
+            for (int o = 0; o < C; ++o) {
+                bff = fmaf(root[o], sg[1][0 * C + o], bff);
+                bfc = fmaf(root[o], sg[1][1 * C + o], bfc);
+                bfj = fmaf(root[o], sg[1][2 * C + o], bfj);
+            }
+            const float tbb = bff + bfc + bfj + 1e-9f;
+            cfv[0][1 * C + c] = (bff * dead_pot + bfc * ev_btn[c] + bfj * v3b) / tbb;
+            float fj = 0.0f, cj = 0.0f, eq_b = 0.0f;
+            for (int o = 0; o < C; ++o) {
+                fj = fmaf(root[o], sg[3][0 * C + o], fj);
+                cj = fmaf(root[o], sg[3][1 * C + o], cj);
+                eq_b = fmaf(root[o] * sg[3][1 * C + o], allin[c * C + o], eq_b);
+            }
+            const float toj = fj + cj + 1e-9f;
+            if (cj > 1e-7f) eq_b /= cj;
+            cfv[0][2 * C + c] =
+                (fj * dead_pot + cj * (eq_b * (jam_pot - stack_bb) - (1.0f - eq_b) * bf * stack_bb)) / toj;
+        }
+        dcfr_node(3, R[0], sg[0], S[0], cfv[0], root, wp, ws);
+        dcfr_node(3, R[1], sg[1], S[1], cfv[1], root, wp, ws);
+        dcfr_node(2, R[2], sg[2], S[2], cfv[2], root, wp, ws);
+        dcfr_node(2, R[3], sg[3], S[3], cfv[3], root, wp, ws);
+    }
+    memcpy(sums_out, S, sizeof(S));
 }

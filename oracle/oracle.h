@@ -108,6 +108,9 @@ void or_class_to_cards(int cls, uint8_t* h1, uint8_t* h2);
 void or_preflop_cell(int h, int v, int samples, int32_t* wins, int32_t* ties);
 void or_preflop_matrix(int samples, int32_t* wins, int32_t* ties, float* eq);
 void or_preflop_exact(int h, int v, double* mean, double* var);
+void or_flop_class_matrix(const uint8_t flop[3], float* M);
+void or_preflop_solve(int n_flops, const float* flop_w, const float* M, const float* allin, int iters,
+                      float stack_bb, float bf, float dead_pot, float* sums_out);
 
 #ifdef __cplusplus
 }

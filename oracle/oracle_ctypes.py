@@ -320,3 +320,24 @@ def flop_matrix_entry(flop, hero, villain):
     L.or_flop_matrix_entry.restype = ctypes.c_float
     f = np.ascontiguousarray(flop, np.uint8)
     return float(L.or_flop_matrix_entry(f.ctypes.data, int(hero[0]), int(hero[1]), int(villain[0]), int(villain[1])))
+
+
+def flop_class_matrix(flop):
+    """[169 (villain class)][169 (hero class)] float32 class equities of one
+    flop (or_flop_class_matrix)."""
+    M = np.zeros((169, 169), np.float32)
+    f = np.asarray(flop, np.uint8)
+    lib().or_flop_class_matrix(_p(f), _p(M))
+    return M
+
+
+def preflop_solve(M, weights, allin, iters=500, stack_bb=20.0, bf=1.35, dead_pot=2.5):
+    """Strategy sums [4 nodes][3 actions][169] of the preflop DCFR loop
+    (or_preflop_solve) over class matrices M [n][169][169]."""
+    M = np.ascontiguousarray(M, np.float32)
+    w = np.ascontiguousarray(weights, np.float32)
+    a = np.ascontiguousarray(allin, np.float32)
+    out = np.zeros((4, 3, 169), np.float32)
+    lib().or_preflop_solve(int(M.shape[0]), _p(w), _p(M), _p(a), int(iters), ctypes.c_float(stack_bb),
+                           ctypes.c_float(bf), ctypes.c_float(dead_pot), _p(out))
+    return out
