@@ -27,6 +27,15 @@
  *   or_preflop_exact     the same board-sampling process enumerated: every
  *       sequence of the five picks (k + 0..47-k), equally likely, giving the
  *       expectation and variance of one sample's score (1 win, 1/2 tie).
+ *   or_flop_class_matrix  kernel_gen_100_flop_matrices +
+ *       kernel_reduce_combo_to_class_matrices, :130-223: per flop, the mean
+ *       of or_flop_matrix_entry over the member pairs of each (hero class,
+ *       villain class) that avoid the flop and each other, over all 1,326
+ *       combos (the reference keeps 1,176, filters eq > 0 and reads unwritten
+ *       entries: its :486, :214, :146).
+ *   or_preflop_solve     the DCFR loop, :588-614 over :228-421 (co-solve,
+ *       closed tree, DCFR step, reach update), FP32 with the reference's
+ *       operation shapes and explicit fused multiply-adds.
  * Pins: the representative hands map back to their classes through
  * or_canonical_class (pinned by closed forms in the tests of or_eval.c);
  * the Monte-Carlo estimate over many samples lies within a few standard
@@ -34,7 +43,18 @@
  * collision rule, pick rule, evaluator).  The particular sample stream (LCG
  * constants, seed, shift) follows the reference's source literally; no
  * output of the reference pins it ("parity unpinned" for the exact stream,
- * pinned in distribution).
+ * pinned in distribution).  or_flop_class_matrix is pinned entry by entry
+ * by an independent brute force over class members of or_flop_matrix_entry
+ * (itself pinned by the quarter-tie and blocked-pair cases) and by the
+ * no-valid-pair rows of a three-ace flop.  or_preflop_solve: the strategy
+ * sums are pinned by closed forms (sums over actions = root reach x sum t^2
+ * at every node; on coin-flip data the big blind folds to an open jam from
+ * the second iteration on) and the orientation of both equity matrices by a
+ * game whose class index orders strength (strong classes call jams, the
+ * weakest folds first).  The closed-tree payoff formulas themselves (the
+ * 1.06 / 0.94 realisation, the 2.1bb investment, the pot terms) are the
+ * reference's heuristics restated from its source; no output of the
+ * reference exists to pin them ("parity unpinned" for those formulas).
  */
 #include "oracle.h"
 
