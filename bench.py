@@ -7,6 +7,10 @@ betting, river chance (48), river betting, showdown.  Pot 6bb, stacks 40bb,
 per street 33%/75% pot, one 2.5x raise and all-in; Dirichlet(0.5) ranges
 over the 1326 combos (926,542 nodes).
 
+A second batched leg computes the river showdown values by exact prefix
+scans in strength order instead of the GEMM (same game, same qualifier
+measurement; `batched_river_scan`; skip with --no-scan).
+
 Each path: `--warmup` untimed iterations, `--steps` timed iterations
 (`value` = the batched path's timed iterations/s, all state resident in
 HBM), then untimed iterations up to `--solve-iters` (default 1250, what
@@ -65,6 +69,9 @@ def algorithmic_work(S, game):
     River showdown, exact strength-order sweep (MTT_RIVER_SD=sweep): each
     column's reach read and its values written (4 + 4 B per live combo) and
     the column's 52 card totals read from both forward halves (832 B).
+    River showdown by prefix scans (MTT_RIVER_SD=scan, the bench's second
+    batched leg): each column read and written (4 + 4 B per live combo) and
+    the board's scan tables read once.
     River backward: regrets read and written (8 B per action slot and
     combo), showdown values read (4 B per column entry), card sums of each
     terminal read (2 players x 53 x 8 B).
@@ -417,6 +424,8 @@ def main():
                          "exploitability every 50 iterations (0: stop after the timed window)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-scan", action="store_true",
+                    help="skip the batched leg whose river showdown is computed by exact prefix scans")
     ap.add_argument("--workload", default="hu_turn_river", choices=["hu_turn_river", "flop_counts", "preflop"],
                     help="hu_turn_river: the metric's CFR+ game (default); flop_counts: exhaustive W/T counts of "
                          "all 1,755 canonical flops; preflop: the preflop DCFR co-solver on the 100-flop subset")
@@ -471,6 +480,36 @@ def main():
     st_b = sb.strategy() if not args.no_plain else None
     sb.close()
     del sb
+
+    # ------------------------------- batched, river showdown by prefix scans ---
+    scan = None
+    if not args.no_scan and not river_sweep_mode():
+        prev = os.environ.get("MTT_RIVER_SD")
+        os.environ["MTT_RIVER_SD"] = "scan"
+        try:
+            t = time.time()
+            ss = S.Solver(game, r0, r1, S.MODE_BF16_SPLIT)
+            setup_s = time.time() - t
+        finally:
+            if prev is None:
+                del os.environ["MTT_RIVER_SD"]
+            else:
+                os.environ["MTT_RIVER_SD"] = prev
+        ss.run(args.warmup)
+        ss.timing(True)
+        t = time.time()
+        ss.run(args.steps)
+        dt_s = time.time() - t
+        kts = ss.kernel_times()
+        ss.timing(False)
+        log(f"batched, river showdown by prefix scans: {args.steps} timed iterations {args.steps / dt_s:.2f} it/s")
+        t = time.time()
+        curve_s = solve_checkpoints(ss, solve_total, name="batched scan")
+        scan = {"iter_per_s": args.steps / dt_s, "timed_steps": args.steps, "setup_s": setup_s,
+                "kernel_ms_per_iter": {k: v[0] / max(v[1], 1) for k, v in kts.items()},
+                "untimed_solve_s": time.time() - t, **expl_summary(curve_s)}
+        ss.close()
+        del ss
 
     # ----------------------------------------------------- plain FP32 ---
     plain = None
@@ -529,6 +568,14 @@ def main():
         roof = hbm_roof(dominant, byts.get(dominant, float("nan")), per_iter.get(dominant, float("nan")))
     sd_roof = hbm_roof("river_showdown", work["showdown_bytes"], sd_ms) if sweep else mfma_roof("river_showdown", sd_ms)
     sd_roof["form"] = "strength-order sweep" if sweep else "sign x blocker GEMM"
+    if scan:
+        # the scan reads each column and writes its values (4 + 4 B per live combo) and reads its board's
+        # tables (3 x 1088 + 23 x 64 words) once
+        scan_bytes = 2 * work["river_showdowns"] * 1081 * 8 + work["boards"] * (3 * 1088 + 23 * 64) * 4
+        scan["roofline_showdown"] = hbm_roof("river_showdown", scan_bytes,
+                                             scan["kernel_ms_per_iter"].get("river_showdown", float("nan")))
+        scan["roofline_showdown"]["form"] = "prefix scans (exact integer sums)"
+        scan["roofline_showdown"]["traffic"] = None
 
     cpu = None
     if not args.no_cpu:
@@ -562,6 +609,8 @@ def main():
         "kernel_ms_per_iter": per_iter,
         "plain_fp32": plain,
         "speedup_vs_plain_fp32": (ips_b / plain["iter_per_s"]) if plain else None,
+        "batched_river_scan": scan,
+        "scan_speedup_vs_plain_fp32": (scan["iter_per_s"] / plain["iter_per_s"]) if (scan and plain) else None,
         "strategy_linf_batched_vs_plain": linf,
         "roofline": roof,
         "roofline_showdown": sd_roof,

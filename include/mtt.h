@@ -384,6 +384,14 @@ int mtt_river_sweep_program(const uint8_t* cards, const uint16_t* grank, int n, 
                             size_t* len);
 int mtt_solver_river_showdown(mtt_solver* h, int board, float* reach, float* values, uint8_t* cards,
                               uint16_t* grank, int* nlive, int* ncol);
+/* The per-board tables of the prefix-scan river showdown (MTT_RIVER_SD=scan)
+ * for the 1081 combos of a river board in strength order (cards[1081][2],
+ * non-decreasing grank[1081]): htab[3][1088] = per position gs | ge << 16,
+ * then for each of its two cards (list * 47 + lo) | (list * 47 + hi) << 16,
+ * [lo, hi) being the entries of the card's 46-combo list inside the
+ * position's tie group; ltab[23][64] = per card list (lane), the positions
+ * of its 46 combos two per word.  Host only. */
+int mtt_river_scan_tables(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* htab, uint32_t* ltab);
 int mtt_solver_destroy(mtt_solver* h);
 
 /* ----------------------------------------------- batched flop subgames -- */

@@ -48,6 +48,31 @@ constexpr int SD_VISIT = 16;
 // Records the kernel may read past the end of the last program.
 constexpr int SD_PAD_RECORDS = SD_VISIT;
 
+// ------------------------------------------------------ prefix scan --
+// The same values by prefix sums, one wave per column (lanes over combos).
+// With the column in integer fixed point R_j (2^sc units, sc chosen so that
+// the column total stays below 2^27) and, in doubled form,
+//     DP[x]     = 2 P(x) - Ptot                   (P: prefix in strength order)
+//     D_c[k]    = 2 ex_c(k) - tot_c               (ex_c: prefix over the 46
+//                                                  combos holding card c)
+// hero i (cards a, b, tie group [gs, ge), whose group covers entries
+// [lo_c, hi_c) of card c's list) has
+//     2 C[i] = DP[gs] + DP[ge] - D_a[lo_a] - D_a[hi_a] - D_b[lo_b] - D_b[hi_b]
+// exactly.  Per board the tables hold, per combo position j, gs | ge << 16,
+// (a's list * 47 + lo_a) | (a's list * 47 + hi_a) << 16 and the same for b
+// (htab[3][NPR]), and per live card the positions of its 46 combos, two per
+// word (ltab[23][64], lane = the card's list index).  River boards only:
+// n = 1081 combos of 47 live cards.  Returns false otherwise, or if a card is
+// invalid or the ranks are not sorted.
+constexpr int SC_LISTS = 47;
+constexpr int SC_LIST = 46;
+constexpr int SC_LWORDS = SC_LIST / 2;
+bool sc_tables(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* htab, uint32_t* ltab);
+// Launches the scan for every board of level 1: columns from D.Bf, values
+// to D.C (0 on padding); htab / ltab: the boards' tables, back to back.
+void sc_launch(const BatchedDev& D, const uint32_t* htab, const uint32_t* ltab, int nboards, int ncol,
+               hipStream_t st);
+
 // Launches the sweep for every board of level 1: columns from D.Bf, card
 // totals from the forward's card sums (L.part), values to D.C.
 // prog: the concatenated programs followed by SD_PAD_RECORDS zero records;

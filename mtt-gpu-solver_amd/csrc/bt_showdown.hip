@@ -250,6 +250,144 @@ __global__ __launch_bounds__(64) void k_sd_sweep(BatchedDev D, const uint32_t* _
 #undef SD_READ
 }
 
+
+// ------------------------------------------------------ prefix scan --
+constexpr int SC_PER = NPR / 64;            // 17 combo positions per lane
+constexpr int SC_WAVES = 4;                 // waves per workgroup (each its own columns and LDS)
+constexpr int SC_CPW = 4;                   // columns per wave
+constexpr int SC_D = SC_LISTS * (SC_LIST + 1);
+static_assert(NPR % 64 == 0 && NPR % 4 == 0, "combo padding");
+
+__device__ __forceinline__ int sc_excl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(x, o, 64);
+        if (lane >= o) x += t;
+    }
+    return x - v;
+}
+
+__device__ __forceinline__ float sc_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One workgroup of SC_WAVES waves per (board, SC_WAVES x SC_CPW columns); a
+// wave takes SC_CPW columns in turn with its board's tables in registers:
+//   1. the column (FP32, 16-byte loads) and its total; R = rint(r 2^sc),
+//      sc = 27 - exponent of the total, written to LDS;
+//   2. each lane reads 17 consecutive positions: local scan, wave scan of
+//      the lane totals;
+//   3. lanes 0..46 gather their card's 46 combos from LDS and write the
+//      doubled list prefixes D_c;
+//   4. the doubled strength prefixes DP overwrite the column in LDS;
+//   5. each lane takes positions lane, lane + 64, ..: six LDS lookups, one
+//      conversion, coalesced stores.
+__global__ __launch_bounds__(64 * SC_WAVES) void k_sd_scan(BatchedDev D, const uint32_t* __restrict__ htab,
+                                                           const uint32_t* __restrict__ ltab, int ncol) {
+    __shared__ int4 As4[SC_WAVES][(NPR + 4) / 4];
+    __shared__ int Ds_[SC_WAVES][SC_D + 3];
+    const LevelDev& L = D.L[1];
+    const int b = blockIdx.y, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ncolp = L.ncolp;
+    int* As = reinterpret_cast<int*>(As4[w]);
+    int* Ds = Ds_[w];
+    uint32_t h0[SC_PER], h1[SC_PER], h2[SC_PER], lp[SC_LWORDS];
+    {
+        const uint32_t* ht = htab + (size_t)b * 3 * NPR;
+#pragma unroll
+        for (int t = 0; t < SC_PER; ++t) {
+            h0[t] = ht[64 * t + lane];
+            h1[t] = ht[NPR + 64 * t + lane];
+            h2[t] = ht[2 * NPR + 64 * t + lane];
+        }
+        const uint32_t* lt = ltab + (size_t)b * SC_LWORDS * 64;
+#pragma unroll
+        for (int k = 0; k < SC_LWORDS; ++k) lp[k] = lt[k * 64 + lane];
+    }
+    const int cw0 = (blockIdx.x * SC_WAVES + w) * SC_CPW;
+    for (int q = 0; q < SC_CPW; ++q) {
+        const int col = cw0 + q;
+        if (col >= ncol) break;                       // wave-uniform; no workgroup barriers below
+        const float4* rc = reinterpret_cast<const float4*>(D.Bf + ((size_t)b * ncolp + col) * NPR);
+        // 1. column, total, fixed point (padding entries are 0)
+        float4 v[5];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = rc[lane + 64 * k];
+        v[4] = lane < (NPR / 4 - 256) ? rc[256 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane == 14) v[4].y = v[4].z = v[4].w = 0.0f;   // positions 1081..1087 are padding
+        if (lane == 15) v[4] = make_float4(0.f, 0.f, 0.f, 0.f);
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s += (v[k].x + v[k].y) + (v[k].z + v[k].w);
+        s = sc_sum(s);
+        int ex = 0;
+        (void)frexpf(s, &ex);
+        const int sc = s > 0.0f ? 27 - ex : 0;       // every R_j and their total below 2^27 (+ rounding)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            if (k == 4 && lane >= NPR / 4 - 256) break;
+            const int4 r = make_int4((int)rintf(ldexpf(v[k].x, sc)), (int)rintf(ldexpf(v[k].y, sc)),
+                                     (int)rintf(ldexpf(v[k].z, sc)), (int)rintf(ldexpf(v[k].w, sc)));
+            As4[w][lane + 64 * k] = r;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // 2. prefix over 17 consecutive positions per lane, then across lanes
+        int x[SC_PER];
+#pragma unroll
+        for (int t = 0; t < SC_PER; ++t) x[t] = As[SC_PER * lane + t];
+        int run = 0;
+#pragma unroll
+        for (int t = 0; t < SC_PER; ++t) {
+            const int r = x[t];
+            x[t] = run;                               // exclusive within the lane
+            run += r;
+        }
+        const int base = sc_excl_scan(run);
+        const int tot = __shfl(base + run, 63, 64);
+        // 3. list prefixes (lanes 0..46): 46 gathers, then the doubled prefixes
+        if (lane < SC_LISTS) {
+            int g[SC_LIST];
+#pragma unroll
+            for (int k = 0; k < SC_LIST; ++k) g[k] = As[(lp[k >> 1] >> (16 * (k & 1))) & 0xFFFFu];
+            int tc = 0;
+#pragma unroll
+            for (int k = 0; k < SC_LIST; ++k) tc += g[k];
+            int e = 0;
+#pragma unroll
+            for (int k = 0; k < SC_LIST; ++k) {
+                Ds[lane * (SC_LIST + 1) + k] = 2 * e - tc;
+                e += g[k];
+            }
+            Ds[lane * (SC_LIST + 1) + SC_LIST] = tc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // 4. doubled strength prefixes over the column
+#pragma unroll
+        for (int t = 0; t < SC_PER; ++t) As[SC_PER * lane + t] = 2 * (base + x[t]) - tot;
+        if (lane == 63) As[NPR] = tot;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // 5. values
+        float* Cc = D.C + ((size_t)b * ncolp + col) * NPR;
+        const float unit = ldexpf(1.0f, -(sc + 1));
+#pragma unroll
+        for (int t = 0; t < SC_PER; ++t) {
+            const int j = 64 * t + lane;
+            const int two_c = As[h0[t] & 0xFFFFu] + As[h0[t] >> 16] - Ds[h1[t] & 0xFFFFu] - Ds[h1[t] >> 16] -
+                              Ds[h2[t] & 0xFFFFu] - Ds[h2[t] >> 16];
+            Cc[j] = (h0[t] >> 16) != 0 ? (float)two_c * unit : 0.0f;   // ge = 0 marks padding
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 }  // namespace
 
 bool sd_program(const uint8_t* cards, const uint16_t* grank, int n, std::vector<uint32_t>* out, uint32_t hdr[4]) {
@@ -365,6 +503,62 @@ bool sd_program(const uint8_t* cards, const uint16_t* grank, int n, std::vector<
             for (int x = 0; x < REC; ++x) P[at + (size_t)REC * (1 + r) + x] = visits[v].recs[r].w[x];
     }
     return true;
+}
+
+
+bool sc_tables(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* htab, uint32_t* ltab) {
+    if (n != 1081) return false;
+    for (int j = 0; j < n; ++j)
+        if (cards[2 * j] >= 52 || cards[2 * j + 1] >= 52 || cards[2 * j] == cards[2 * j + 1]) return false;
+    for (int j = 1; j < n; ++j)
+        if (grank[j] < grank[j - 1]) return false;
+    // live cards in ascending order -> list index; each list: its combos' positions, ascending
+    int lid[52];
+    for (int c = 0; c < 52; ++c) lid[c] = -1;
+    std::vector<int> list[52];
+    for (int j = 0; j < n; ++j) {
+        list[cards[2 * j]].push_back(j);
+        list[cards[2 * j + 1]].push_back(j);
+    }
+    int nl = 0;
+    for (int c = 0; c < 52; ++c)
+        if (!list[c].empty()) {
+            if ((int)list[c].size() != SC_LIST) return false;
+            lid[c] = nl++;
+        }
+    if (nl != SC_LISTS) return false;
+    for (size_t i = 0; i < (size_t)3 * NPR; ++i) htab[i] = 0;
+    for (size_t i = 0; i < (size_t)SC_LWORDS * 64; ++i) ltab[i] = 0;
+    for (int c = 0; c < 52; ++c) {
+        if (lid[c] < 0) continue;
+        for (int k = 0; k < SC_LIST; ++k)
+            ltab[(k >> 1) * 64 + lid[c]] |= (uint32_t)list[c][k] << (16 * (k & 1));
+    }
+    // entries of card c's list before position x
+    auto below = [&](int c, int x) {
+        return (int)(std::lower_bound(list[c].begin(), list[c].end(), x) - list[c].begin());
+    };
+    int gs = 0;
+    for (int j = 0; j < n; ++j) {
+        if (j > 0 && grank[j] != grank[j - 1]) gs = j;
+        int ge = j + 1;
+        while (ge < n && grank[ge] == grank[j]) ++ge;
+        htab[j] = (uint32_t)gs | (uint32_t)ge << 16;
+        for (int s = 0; s < 2; ++s) {
+            const int c = cards[2 * j + s];
+            const uint32_t lo = (uint32_t)(lid[c] * (SC_LIST + 1) + below(c, gs));
+            const uint32_t hi = (uint32_t)(lid[c] * (SC_LIST + 1) + below(c, ge));
+            htab[(size_t)(1 + s) * NPR + j] = lo | hi << 16;
+        }
+    }
+    return true;
+}
+
+void sc_launch(const BatchedDev& D, const uint32_t* htab, const uint32_t* ltab, int nboards, int ncol,
+               hipStream_t st) {
+    const int per = SC_WAVES * SC_CPW;
+    const dim3 g((ncol + per - 1) / per, nboards);
+    hipLaunchKernelGGL(k_sd_scan, g, dim3(64 * SC_WAVES), 0, st, D, htab, ltab, ncol);
 }
 
 void sd_launch(const BatchedDev& D, const uint32_t* prog, const uint4* hdrs, const int* colinfo, int nboards, int ncol,

@@ -1194,7 +1194,9 @@ struct BatchedState {
     int dm[2] = {8, 8};         // register-stack depth per level (4, 6 or 8)
     JitSweeps jit;              // generated per-template sweeps (when enabled)
     int gemm_shape = 100;       // river GEMM form and shape (autotuned; see river_gemm)
-    bool sd_sweep = false;      // river showdown by the strength-order sweep (else the GEMM)
+    int sd_mode = 0;            // river showdown: 0 GEMM, 1 strength-order sweep, 2 prefix scan
+    uint32_t* d_schtab = nullptr;   // prefix scan: [boards1][3][NPR] per-combo group and list indices
+    uint32_t* d_scltab = nullptr;   // prefix scan: [boards1][SC_LWORDS][64] card lists
     uint32_t* d_sdprog = nullptr;   // sweep programs of the river boards (SD_REC-word records)
     uint32_t* d_sdhdr = nullptr;    // [boards1][4] first record and first three blocks of each board's program
     int* d_colinfo = nullptr;       // [ncol] terminal | player << 31 of each river showdown column
@@ -1465,13 +1467,16 @@ std::vector<JitLevel> jit_levels(const LevelHost& H0, const LevelHost& H1, bool 
 int river_gemm_autotune(BatchedState& B, hipStream_t st);
 
 // River showdown by the sign x blocker GEMM (autotuned form, or
-// MTT_RIVER_GEMM=<shape code>), or by the exact strength-order sweep when
-// MTT_RIVER_SD=sweep (and no GEMM form is forced).
-bool river_sweep_mode() {
+// MTT_RIVER_GEMM=<shape code>), by the exact strength-order sweep
+// (MTT_RIVER_SD=sweep) or by prefix scans (MTT_RIVER_SD=scan); a forced
+// GEMM form wins.  0 GEMM, 1 sweep, 2 scan.
+int river_sd_mode() {
     const char* g = getenv("MTT_RIVER_GEMM");
-    if (g && *g) return false;
+    if (g && *g) return 0;
     const char* m = getenv("MTT_RIVER_SD");
-    return m && strcmp(m, "sweep") == 0;
+    if (m && strcmp(m, "sweep") == 0) return 1;
+    if (m && strcmp(m, "scan") == 0) return 2;
+    return 0;
 }
 
 int batched_create(SolverImpl& s) {
@@ -1594,10 +1599,23 @@ int batched_create(SolverImpl& s) {
         if (rc) return rc;
         B.D.grank = d_rank; B.D.cx = d_cx; B.D.cy = d_cy; B.D.rmap = d_rmap;
         size_t nbc = (size_t)nb1 * L1.ncolp * NPR;
-        B.sd_sweep = river_sweep_mode();
+        B.sd_mode = river_sd_mode();
         rc = A.alloc(nbc, &B.D.C);
         if (rc) return rc;
-        if (B.sd_sweep) {
+        if (B.sd_mode == 2) {
+            // FP32 showdown columns and the per-board scan tables
+            rc = A.alloc(nbc, &B.D.Bf);
+            if (rc) return rc;
+            MTT_HIP(hipMemsetAsync(B.D.Bf, 0, nbc * 4, s.stream));
+            std::vector<uint32_t> ht((size_t)nb1 * 3 * NPR), lt((size_t)nb1 * SC_LWORDS * 64);
+            for (int bi = 0; bi < nb1; ++bi)
+                MTT_ARG(sc_tables(&cc1[(size_t)bi * NPR * 2], &rank1[(size_t)bi * NPR], nlive1[bi],
+                                  &ht[(size_t)bi * 3 * NPR], &lt[(size_t)bi * SC_LWORDS * 64]),
+                        "river board %d: not 1081 combos in strength order", bi);
+            rc = A.upload(ht, &B.d_schtab);
+            rc = rc ? rc : A.upload(lt, &B.d_scltab);
+            if (rc) return rc;
+        } else if (B.sd_mode == 1) {
             // FP32 showdown columns, the per-board sweep programs and the column -> (terminal, player) map
             rc = A.alloc(nbc, &B.D.Bf);
             if (rc) return rc;
@@ -1732,11 +1750,11 @@ int batched_create(SolverImpl& s) {
     if (bt_jit_enabled()) {
         const std::vector<JitLevel> lv = jit_levels(H0, H1, two);
         JitConfig jc = bt_jit_config();
-        jc.sd_fp32 = B.sd_sweep ? 1 : 0;
+        jc.sd_fp32 = B.sd_mode ? 1 : 0;
         rc = bt_jit_build(lv.data(), (int)lv.size(), jc, &B.jit);
         if (rc) return rc;
     }
-    if (!B.sd_sweep) {
+    if (!B.sd_mode) {
         rc = river_gemm_autotune(B, s.stream);
         if (rc) return rc;
     }
@@ -1887,7 +1905,9 @@ int batched_run(SolverImpl& s, int iters) {
                       else if (B.dm[1] == 4) hipLaunchKernelGGL((k_bt_forward<1, 4, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr);
                       else hipLaunchKernelGGL((k_bt_forward<1, 8, SWEEP_NT>), g, dim3(SWEEP_NT), sh, s.stream, B.D, ranges, ws, nullptr));
         }
-        if (B.sd_sweep) {
+        if (B.sd_mode == 2) {
+            MTT_TIMED(s, "river_showdown", sc_launch(B.D, B.d_schtab, B.d_scltab, B.nb1, B.D.L[1].ncol, s.stream));
+        } else if (B.sd_mode == 1) {
             MTT_TIMED(s, "river_showdown",
                       sd_launch(B.D, B.d_sdprog, reinterpret_cast<const uint4*>(B.d_sdhdr), B.d_colinfo, B.nb1,
                                 B.D.L[1].ncol, s.stream));
@@ -1995,7 +2015,7 @@ int hu_sweep_source(const mtt_hu_game& g, std::string* src) {
     if (rc) return rc;
     const std::vector<JitLevel> lv = jit_levels(H0, H1, two);
     JitConfig jc = bt_jit_config();
-    jc.sd_fp32 = river_sweep_mode() ? 1 : 0;
+    jc.sd_fp32 = river_sd_mode() ? 1 : 0;
     *src = bt_jit_source(lv.data(), (int)lv.size(), jc);
     return MTT_OK;
 }
@@ -2525,6 +2545,14 @@ struct mtt_batch {
 };
 
 extern "C" {
+
+int mtt_river_scan_tables(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* htab, uint32_t* ltab) {
+    using namespace mtt;
+    MTT_ARG(cards && grank && htab && ltab, "mtt_river_scan_tables: null argument");
+    MTT_ARG(bt::sc_tables(cards, grank, n, htab, ltab),
+            "mtt_river_scan_tables: %d combos (need the 1081 of a river board), bad cards or unsorted ranks", n);
+    return MTT_OK;
+}
 
 int mtt_river_sweep_program(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* out, size_t cap,
                             size_t* len) {

@@ -58,6 +58,7 @@ for name, args in {
     "mtt_hu_sweep_compile": [_P, _P, _P],
     "mtt_solver_river_showdown": [_P, ctypes.c_int, _P, _P, _P, _P, _P, _P],
     "mtt_river_sweep_program": [_P, _P, ctypes.c_int, _P, ctypes.c_size_t, _P],
+    "mtt_river_scan_tables": [_P, _P, ctypes.c_int, _P, _P],
     "mtt_solver_get_state": [_P, _P, _P, ctypes.c_size_t],
     "mtt_solver_timing": [_P, _I],
     "mtt_solver_kernel_times": [_P, _I, ctypes.c_char_p, _I, _P, _P],

@@ -194,10 +194,10 @@ def test_dropin_kernels_manual_iteration(solver_api, oracle, torch_dev, mtt, str
         assert np.abs(Sg - Sc).max() < 1e-4 * max(1.0, np.abs(Sc).max()), t
 
 
-@pytest.mark.parametrize("river", ["gemm", "sweep"])
+@pytest.mark.parametrize("river", ["gemm", "sweep", "scan"])
 def test_batched_config1_matches_oracle(solver_api, oracle, torch_dev, river, monkeypatch):
     """Batched path on the river subgame, river showdown by the BF16 hi+lo
-    GEMM (default) or by the exact strength-order sweep."""
+    GEMM, by the exact strength-order sweep or by prefix scans."""
     monkeypatch.delenv("MTT_RIVER_GEMM", raising=False)
     monkeypatch.setenv("MTT_RIVER_SD", river)
     g, r0, r1 = solver_api.config1(0)
@@ -248,7 +248,8 @@ def test_generated_sweeps_match_interpreted(solver_api, torch_dev, which, monkey
     b.close()
 
 
-def test_batched_config3_matches_oracle(solver_api, oracle, torch_dev):
+@pytest.mark.parametrize("river", ["gemm", "scan"])
+def test_batched_config3_matches_oracle(solver_api, oracle, torch_dev, river, monkeypatch):
     """The benchmark's turn+river game (33%/75%/2.5x/all-in per street, nine
     river lines) on the batched path with generated sweeps, against the FP32
     oracle.  Iteration 1: every regret and strategy sum within tolerance.
@@ -259,9 +260,12 @@ def test_batched_config3_matches_oracle(solver_api, oracle, torch_dev):
     combo's regrets by O(pot) there and at its ancestors (and its strategy
     sum there).  So at most 1e-7 of the entries may differ, and each must
     belong to a combo with an iteration-1 positive-regret sum in (0, 1e-6] in
-    either implementation."""
+    either implementation.  River showdown by the BF16 hi+lo GEMM or by the
+    exact prefix scans."""
+    monkeypatch.delenv("MTT_RIVER_GEMM", raising=False)
+    monkeypatch.setenv("MTT_RIVER_SD", river)
     g, r0, r1 = solver_api.config3(0)
-    print("config3 batched bf16-split vs oracle, 2 iterations", flush=True)
+    print(f"config3 batched ({river} river showdown) vs oracle, 2 iterations", flush=True)
     s = solver_api.Solver(g, r0, r1, solver_api.MODE_BF16_SPLIT)
     assert s.jit_info()[0]
     h32, nodes, ni = _setup(solver_api, oracle, g, r0, r1, 1)
@@ -355,16 +359,21 @@ def test_plain_config3_matches_oracle(solver_api, oracle, torch_dev):
     b.close()
 
 
+@pytest.mark.parametrize("mode", ["sweep", "scan"])
 @pytest.mark.parametrize("which", ["config1", "config3_small", "config3"])
-def test_river_sweep_matches_dense_showdown(solver_api, torch_dev, which, monkeypatch):
+def test_river_sweep_matches_dense_showdown(solver_api, torch_dev, which, mode, monkeypatch):
     """River showdown values by the strength-order sweep (MTT_RIVER_SD=sweep)
-    against S r of the same columns in FP64, S = sign x blocker of the
-    board's strength order (the reference's showdown, equity matrix x reach,
+    or the prefix scans (MTT_RIVER_SD=scan) against S r of the same columns
+    in FP64, S = sign x blocker of the board's strength order (the
+    reference's showdown, equity matrix x reach,
     honest_multistreet_solver.cu:128-196, as 2 (E - 1/2) r).  The sweep sums
     in exact 2^48 fixed point and rounds once, so every value agrees to FP32
-    rounding (|err| <= 2^-23 |exact| + 1e-12); padded combos stay 0."""
+    rounding (|err| <= 2^-23 |exact| + 1e-12).  The scan sums exactly in
+    integer units u = 2^(e - 27) of the column (2^e > the column total), so
+    each value is off by at most half a unit per live combo plus the final
+    rounding, and on average by far less.  Padded combos stay 0."""
     monkeypatch.delenv("MTT_RIVER_GEMM", raising=False)
-    monkeypatch.setenv("MTT_RIVER_SD", "sweep")
+    monkeypatch.setenv("MTT_RIVER_SD", mode)
     S = solver_api
     g, r0, r1 = getattr(S, which)(1 if which == "config3_small" else 0)
     s = S.Solver(g, r0, r1, S.MODE_BF16_SPLIT)
@@ -382,10 +391,16 @@ def test_river_sweep_matches_dense_showdown(solver_api, torch_dev, which, monkey
         R = np.rint(reach[:, :nl].astype(np.float64) * 2.0 ** 48)
         exact = (R @ Sm.T) * 2.0 ** -48
         err = np.abs(vals[:, :nl].astype(np.float64) - exact)
-        print(f"{which} board {board}: {nl} combos, {reach.shape[0]} columns, max |value| {np.abs(exact).max():.4g}, "
-              f"max err {err.max():.3g}", flush=True)
+        print(f"{which} board {board} ({mode}): {nl} combos, {reach.shape[0]} columns, max |value| "
+              f"{np.abs(exact).max():.4g}, max err {err.max():.3g}", flush=True)
         assert np.abs(exact).max() > 0
-        assert np.all(err <= 2.0 ** -23 * np.abs(exact) + 1e-12), (board, float(err.max()))
+        if mode == "sweep":
+            assert np.all(err <= 2.0 ** -23 * np.abs(exact) + 1e-12), (board, float(err.max()))
+        else:
+            tot = reach[:, :nl].astype(np.float64).sum(axis=1)
+            unit = np.where(tot > 0, 2.0 ** (np.floor(np.log2(np.maximum(tot, 1e-300))) + 1 - 27), 0.0)[:, None]
+            assert np.all(err <= nl * unit + 2.0 ** -23 * np.abs(exact) + 1e-12), (board, float(err.max()))
+            assert err.mean() <= 4.0 * unit.max() + 1e-12
         assert not np.any(vals[:, nl:])
     s.close()
 

@@ -201,3 +201,89 @@ def test_sweep_program_rejects_bad_input(lib):
     assert lib.mtt_river_sweep_program(same.ctypes.data, ok_rank.ctypes.data, 2, None, 0, ctypes.byref(ln)) != 0
     assert lib.mtt_river_sweep_program(cards.ctypes.data, ok_rank.ctypes.data, 1089, None, 0,
                                        ctypes.byref(ln)) != 0
+
+
+# ------------------------------------------------------- prefix scan -----
+# The scan form (MTT_RIVER_SD=scan): per column, doubled strength prefixes
+# DP[x] = 2 P(x) - Ptot and, per live card, doubled prefixes over its 46
+# combos D_c[k] = 2 ex_c(k) - tot_c; a hero's doubled value is six lookups.
+# The replay below follows the kernel's arithmetic on integer columns.
+
+def scan_tables(lib, cards, grank):
+    lib.mtt_river_scan_tables.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_int] + [ctypes.c_void_p] * 2
+    c = np.ascontiguousarray(cards, np.uint8).reshape(-1)
+    g = np.ascontiguousarray(grank, np.uint16)
+    h = np.zeros((3, 1088), np.uint32)
+    lt = np.zeros((23, 64), np.uint32)
+    rc = lib.mtt_river_scan_tables(c.ctypes.data, g.ctypes.data, len(grank), h.ctypes.data, lt.ctypes.data)
+    return rc, h, lt
+
+
+def replay_scan(h, lt, n, R):
+    ncol = R.shape[0]
+    Rp = np.zeros((ncol, 1088), np.int64)
+    Rp[:, :n] = R
+    P = np.zeros((ncol, 1089), np.int64)
+    P[:, 1:] = np.cumsum(Rp, axis=1)
+    tot = P[:, 1088:1089]
+    DP = 2 * P - tot
+    pos = np.zeros((47, 46), np.int64)
+    for k in range(46):
+        pos[:, k] = (lt[k >> 1, :47] >> (16 * (k & 1))) & 0xFFFF
+    g = Rp[:, pos]                                     # [ncol][47][46]
+    ex = np.zeros((ncol, 47, 47), np.int64)
+    ex[:, :, 1:] = np.cumsum(g, axis=2)
+    Dl = (2 * ex - ex[:, :, 46:47]).reshape(ncol, 47 * 47)
+    lo16 = lambda w: (w & 0xFFFF).astype(np.int64)
+    hi16 = lambda w: (w >> 16).astype(np.int64)
+    two_c = (DP[:, lo16(h[0])] + DP[:, hi16(h[0])] - Dl[:, lo16(h[1])] - Dl[:, hi16(h[1])]
+             - Dl[:, lo16(h[2])] - Dl[:, hi16(h[2])])
+    assert np.all(two_c % 2 == 0)
+    assert np.all(hi16(h[0])[n:] == 0) and np.all(hi16(h[0])[:n] > 0)   # ge = 0 marks padding only
+    return two_c[:, :n] // 2
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_scan_matches_dense_on_river_boards(lib, seed):
+    rng = np.random.default_rng(seed + 10)
+    board = [int(x) for x in rng.choice(52, 5, replace=False)]
+    cards, grank = river_board(lib, board)
+    rc, h, lt = scan_tables(lib, cards, grank)
+    assert rc == 0
+    R = columns(rng, 1081, 6)
+    assert np.array_equal(replay_scan(h, lt, 1081, R), dense(cards, grank, R))
+
+
+@pytest.mark.parametrize("kind", ["one_group", "no_ties", "random", "pairs"])
+def test_scan_matches_dense_synthetic_groups(lib, kind):
+    """The 1081 combos of a river board in a shuffled order with synthetic
+    tie structures: every combo tied, no ties, random group sizes, groups of
+    two (which put two combos of one card in a group)."""
+    rng = np.random.default_rng(len(kind))
+    board = [3, 17, 28, 40, 51]
+    live = [c for c in range(52) if c not in board]
+    cards = np.array(list(itertools.combinations(live, 2)), np.int64)[rng.permutation(1081)]
+    if kind == "one_group":
+        grank = np.zeros(1081, np.int64)
+    elif kind == "no_ties":
+        grank = np.arange(1081, dtype=np.int64)
+    elif kind == "pairs":
+        grank = np.arange(1081, dtype=np.int64) // 2
+    else:
+        grank = np.sort(rng.integers(0, 150, 1081))
+    rc, h, lt = scan_tables(lib, cards, grank)
+    assert rc == 0
+    R = columns(rng, 1081, 4)
+    assert np.array_equal(replay_scan(h, lt, 1081, R), dense(cards, grank, R))
+
+
+def test_scan_tables_reject_bad_input(lib):
+    cards, grank = river_board(lib, [0, 9, 18, 27, 36])
+    assert scan_tables(lib, cards, grank)[0] == 0
+    assert scan_tables(lib, cards[:1080], grank[:1080])[0] != 0          # not a river board's 1081
+    bad = grank.copy()
+    bad[5], bad[900] = bad[900], bad[5]
+    assert scan_tables(lib, cards, bad)[0] != 0                           # ranks not sorted
+    dup = cards.copy()
+    dup[7] = dup[8]                                                       # a card list of 47 and one of 45
+    assert scan_tables(lib, dup, grank)[0] != 0
