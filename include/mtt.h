@@ -386,11 +386,15 @@ int mtt_solver_river_showdown(mtt_solver* h, int board, float* reach, float* val
                               uint16_t* grank, int* nlive, int* ncol);
 /* The per-board tables of the prefix-scan river showdown (MTT_RIVER_SD=scan)
  * for the 1081 combos of a river board in strength order (cards[1081][2],
- * non-decreasing grank[1081]): htab[3][1088] = per position gs | ge << 16,
- * then for each of its two cards (list * 47 + lo) | (list * 47 + hi) << 16,
- * [lo, hi) being the entries of the card's 46-combo list inside the
- * position's tie group; ltab[23][64] = per card list (lane), the positions
- * of its 46 combos two per word.  Host only. */
+ * non-decreasing grank[1081]), as byte offsets (4 x index) into the kernel's
+ * LDS arrays: htab[3][1088] = per position 4 gs | 4 ge << 16, then for each
+ * of its two cards 4 (list * 47 + lo) | 4 (list * 47 + hi) << 16, [lo, hi)
+ * being the entries of the card's 46-combo list inside the position's tie
+ * group (padding positions: the zero slots 4 * 1089 and 4 * 2209);
+ * ltab[2][12][64] = 4 x the positions of each live card's
+ * 46 combos (cards in ascending order, list c in round c / 32 on lanes
+ * 2 (c % 32) for entries 0..22 and 2 (c % 32) + 1 for 23..45), two per word.
+ * Host only. */
 int mtt_river_scan_tables(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* htab, uint32_t* ltab);
 int mtt_solver_destroy(mtt_solver* h);
 

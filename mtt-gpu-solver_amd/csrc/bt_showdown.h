@@ -58,15 +58,20 @@ constexpr int SD_PAD_RECORDS = SD_VISIT;
 // hero i (cards a, b, tie group [gs, ge), whose group covers entries
 // [lo_c, hi_c) of card c's list) has
 //     2 C[i] = DP[gs] + DP[ge] - D_a[lo_a] - D_a[hi_a] - D_b[lo_b] - D_b[hi_b]
-// exactly.  Per board the tables hold, per combo position j, gs | ge << 16,
-// (a's list * 47 + lo_a) | (a's list * 47 + hi_a) << 16 and the same for b
-// (htab[3][NPR]), and per live card the positions of its 46 combos, two per
-// word (ltab[23][64], lane = the card's list index).  River boards only:
+// exactly.  Per board the tables hold LDS byte offsets (4 x index): per combo
+// position j, 4 gs | 4 ge << 16, 4 (a's list * 47 + lo_a) | 4 (a's list * 47
+// + hi_a) << 16 and the same for b (htab[3][NPR]; padding positions point at
+// zero slots: 4 (NPR + 1) and 4 * 47 * 47), and per live card its 46 combos'
+// positions, two per word: ltab[2][12][64], list c in round c / 32 on lanes
+// 2 (c % 32) (entries 0..22) and 2 (c % 32) + 1 (entries 23..45).  River
+// boards only:
 // n = 1081 combos of 47 live cards.  Returns false otherwise, or if a card is
 // invalid or the ranks are not sorted.
 constexpr int SC_LISTS = 47;
 constexpr int SC_LIST = 46;
-constexpr int SC_LWORDS = SC_LIST / 2;
+constexpr int SC_LHALF = SC_LIST / 2;                       // entries per lane
+constexpr int SC_LHALF_WORDS = (SC_LHALF + 1) / 2;          // 12 words per lane and round
+constexpr int SC_LTAB_WORDS = 2 * SC_LHALF_WORDS * 64;      // per board
 bool sc_tables(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* htab, uint32_t* ltab);
 // Launches the scan for every board of level 1: columns from D.Bf, values
 // to D.C (0 on padding); htab / ltab: the boards' tables, back to back.

@@ -253,9 +253,11 @@ __global__ __launch_bounds__(64) void k_sd_sweep(BatchedDev D, const uint32_t* _
 
 // ------------------------------------------------------ prefix scan --
 constexpr int SC_PER = NPR / 64;            // 17 combo positions per lane
-constexpr int SC_WAVES = 4;                 // waves per workgroup (each its own columns and LDS)
-constexpr int SC_CPW = 4;                   // columns per wave
+constexpr int SC_CPW = 16;                  // columns per wave (one wave per workgroup; 16-32 measure alike,
+                                            // 2-8 pay the per-wave table and list set-up more often)
+constexpr int SC_AZ = NPR + 1;              // As slot that stays 0 (padding positions read it)
 constexpr int SC_D = SC_LISTS * (SC_LIST + 1);
+constexpr int SC_DZ = SC_D;                 // Ds slot that stays 0
 static_assert(NPR % 64 == 0 && NPR % 4 == 0, "combo padding");
 
 __device__ __forceinline__ int sc_excl_scan(int v) {
@@ -275,50 +277,65 @@ __device__ __forceinline__ float sc_sum(float v) {
     return v;
 }
 
-// One workgroup of SC_WAVES waves per (board, SC_WAVES x SC_CPW columns); a
-// wave takes SC_CPW columns in turn with its board's tables in registers:
-//   1. the column (FP32, 16-byte loads) and its total; R = rint(r 2^sc),
-//      sc = 27 - exponent of the total, written to LDS;
-//   2. each lane reads 17 consecutive positions: local scan, wave scan of
-//      the lane totals;
-//   3. lanes 0..46 gather their card's 46 combos from LDS and write the
-//      doubled list prefixes D_c;
-//   4. the doubled strength prefixes DP overwrite the column in LDS;
-//   5. each lane takes positions lane, lane + 64, ..: six LDS lookups, one
-//      conversion, coalesced stores.
-__global__ __launch_bounds__(64 * SC_WAVES) void k_sd_scan(BatchedDev D, const uint32_t* __restrict__ htab,
-                                                           const uint32_t* __restrict__ ltab, int ncol) {
-    __shared__ int4 As4[SC_WAVES][(NPR + 4) / 4];
-    __shared__ int Ds_[SC_WAVES][SC_D + 3];
+// LDS word at a byte offset (the tables hold byte offsets, so each lookup is
+// one mask or shift and the array's base is the instruction's immediate).
+__device__ __forceinline__ int sc_at(const int* base, uint32_t byte_off) {
+    return *reinterpret_cast<const int*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// One wave per (board, SC_CPW columns), the columns in turn:
+//   1. the column (FP32, 16-byte loads, requested while the previous column
+//      was finishing) and its total; R = floor(r 2^sc + 1/2), sc = 27 -
+//      exponent of the total, written to LDS;
+//   2. card lists in two rounds of 32 lists, two lanes per list (23 entries
+//      each, LDS byte offsets held in registers): gathers, prefix, the
+//      pair's total by a lane swap, doubled list prefixes D_c to LDS;
+//   3. each lane reads 17 consecutive positions: local scan, wave scan of
+//      the lane totals; the doubled strength prefixes DP overwrite the
+//      column in LDS;
+//   4. each lane takes positions lane, lane + 64, .. (their table words held
+//      in registers for all the wave's columns): six LDS lookups, one
+//      conversion, coalesced stores (padding positions look up zero slots).
+template <int CPW>
+__global__ __launch_bounds__(64) void k_sd_scan(BatchedDev D, const uint32_t* __restrict__ htab,
+                                                const uint32_t* __restrict__ ltab, int ncol) {
+    __shared__ int4 As4[(NPR + 4) / 4];
+    __shared__ int Ds[SC_D + 4];
+    int* As = reinterpret_cast<int*>(As4);
     const LevelDev& L = D.L[1];
-    const int b = blockIdx.y, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.y, lane = threadIdx.x;
     const int ncolp = L.ncolp;
-    int* As = reinterpret_cast<int*>(As4[w]);
-    int* Ds = Ds_[w];
-    uint32_t h0[SC_PER], h1[SC_PER], h2[SC_PER], lp[SC_LWORDS];
+    const int cw0 = blockIdx.x * CPW;
+    const int ncw = min(CPW, ncol - cw0);
+    const uint32_t* ht = htab + (size_t)b * 3 * NPR;
+    uint32_t lp[2][SC_LHALF_WORDS];
     {
-        const uint32_t* ht = htab + (size_t)b * 3 * NPR;
+        const uint32_t* lt = ltab + (size_t)b * SC_LTAB_WORDS;
 #pragma unroll
-        for (int t = 0; t < SC_PER; ++t) {
-            h0[t] = ht[64 * t + lane];
-            h1[t] = ht[NPR + 64 * t + lane];
-            h2[t] = ht[2 * NPR + 64 * t + lane];
-        }
-        const uint32_t* lt = ltab + (size_t)b * SC_LWORDS * 64;
+        for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int k = 0; k < SC_LWORDS; ++k) lp[k] = lt[k * 64 + lane];
+            for (int k = 0; k < SC_LHALF_WORDS; ++k) lp[r][k] = lt[(r * SC_LHALF_WORDS + k) * 64 + lane];
     }
-    const int cw0 = (blockIdx.x * SC_WAVES + w) * SC_CPW;
-    for (int q = 0; q < SC_CPW; ++q) {
-        const int col = cw0 + q;
-        if (col >= ncol) break;                       // wave-uniform; no workgroup barriers below
-        const float4* rc = reinterpret_cast<const float4*>(D.Bf + ((size_t)b * ncolp + col) * NPR);
-        // 1. column, total, fixed point (padding entries are 0)
-        float4 v[5];
+    if (lane == 0) {
+        As[SC_AZ] = 0;
+        Ds[SC_DZ] = 0;
+    }
+    const int half = lane & 1;
+    // the column's 272 float4, five per lane (the fifth read on every lane, kept on lanes 0..15)
+    const float* Bcol = D.Bf + ((size_t)b * ncolp + cw0) * NPR;
+    float4 v[5];
+    auto fetch = [&](int q) {
+        const float4* rc = reinterpret_cast<const float4*>(Bcol + (size_t)q * NPR);
 #pragma unroll
         for (int k = 0; k < 4; ++k) v[k] = rc[lane + 64 * k];
-        v[4] = lane < (NPR / 4 - 256) ? rc[256 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane == 14) v[4].y = v[4].z = v[4].w = 0.0f;   // positions 1081..1087 are padding
+        v[4] = rc[256 + (lane & 15)];
+    };
+    fetch(0);
+    for (int q = 0; q < ncw; ++q) {
+        const int col = cw0 + q;
+        // 1. total and fixed point (positions 1081..1087 are padding)
+        if (lane >= 16) v[4] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane == 14) v[4].y = v[4].z = v[4].w = 0.0f;
         if (lane == 15) v[4] = make_float4(0.f, 0.f, 0.f, 0.f);
         float s = 0.0f;
 #pragma unroll
@@ -327,61 +344,73 @@ __global__ __launch_bounds__(64 * SC_WAVES) void k_sd_scan(BatchedDev D, const u
         int ex = 0;
         (void)frexpf(s, &ex);
         const int sc = s > 0.0f ? 27 - ex : 0;       // every R_j and their total below 2^27 (+ rounding)
+        const float up = ldexpf(1.0f, sc);
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            if (k == 4 && lane >= NPR / 4 - 256) break;
-            const int4 r = make_int4((int)rintf(ldexpf(v[k].x, sc)), (int)rintf(ldexpf(v[k].y, sc)),
-                                     (int)rintf(ldexpf(v[k].z, sc)), (int)rintf(ldexpf(v[k].w, sc)));
-            As4[w][lane + 64 * k] = r;
+            const int4 r = make_int4((int)fmaf(v[k].x, up, 0.5f), (int)fmaf(v[k].y, up, 0.5f),
+                                     (int)fmaf(v[k].z, up, 0.5f), (int)fmaf(v[k].w, up, 0.5f));
+            if (k < 4 || lane < 16) As4[lane + 64 * k] = r;
         }
+        if (q + 1 < ncw) fetch(q + 1);                // the next column arrives under this one's work
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        // 2. prefix over 17 consecutive positions per lane, then across lanes
-        int x[SC_PER];
+        // 2. card lists: round r holds lists 32 r + lane / 2, half lane & 1 (entries 23 half ..)
 #pragma unroll
-        for (int t = 0; t < SC_PER; ++t) x[t] = As[SC_PER * lane + t];
-        int run = 0;
+        for (int r = 0; r < 2; ++r) {
+            const int li = 32 * r + (lane >> 1);
+            int g[SC_LHALF];
 #pragma unroll
-        for (int t = 0; t < SC_PER; ++t) {
-            const int r = x[t];
-            x[t] = run;                               // exclusive within the lane
-            run += r;
-        }
-        const int base = sc_excl_scan(run);
-        const int tot = __shfl(base + run, 63, 64);
-        // 3. list prefixes (lanes 0..46): 46 gathers, then the doubled prefixes
-        if (lane < SC_LISTS) {
-            int g[SC_LIST];
+            for (int m = 0; m < SC_LHALF; ++m)
+                g[m] = sc_at(As, (m & 1) ? lp[r][m >> 1] >> 16 : lp[r][m >> 1] & 0xFFFFu);
+            int hs = 0;
 #pragma unroll
-            for (int k = 0; k < SC_LIST; ++k) g[k] = As[(lp[k >> 1] >> (16 * (k & 1))) & 0xFFFFu];
-            int tc = 0;
+            for (int m = 0; m < SC_LHALF; ++m) hs += g[m];
+            const int other = __shfl_xor(hs, 1, 64);
+            const int tc = hs + other;
+            int e = half ? other : 0;
+            if (li < SC_LISTS) {
+                int* Dl = Ds + li * (SC_LIST + 1) + SC_LHALF * half;
 #pragma unroll
-            for (int k = 0; k < SC_LIST; ++k) tc += g[k];
-            int e = 0;
-#pragma unroll
-            for (int k = 0; k < SC_LIST; ++k) {
-                Ds[lane * (SC_LIST + 1) + k] = 2 * e - tc;
-                e += g[k];
+                for (int m = 0; m < SC_LHALF; ++m) {
+                    Dl[m] = 2 * e - tc;
+                    e += g[m];
+                }
+                if (half) Dl[SC_LHALF] = tc;          // entry 46: the list total
             }
-            Ds[lane * (SC_LIST + 1) + SC_LIST] = tc;
+        }
+        // 3. prefix over 17 consecutive positions per lane, then across lanes; DP over the column
+        {
+            int x[SC_PER];
+#pragma unroll
+            for (int t = 0; t < SC_PER; ++t) x[t] = As[SC_PER * lane + t];
+            int run = 0;
+#pragma unroll
+            for (int t = 0; t < SC_PER; ++t) {
+                const int r = x[t];
+                x[t] = run;                           // exclusive within the lane
+                run += r;
+            }
+            const int base = sc_excl_scan(run);
+            const int tot = __shfl(base + run, 63, 64);
+            const int off = 2 * base - tot;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int t = 0; t < SC_PER; ++t) As[SC_PER * lane + t] = 2 * x[t] + off;
+            if (lane == 63) As[NPR] = tot;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        // 4. doubled strength prefixes over the column
-#pragma unroll
-        for (int t = 0; t < SC_PER; ++t) As[SC_PER * lane + t] = 2 * (base + x[t]) - tot;
-        if (lane == 63) As[NPR] = tot;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // 5. values
+        // 4. values
         float* Cc = D.C + ((size_t)b * ncolp + col) * NPR;
         const float unit = ldexpf(1.0f, -(sc + 1));
 #pragma unroll
         for (int t = 0; t < SC_PER; ++t) {
             const int j = 64 * t + lane;
-            const int two_c = As[h0[t] & 0xFFFFu] + As[h0[t] >> 16] - Ds[h1[t] & 0xFFFFu] - Ds[h1[t] >> 16] -
-                              Ds[h2[t] & 0xFFFFu] - Ds[h2[t] >> 16];
-            Cc[j] = (h0[t] >> 16) != 0 ? (float)two_c * unit : 0.0f;   // ge = 0 marks padding
+            const uint32_t w0 = ht[j], w1 = ht[NPR + j], w2 = ht[2 * NPR + j];
+            const int two_c = sc_at(As, w0 & 0xFFFFu) + sc_at(As, w0 >> 16) - sc_at(Ds, w1 & 0xFFFFu) -
+                              sc_at(Ds, w1 >> 16) - sc_at(Ds, w2 & 0xFFFFu) - sc_at(Ds, w2 >> 16);
+            Cc[j] = (float)two_c * unit;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -527,12 +556,21 @@ bool sc_tables(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* hta
             lid[c] = nl++;
         }
     if (nl != SC_LISTS) return false;
-    for (size_t i = 0; i < (size_t)3 * NPR; ++i) htab[i] = 0;
-    for (size_t i = 0; i < (size_t)SC_LWORDS * 64; ++i) ltab[i] = 0;
+    // padding positions: every lookup reads a zero slot (As[NPR + 1], Ds[47 * 47])
+    const uint32_t az = 4u * (NPR + 1), dz = 4u * (SC_LISTS * (SC_LIST + 1));
+    for (int j = 0; j < NPR; ++j) {
+        htab[j] = az | az << 16;
+        htab[NPR + j] = dz | dz << 16;
+        htab[2 * NPR + j] = dz | dz << 16;
+    }
+    for (size_t i = 0; i < (size_t)SC_LTAB_WORDS; ++i) ltab[i] = 0;
     for (int c = 0; c < 52; ++c) {
         if (lid[c] < 0) continue;
-        for (int k = 0; k < SC_LIST; ++k)
-            ltab[(k >> 1) * 64 + lid[c]] |= (uint32_t)list[c][k] << (16 * (k & 1));
+        const int r = lid[c] / 32, l2 = 2 * (lid[c] % 32);
+        for (int k = 0; k < SC_LIST; ++k) {
+            const int h = k / SC_LHALF, m = k % SC_LHALF;
+            ltab[(r * SC_LHALF_WORDS + (m >> 1)) * 64 + l2 + h] |= 4u * (uint32_t)list[c][k] << (16 * (m & 1));
+        }
     }
     // entries of card c's list before position x
     auto below = [&](int c, int x) {
@@ -543,12 +581,12 @@ bool sc_tables(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* hta
         if (j > 0 && grank[j] != grank[j - 1]) gs = j;
         int ge = j + 1;
         while (ge < n && grank[ge] == grank[j]) ++ge;
-        htab[j] = (uint32_t)gs | (uint32_t)ge << 16;
+        htab[j] = 4u * (uint32_t)gs | 4u * (uint32_t)ge << 16;
         for (int s = 0; s < 2; ++s) {
             const int c = cards[2 * j + s];
             const uint32_t lo = (uint32_t)(lid[c] * (SC_LIST + 1) + below(c, gs));
             const uint32_t hi = (uint32_t)(lid[c] * (SC_LIST + 1) + below(c, ge));
-            htab[(size_t)(1 + s) * NPR + j] = lo | hi << 16;
+            htab[(size_t)(1 + s) * NPR + j] = 4u * lo | 4u * hi << 16;
         }
     }
     return true;
@@ -556,9 +594,8 @@ bool sc_tables(const uint8_t* cards, const uint16_t* grank, int n, uint32_t* hta
 
 void sc_launch(const BatchedDev& D, const uint32_t* htab, const uint32_t* ltab, int nboards, int ncol,
                hipStream_t st) {
-    const int per = SC_WAVES * SC_CPW;
-    const dim3 g((ncol + per - 1) / per, nboards);
-    hipLaunchKernelGGL(k_sd_scan, g, dim3(64 * SC_WAVES), 0, st, D, htab, ltab, ncol);
+    const dim3 g((ncol + SC_CPW - 1) / SC_CPW, nboards);
+    hipLaunchKernelGGL(k_sd_scan<SC_CPW>, g, dim3(64), 0, st, D, htab, ltab, ncol);
 }
 
 void sd_launch(const BatchedDev& D, const uint32_t* prog, const uint4* hdrs, const int* colinfo, int nboards, int ncol,

@@ -1196,7 +1196,7 @@ struct BatchedState {
     int gemm_shape = 100;       // river GEMM form and shape (autotuned; see river_gemm)
     int sd_mode = 0;            // river showdown: 0 GEMM, 1 strength-order sweep, 2 prefix scan
     uint32_t* d_schtab = nullptr;   // prefix scan: [boards1][3][NPR] per-combo group and list indices
-    uint32_t* d_scltab = nullptr;   // prefix scan: [boards1][SC_LWORDS][64] card lists
+    uint32_t* d_scltab = nullptr;   // prefix scan: [boards1][2][12][64] card lists
     uint32_t* d_sdprog = nullptr;   // sweep programs of the river boards (SD_REC-word records)
     uint32_t* d_sdhdr = nullptr;    // [boards1][4] first record and first three blocks of each board's program
     int* d_colinfo = nullptr;       // [ncol] terminal | player << 31 of each river showdown column
@@ -1607,10 +1607,10 @@ int batched_create(SolverImpl& s) {
             rc = A.alloc(nbc, &B.D.Bf);
             if (rc) return rc;
             MTT_HIP(hipMemsetAsync(B.D.Bf, 0, nbc * 4, s.stream));
-            std::vector<uint32_t> ht((size_t)nb1 * 3 * NPR), lt((size_t)nb1 * SC_LWORDS * 64);
+            std::vector<uint32_t> ht((size_t)nb1 * 3 * NPR), lt((size_t)nb1 * SC_LTAB_WORDS);
             for (int bi = 0; bi < nb1; ++bi)
                 MTT_ARG(sc_tables(&cc1[(size_t)bi * NPR * 2], &rank1[(size_t)bi * NPR], nlive1[bi],
-                                  &ht[(size_t)bi * 3 * NPR], &lt[(size_t)bi * SC_LWORDS * 64]),
+                                  &ht[(size_t)bi * 3 * NPR], &lt[(size_t)bi * SC_LTAB_WORDS]),
                         "river board %d: not 1081 combos in strength order", bi);
             rc = A.upload(ht, &B.d_schtab);
             rc = rc ? rc : A.upload(lt, &B.d_scltab);

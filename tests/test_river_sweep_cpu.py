@@ -207,6 +207,8 @@ def test_sweep_program_rejects_bad_input(lib):
 # The scan form (MTT_RIVER_SD=scan): per column, doubled strength prefixes
 # DP[x] = 2 P(x) - Ptot and, per live card, doubled prefixes over its 46
 # combos D_c[k] = 2 ex_c(k) - tot_c; a hero's doubled value is six lookups.
+# Card lists: list c in round c // 32 on lanes 2 (c % 32) (entries 0..22) and
+# 2 (c % 32) + 1 (entries 23..45).
 # The replay below follows the kernel's arithmetic on integer columns.
 
 def scan_tables(lib, cards, grank):
@@ -214,7 +216,7 @@ def scan_tables(lib, cards, grank):
     c = np.ascontiguousarray(cards, np.uint8).reshape(-1)
     g = np.ascontiguousarray(grank, np.uint16)
     h = np.zeros((3, 1088), np.uint32)
-    lt = np.zeros((23, 64), np.uint32)
+    lt = np.zeros((2, 12, 64), np.uint32)
     rc = lib.mtt_river_scan_tables(c.ctypes.data, g.ctypes.data, len(grank), h.ctypes.data, lt.ctypes.data)
     return rc, h, lt
 
@@ -228,18 +230,26 @@ def replay_scan(h, lt, n, R):
     tot = P[:, 1088:1089]
     DP = 2 * P - tot
     pos = np.zeros((47, 46), np.int64)
-    for k in range(46):
-        pos[:, k] = (lt[k >> 1, :47] >> (16 * (k & 1))) & 0xFFFF
+    for c in range(47):
+        r, l2 = divmod(c, 32)
+        for k in range(46):
+            h_, m = divmod(k, 23)
+            pos[c, k] = ((int(lt[r, m >> 1, 2 * l2 + h_]) >> (16 * (m & 1))) & 0xFFFF) // 4
     g = Rp[:, pos]                                     # [ncol][47][46]
     ex = np.zeros((ncol, 47, 47), np.int64)
     ex[:, :, 1:] = np.cumsum(g, axis=2)
-    Dl = (2 * ex - ex[:, :, 46:47]).reshape(ncol, 47 * 47)
-    lo16 = lambda w: (w & 0xFFFF).astype(np.int64)
-    hi16 = lambda w: (w >> 16).astype(np.int64)
-    two_c = (DP[:, lo16(h[0])] + DP[:, hi16(h[0])] - Dl[:, lo16(h[1])] - Dl[:, hi16(h[1])]
+    Dl = np.zeros((ncol, 47 * 47 + 1), np.int64)       # + the zero slot
+    Dl[:, :47 * 47] = (2 * ex - ex[:, :, 46:47]).reshape(ncol, 47 * 47)
+    DPz = np.zeros((ncol, 1090), np.int64)             # DP[0..1088] + the zero slot 1089
+    DPz[:, :1089] = DP
+    assert np.all(h % 4 == 0) and np.all((h >> 16) % 4 == 0)
+    lo16 = lambda w: (w & 0xFFFF).astype(np.int64) // 4
+    hi16 = lambda w: (w >> 16).astype(np.int64) // 4
+    two_c = (DPz[:, lo16(h[0])] + DPz[:, hi16(h[0])] - Dl[:, lo16(h[1])] - Dl[:, hi16(h[1])]
              - Dl[:, lo16(h[2])] - Dl[:, hi16(h[2])])
     assert np.all(two_c % 2 == 0)
-    assert np.all(hi16(h[0])[n:] == 0) and np.all(hi16(h[0])[:n] > 0)   # ge = 0 marks padding only
+    assert np.all(two_c[:, n:] == 0)                    # padding positions read the zero slots
+    assert np.all(lo16(h[0])[n:] == 1089) and np.all(hi16(h[0])[:n] <= 1088)
     return two_c[:, :n] // 2
 
 
