@@ -101,7 +101,8 @@ uint32_t mtt_eval7(const uint8_t cards[7]);
 int mtt_eval7_host_batch(const uint8_t* cards, int64_t n, uint32_t* out);
 /* Device batch: d_cards n*7 bytes, d_out n scores. */
 int mtt_eval7_batch(const uint8_t* d_cards, int64_t n, uint32_t* d_out, mtt_stream_t stream);
-/* get_canonical_class, mtt_ev_dumper.cu:26-32. */
+/* get_canonical_class, mtt_ev_dumper.cu:26-32; -1 (with mtt_last_error)
+ * unless c1, c2 are two distinct cards below 52. */
 int mtt_canonical_class(int c1, int c2);
 /* generate_all_1755_flops, mtt_ev_dumper.cu:34-51: out 1755*4 bytes
  * {c1,c2,c3,orbit_weight}; returns the flop count. */

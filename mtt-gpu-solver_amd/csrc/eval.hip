@@ -32,6 +32,9 @@ void set_error(const char* fmt, ...) {
 
 int hip_fail(hipError_t e, const char* what, const char* file, int line) {
     set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+    // the runtime keeps a failed call's error as the thread's last error; it is reported here, so clear
+    // it (else the next hipGetLastError after a good launch would return it)
+    (void)hipGetLastError();
     return e == hipErrorOutOfMemory ? MTT_ERR_OOM : MTT_ERR_HIP;
 }
 
@@ -660,6 +663,10 @@ int mtt_eval7_batch(const uint8_t* d_cards, int64_t n, uint32_t* d_out, hipStrea
 }
 
 int mtt_canonical_class(int c1, int c2) {
+    if (c1 < 0 || c1 >= 52 || c2 < 0 || c2 >= 52 || c1 == c2) {
+        set_error("mtt_canonical_class: cards %d, %d are not two distinct cards below 52", c1, c2);
+        return -1;
+    }
     int r1 = c1 >> 2, s1 = c1 & 3, r2 = c2 >> 2, s2 = c2 & 3;
     int hi = r1 > r2 ? r1 : r2, lo = r1 < r2 ? r1 : r2;
     if (r1 == r2) return 12 - hi;

@@ -2599,6 +2599,7 @@ int mtt_hu_sweep_compile(const mtt_hu_game* game, size_t* code_bytes, double* se
 }
 
 int mtt_batch_create(const mtt_hu_game* games, int n, const float* ranges, const mtt_solver_cfg* cfg, mtt_batch** out) {
+    (void)hipGetLastError();   // an error left by an earlier call is not this call's
     using namespace mtt;
     MTT_ARG(games && ranges && out, "mtt_batch_create: null pointer");
     MTT_ARG(n >= 1 && n <= 65535, "mtt_batch_create: instance count %d out of 1..65535", n);

@@ -242,7 +242,8 @@ int build_hu_tree(const mtt_hu_game& g, HuTree& out, bool flat) {
         for (int k = 0; k < m.n_sizes; ++k)
             MTT_ARG(m.sizes[k] > 0 && std::isfinite(m.sizes[k]), "hu game: bad bet size");
         MTT_ARG(m.raise_cap >= 0 && m.raise_cap <= 8, "hu game: raise cap out of range");
-        MTT_ARG(m.raise_mult == 0 || m.raise_mult > 1, "hu game: raise multiple must exceed 1");
+        MTT_ARG(m.raise_mult == 0 || (m.raise_mult > 1 && std::isfinite(m.raise_mult)),
+                "hu game: raise multiple must be 0 or a finite value above 1");
     }
     out.first_street = first;
     out.board_cards = g.n_board;

@@ -731,8 +731,19 @@ int mtt_icm_bubble_factors_fp64(const double* stacks, int n, const double* payou
 
 int mtt_multi_create(const mtt_4way_game* game, int n_flops, const uint8_t* flops, const float* ranges,
                      const float* bubble_factors, const mtt_solver_cfg* cfg, mtt_multi** out) {
+    (void)hipGetLastError();   // an error left by an earlier call is not this call's
     MTT_ARG(game && flops && ranges && bubble_factors && out, "mtt_multi_create: null pointer");
     MTT_ARG(n_flops >= 1 && n_flops <= 65535, "mtt_multi_create: n_flops %d out of 1..65535", n_flops);
+    for (int i = 0; i < 16; ++i)
+        MTT_ARG(std::isfinite(bubble_factors[i]) && bubble_factors[i] > 0.0f,
+                "mtt_multi_create: bubble factor %d must be finite and > 0", i);
+    for (size_t f = 0; f < (size_t)n_flops; ++f)
+        for (int p = 0; p < P4; ++p)
+            for (int i = 0; i < kAliveFlop; ++i) {
+                const float v = ranges[(f * P4 + p) * NP + i];
+                MTT_ARG(std::isfinite(v) && v >= 0.0f,
+                        "mtt_multi_create: flop %zu seat %d combo %d: weight must be finite and >= 0", f, p, i);
+            }
     *out = nullptr;
     auto h = new (std::nothrow) mtt_multi;
     if (!h) { set_error("mtt_multi_create: out of host memory"); return MTT_ERR_OOM; }

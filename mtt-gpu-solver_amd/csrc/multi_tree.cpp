@@ -197,8 +197,11 @@ int check_game4(const mtt_4way_game* g) {
     }
     const mtt_street_menu& m = g->menu;
     MTT_ARG(m.n_sizes >= 0 && m.n_sizes <= 3, "4-way game: n_sizes out of range");
+    for (int k = 0; k < m.n_sizes; ++k)
+        MTT_ARG(m.sizes[k] > 0 && std::isfinite(m.sizes[k]), "4-way game: bet size %d must be finite and > 0", k);
     MTT_ARG(m.raise_cap >= 0 && m.raise_cap <= 4, "4-way game: raise cap out of range");
-    MTT_ARG(m.raise_mult == 0 || m.raise_mult > 1, "4-way game: raise multiple must exceed 1");
+    MTT_ARG(m.raise_mult == 0 || (m.raise_mult > 1 && std::isfinite(m.raise_mult)),
+            "4-way game: raise multiple must be 0 or a finite value above 1");
     return MTT_OK;
 }
 

@@ -399,6 +399,7 @@ extern "C" {
 
 int mtt_solver_create(const mtt_hu_game* game, const float* range0, const float* range1, const mtt_solver_cfg* cfg,
                       mtt_solver** out) {
+    (void)hipGetLastError();   // an error left by an earlier call is not this call's
     MTT_ARG(out, "mtt_solver_create: null out");
     *out = nullptr;
     mtt_solver* h = new (std::nothrow) mtt_solver;

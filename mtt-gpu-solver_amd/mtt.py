@@ -24,7 +24,8 @@ NODE_DECISION, NODE_FOLD, NODE_SHOWDOWN, NODE_CHANCE = 0, 1, 2, 3
 
 
 class MTTError(RuntimeError):
-    pass
+    """A status other than MTT_OK; .status holds it (MTT_ERR_ARG, ...)."""
+    status = None
 
 
 class FlatNode(ctypes.Structure):
@@ -131,7 +132,9 @@ def last_error():
 
 def check(status, what=""):
     if status != MTT_OK:
-        raise MTTError(f"{what} failed with status {status}: {last_error()}")
+        e = MTTError(f"{what} failed with status {status}: {last_error()}")
+        e.status = status
+        raise e
     return status
 
 

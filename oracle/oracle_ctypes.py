@@ -15,11 +15,14 @@ _P = ctypes.c_void_p
 
 
 def lib():
+    """The oracle library; MTT_ORACLE_LIB names another build of the same
+    sources (the sanitizer test's instrumented one)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} missing: run make -C oracle")
-        L = ctypes.CDLL(LIB_PATH)
+        path = os.environ.get("MTT_ORACLE_LIB", LIB_PATH)
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run make -C oracle")
+        L = ctypes.CDLL(path)
         L.or_eval7.restype = ctypes.c_uint32
         L.or_eval7.argtypes = [_P]
         L.or_eval7_batch.argtypes = [_P, ctypes.c_int64, _P]
