@@ -96,54 +96,105 @@ __device__ uint32_t eval7_multiset(const int* c) {
     return rm;
 }
 
-// One thread per (hero class, villain class) cell: the reference's sample
-// stream (32-bit LCG rng = rng * 1664525 + 1013904223 seeded with
-// cell * 2654435761 + 0x9E3779B9; board card k = deck[k + (rng >> 16) %
-// (48 - k)]), counting wins and ties of the hero.
+// The reference's LCG, rng <- rng * 1664525 + 1013904223 (mod 2^32), advanced
+// n steps at once: the n-fold composition of the affine map, by squaring.
+__device__ __forceinline__ uint32_t lcg_jump(uint32_t s, uint32_t n) {
+    uint32_t A = 1u, C = 0u, a = 1664525u, c = 1013904223u;
+    while (n) {
+        if (n & 1u) {
+            A = a * A;
+            C = a * C + c;
+        }
+        c = a * c + c;
+        a = a * a;
+        n >>= 1;
+    }
+    return A * s + C;
+}
+
+// Position of the r-th (0-based) set bit of m (r < popcount(m)): a binary
+// search by popcounts over halves, quarters, .. of the word, branch-free.
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
+    const int lo = __builtin_popcount((uint32_t)m);
+    const bool up = r >= lo;
+    uint32_t x = up ? (uint32_t)(m >> 32) : (uint32_t)m;
+    int pos = up ? 32 : 0;
+    r -= up ? lo : 0;
+#pragma unroll
+    for (int w = 16; w >= 1; w >>= 1) {
+        const int c = __builtin_popcount(x & ((1u << w) - 1u));
+        const bool hi = r >= c;
+        r -= hi ? c : 0;
+        x = hi ? x >> w : x;
+        pos += hi ? w : 0;
+    }
+    return pos;
+}
+
+// kMcSplit lanes per (hero class, villain class) cell, each taking a
+// contiguous share of the cell's samples: the reference's sample stream
+// (32-bit LCG seeded with cell * 2654435761 + 0x9E3779B9, five steps per
+// sample; board card k = deck[k + (rng >> 16) % (48 - k)]) entered at the
+// share's first sample by a jump; the lanes' win and tie counts are summed
+// exactly, so the counts are the single-stream ones.
+constexpr int kMcSplit = 16;
 __global__ __launch_bounds__(256) void k_preflop_matrix(int samples, float* __restrict__ eq,
                                                         int32_t* __restrict__ wins, int32_t* __restrict__ ties) {
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= 169 * 169) return;
-    const int h = cell / 169, v = cell % 169;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cell = tid / kMcSplit, part = tid % kMcSplit;
+    const bool valid = cell < 169 * 169;
+    const int h = valid ? cell / 169 : 0, v = valid ? cell % 169 : 1;
+    int w = 0, t = 0;
+    if (valid && h != v) {
+        int h1, h2, v1, v2;
+        class_cards(h, h1, h2);
+        class_cards(v, v1, v2);
+        const int ov1 = v1, ov2 = v2;
+        if (h1 == ov1 || h2 == ov1) v1 = (ov1 >> 2) * 4 + 2;     // collision rule, :103-106
+        if (h1 == ov2 || h2 == ov2) v2 = (ov2 >> 2) * 4 + 3;
+        const uint64_t dead = (1ull << h1) | (1ull << h2) | (1ull << v1) | (1ull << v2);
+        // deck[i] = i-th smallest card outside `dead`: the 48 live cards as a mask, the i-th one found
+        // by popcount steps
+        const uint64_t live = ~dead & ((1ull << 52) - 1);
+        const int per = (samples + kMcSplit - 1) / kMcSplit;
+        const int s0 = min(samples, part * per), s1 = min(samples, s0 + per);
+        uint32_t rng = lcg_jump((uint32_t)cell * 2654435761u + 0x9E3779B9u, 5u * (uint32_t)s0);
+        int hh[7] = {h1, h2, 0, 0, 0, 0, 0}, vv[7] = {v1, v2, 0, 0, 0, 0, 0};
+        for (int s = s0; s < s1; ++s) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                rng = rng * 1664525u + 1013904223u;
+                const int pick = k + (int)((rng >> 16) % (uint32_t)(48 - k));
+                const int card = nth_set_bit(live, pick);
+                hh[2 + k] = card;
+                vv[2 + k] = card;
+            }
+            const uint32_t sh = eval7_multiset(hh), sv = eval7_multiset(vv);
+            w += sh > sv;
+            t += sh == sv;
+        }
+    }
+    // the cell's kMcSplit lanes are consecutive lanes of one wave
+#pragma unroll
+    for (int o = kMcSplit / 2; o > 0; o >>= 1) {
+        w += __shfl_xor(w, o, 64);
+        t += __shfl_xor(t, o, 64);
+    }
+    if (!valid || part != 0) return;
     if (h == v) {
         eq[cell] = 0.5f;
         if (wins) wins[cell] = 0;
         if (ties) ties[cell] = 0;
         return;
     }
-    int h1, h2, v1, v2;
-    class_cards(h, h1, h2);
-    class_cards(v, v1, v2);
-    const int ov1 = v1, ov2 = v2;
-    if (h1 == ov1 || h2 == ov1) v1 = (ov1 >> 2) * 4 + 2;     // collision rule, :103-106
-    if (h1 == ov2 || h2 == ov2) v2 = (ov2 >> 2) * 4 + 3;
-    const uint64_t dead = (1ull << h1) | (1ull << h2) | (1ull << v1) | (1ull << v2);
-    // deck[i] = i-th smallest card outside `dead`: the 48 live cards as a mask,
-    // the i-th one found by popcount steps
-    const uint64_t live = ~dead & ((1ull << 52) - 1);
-    uint32_t rng = (uint32_t)cell * 2654435761u + 0x9E3779B9u;
-    int hh[7] = {h1, h2, 0, 0, 0, 0, 0}, vv[7] = {v1, v2, 0, 0, 0, 0, 0};
-    int w = 0, t = 0;
-    for (int s = 0; s < samples; ++s) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            rng = rng * 1664525u + 1013904223u;
-            int pick = k + (int)((rng >> 16) % (uint32_t)(48 - k));
-            uint64_t m = live;
-            for (int i = 0; i < pick; ++i) m &= m - 1;        // drop the pick lowest live cards
-            const int card = __builtin_ctzll(m);
-            hh[2 + k] = card;
-            vv[2 + k] = card;
-        }
-        const uint32_t sh = eval7_multiset(hh), sv = eval7_multiset(vv);
-        w += sh > sv;
-        t += sh == sv;
-    }
     eq[cell] = ((float)w + 0.5f * (float)t) / (float)samples;
     if (wins) wins[cell] = w;
     if (ties) ties[cell] = t;
 }
 
+}  // namespace
+
+namespace {
 
 // ------------------------------------------------ class equity matrices --
 // Tables: the cards of the 1,326 combos (lexicographic), the members of each
@@ -721,8 +772,8 @@ extern "C" {
 int mtt_gen_preflop_matrix(float* d_eq, int32_t* d_wins, int32_t* d_ties, int samples, hipStream_t stream) {
     MTT_ARG(d_eq, "mtt_gen_preflop_matrix: null output");
     MTT_ARG(samples > 0, "mtt_gen_preflop_matrix: samples %d must be positive", samples);
-    hipLaunchKernelGGL(k_preflop_matrix, dim3((169 * 169 + 255) / 256), dim3(256), 0, stream, samples, d_eq, d_wins,
-                       d_ties);
+    hipLaunchKernelGGL(k_preflop_matrix, dim3((169 * 169 * kMcSplit + 255) / 256), dim3(256), 0, stream, samples,
+                       d_eq, d_wins, d_ties);
     MTT_HIP(hipGetLastError());
     return MTT_OK;
 }
