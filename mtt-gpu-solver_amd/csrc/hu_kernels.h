@@ -69,7 +69,7 @@ int launch_hu_terminals_ref(const int* d_terms, int num, const mtt_flat_node* d_
                             int n_alive, hipStream_t s);
 int launch_hu_terminals_grouped(const PlainTerminalPlan& plan, const mtt_flat_node* d_nodes, const uint8_t* d_deck,
                                 const mtt_alive_combo* d_combos, const float* d_reach, const float* d_mats,
-                                double* d_sums, float* d_cfv, int n_alive, hipStream_t s);
+                                long long* d_sums, float* d_cfv, int n_alive, hipStream_t s);
 int build_terminal_plan(const std::vector<mtt_flat_node>& nodes, const std::vector<int>& terms, int n_alive,
                         const uint8_t* c1, const uint8_t* c2, PlainTerminalPlan& plan, DeviceArena& arena);
 

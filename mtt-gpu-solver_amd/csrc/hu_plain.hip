@@ -80,36 +80,50 @@ __global__ __launch_bounds__(256) void k_hu_forward(const int* __restrict__ dept
 
 // ------------------------------------------------------ terminal sums --
 // Per terminal and player: sum of the player's reach over the combos that
-// hold each card (52) and the total (slot 52), in FP64: the blocker-adjusted
-// mass tot - R[c1] - R[c2] + r_i cancels heavily when most of the opponent's
-// reach shares a card with the hero.  Deterministic: thread c sums its
-// card's combo list in combo order.
+// hold each card (52) and the total (slot 52), in 64-bit fixed point with 48
+// fraction bits: exact integers, so the blocker-adjusted mass
+// tot - R[c1] - R[c2] + r_i, which cancels heavily when most of the
+// opponent's reach shares a card with the hero, carries no rounding until
+// its one conversion to FP32.  Integer arithmetic only (reach < 2^15).
+__device__ __forceinline__ long long fx48_of(float r) {
+    // r * 2^48 rounded to the nearest integer, from the FP32 fields (r >= 0)
+    const uint32_t bits = __float_as_uint(r);
+    const int e = (int)((bits >> 23) & 0xFFu);
+    if (e == 0 || (bits >> 31)) return 0;             // zero, subnormal (< 2^-126) or negative
+    const long long m = (long long)((bits & 0x7FFFFFu) | 0x800000u);
+    const int sh = e - 102;                           // r * 2^48 = m * 2^(e - 127 - 23 + 48)
+    if (sh >= 0) return m << sh;
+    if (sh < -24) return 0;
+    return (m + (1ll << (-sh - 1))) >> (-sh);
+}
+__device__ __forceinline__ float fx48_to_float(long long v) { return ldexpf((float)v, -48); }
+
 __global__ __launch_bounds__(64) void k_hu_term_sums(const int* __restrict__ terms, int num,
                                                      const float* __restrict__ reach,
                                                      const int16_t* __restrict__ card_combos,  // [52][51]
-                                                     double* __restrict__ sums) {              // [num][2][64]
-    __shared__ double s[2][64];
+                                                     long long* __restrict__ sums) {           // [num][2][64]
+    __shared__ long long s[2][64];
     int t = blockIdx.x;
     if (t >= num) return;
     int id = terms[t];
     int c = threadIdx.x;
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        double acc = 0.0;
+        long long acc = 0;
         if (c < 52) {
             const float* r = reach + ((size_t)id * 2 + p) * NP;
             for (int k = 0; k < 51; ++k) {
                 int j = card_combos[c * 51 + k];
-                if (j >= 0) acc += (double)r[j];
+                if (j >= 0) acc += fx48_of(r[j]);
             }
         }
-        s[p][c] = c < 52 ? acc : 0.0;
+        s[p][c] = c < 52 ? acc : 0;
     }
     __syncthreads();
     if (c < 2) {
-        double tot = 0.0;
+        long long tot = 0;
         for (int k = 0; k < 52; ++k) tot += s[c][k];
-        s[c][52] = 0.5 * tot;  // every combo holds two cards
+        s[c][52] = tot >> 1;  // every combo holds two cards: the sum is even
     }
     __syncthreads();
     sums[((size_t)t * 2 + 0) * 64 + c] = s[0][c];
@@ -117,8 +131,9 @@ __global__ __launch_bounds__(64) void k_hu_term_sums(const int* __restrict__ ter
 }
 
 // mass_p(i) = sum_j nb(i,j) r_opp(j) = tot - R[c1] - R[c2] + r_opp(i)
-__device__ __forceinline__ float blocker_mass(const double* __restrict__ s, const mtt_alive_combo& c, float r_self) {
-    return (float)(s[52] - s[c.c1] - s[c.c2] + (double)r_self);
+__device__ __forceinline__ float blocker_mass(const long long* __restrict__ s, const mtt_alive_combo& c,
+                                              float r_self) {
+    return fx48_to_float(s[52] - s[c.c1] - s[c.c2] + fx48_of(r_self));
 }
 
 // ---------------------------------------------------------------- folds --
@@ -126,8 +141,9 @@ __global__ __launch_bounds__(256) void k_hu_fold_values(const int* __restrict__ 
                                                         int n_fold, const mtt_flat_node* __restrict__ nodes,
                                                         const uint8_t* __restrict__ deck,
                                                         const mtt_alive_combo* __restrict__ combos,
-                                                        const float* __restrict__ reach, const double* __restrict__ sums,
-                                                        float* __restrict__ cfv, int n_alive) {
+                                                        const float* __restrict__ reach,
+                                                        const long long* __restrict__ sums, float* __restrict__ cfv,
+                                                        int n_alive) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= NP) return;
     for (int f = blockIdx.y; f < n_fold; f += gridDim.y) {
@@ -139,8 +155,8 @@ __global__ __launch_bounds__(256) void k_hu_fold_values(const int* __restrict__ 
         if (tid < n_alive) {
             mtt_alive_combo c = combos[tid];
             if (!(c.mask & dealt_mask(n.river_idx, deck))) {
-                const double* sw = sums + ((size_t)t * 2 + w) * 64;
-                const double* sl = sums + ((size_t)t * 2 + l) * 64;
+                const long long* sw = sums + ((size_t)t * 2 + w) * 64;
+                const long long* sl = sums + ((size_t)t * 2 + l) * 64;
                 float rw = reach[((size_t)id * 2 + w) * NP + tid];
                 float rl = reach[((size_t)id * 2 + l) * NP + tid];
                 v[w] = (n.pot_size - n.committed[w]) * blocker_mass(sl, c, rl);
@@ -167,7 +183,7 @@ __global__ __launch_bounds__(256) void k_hu_showdown_grouped(
     const SdTile* __restrict__ tiles, const int* __restrict__ col_term, const uint8_t* __restrict__ col_player,
     const int* __restrict__ terms, const mtt_flat_node* __restrict__ nodes, const uint8_t* __restrict__ deck,
     const mtt_alive_combo* __restrict__ combos, const float* __restrict__ mats, const float* __restrict__ reach,
-    const double* __restrict__ sums, float* __restrict__ cfv, int n_alive) {
+    const long long* __restrict__ sums, float* __restrict__ cfv, int n_alive) {
     __shared__ float As[32][64];
     __shared__ float Bs[32][65];
     __shared__ const float* bptr[64];
@@ -234,7 +250,7 @@ __global__ __launch_bounds__(256) void k_hu_showdown_grouped(
         int pl = col_player[gc];
         mtt_flat_node n = nodes[id];
         uint64_t dm = dealt_mask(n.river_idx, deck);
-        const double* so = sums + ((size_t)t * 2 + (1 - pl)) * 64;
+        const long long* so = sums + ((size_t)t * 2 + (1 - pl)) * 64;
         const float* ro = reach + ((size_t)id * 2 + (1 - pl)) * NP;
         float* out = cfv + ((size_t)id * 2 + pl) * NP;
 #pragma unroll
@@ -261,7 +277,7 @@ __global__ __launch_bounds__(256) void k_hu_evaluate_terminals(const int* __rest
                                                                const float* __restrict__ reach,
                                                                const float* __restrict__ mats,
                                                                float* __restrict__ cfv, int n_alive) {
-    __shared__ double card[2][64];
+    __shared__ long long card[2][64];
     __shared__ float rr[2][NP];
     int t = blockIdx.y;
     if (t >= num) return;
@@ -274,20 +290,20 @@ __global__ __launch_bounds__(256) void k_hu_evaluate_terminals(const int* __rest
     __syncthreads();
     if (threadIdx.x < 64) {
         int c = threadIdx.x;
-        double a0 = 0.0, a1 = 0.0;
+        long long a0 = 0, a1 = 0;
         if (c < 52)
             for (int j = 0; j < n_alive; ++j) {
                 mtt_alive_combo cb = combos[j];
-                if (cb.c1 == c || cb.c2 == c) { a0 += rr[0][j]; a1 += rr[1][j]; }
+                if (cb.c1 == c || cb.c2 == c) { a0 += fx48_of(rr[0][j]); a1 += fx48_of(rr[1][j]); }
             }
         card[0][c] = a0;
         card[1][c] = a1;
     }
     __syncthreads();
     if (threadIdx.x < 2) {
-        double tot = 0.0;
+        long long tot = 0;
         for (int k = 0; k < 52; ++k) tot += card[threadIdx.x][k];
-        card[threadIdx.x][52] = 0.5 * tot;
+        card[threadIdx.x][52] = tot >> 1;
     }
     __syncthreads();
     int h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -417,7 +433,7 @@ int launch_hu_terminals_ref(const int* d_terms, int num, const mtt_flat_node* d_
 
 int launch_hu_terminals_grouped(const PlainTerminalPlan& plan, const mtt_flat_node* d_nodes, const uint8_t* d_deck,
                                 const mtt_alive_combo* d_combos, const float* d_reach, const float* d_mats,
-                                double* d_sums, float* d_cfv, int n_alive, hipStream_t s) {
+                                long long* d_sums, float* d_cfv, int n_alive, hipStream_t s) {
     if (plan.n_terms <= 0) return MTT_OK;
     hipLaunchKernelGGL(k_hu_term_sums, dim3(plan.n_terms), dim3(64), 0, s, plan.d_terms, plan.n_terms, d_reach,
                        plan.d_card_combos, d_sums);

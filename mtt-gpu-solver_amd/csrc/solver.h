@@ -87,7 +87,7 @@ struct SolverImpl {
     float* d_S = nullptr;
     float* d_sigma = nullptr;
     float* d_mats = nullptr;              // [n_mat][1216][1216]
-    double* d_sums = nullptr;             // [terms][2][64] card sums (FP64)
+    long long* d_sums = nullptr;          // [terms][2][64] card sums (2^48 fixed point)
     PlainTerminalPlan plan;
     // batched path
     BatchedState* batched = nullptr;       // owned; freed by batched_destroy
