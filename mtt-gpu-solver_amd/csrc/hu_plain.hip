@@ -84,17 +84,14 @@ __global__ __launch_bounds__(256) void k_hu_forward(const int* __restrict__ dept
 // fraction bits: exact integers, so the blocker-adjusted mass
 // tot - R[c1] - R[c2] + r_i, which cancels heavily when most of the
 // opponent's reach shares a card with the hero, carries no rounding until
-// its one conversion to FP32.  Integer arithmetic only (reach < 2^15).
+// its one conversion to FP32.  Integer arithmetic only (reach < 2^15):
+// r * 2^48 truncated, from the integer and fraction parts of r * 2^16 (both
+// exact in FP32).
 __device__ __forceinline__ long long fx48_of(float r) {
-    // r * 2^48 rounded to the nearest integer, from the FP32 fields (r >= 0)
-    const uint32_t bits = __float_as_uint(r);
-    const int e = (int)((bits >> 23) & 0xFFu);
-    if (e == 0 || (bits >> 31)) return 0;             // zero, subnormal (< 2^-126) or negative
-    const long long m = (long long)((bits & 0x7FFFFFu) | 0x800000u);
-    const int sh = e - 102;                           // r * 2^48 = m * 2^(e - 127 - 23 + 48)
-    if (sh >= 0) return m << sh;
-    if (sh < -24) return 0;
-    return (m + (1ll << (-sh - 1))) >> (-sh);
+    const float t = r * 65536.0f;
+    const uint32_t hi = (uint32_t)t;
+    const uint32_t lo = (uint32_t)((t - (float)hi) * 4294967296.0f);
+    return (long long)(((unsigned long long)hi << 32) | lo);
 }
 __device__ __forceinline__ float fx48_to_float(long long v) { return ldexpf((float)v, -48); }
 

@@ -112,20 +112,34 @@ __global__ __launch_bounds__(256) void k4_backward(const int* __restrict__ idx, 
 // ---------------------------------------------------------- terminals ----
 // One workgroup per (terminal, flop).  Reaches of the four players are
 // staged in LDS; for each opponent o the exclusive prefix over strength
-// positions (block scan, FP64) and over each card's list (one thread per
-// card) give, per hero, o's reach strictly weaker (F), weaker or tied (G)
-// and in total (W), all without combos that share a hero card.
-__device__ __forceinline__ double wave_incl_scan(double v) {
+// positions (block scan) and over each card's list (one thread per card)
+// give, per hero, o's reach strictly weaker (F), weaker or tied (G) and in
+// total (W), all without combos that share a hero card.  The prefixes are
+// exact integers (reach in 2^48 fixed point), so the outcome masses F,
+// G - F and W - G are exact before their one rounding to FP32; the payoff
+// tables and their contraction are FP32.
+__device__ __forceinline__ long long wave_incl_scan(long long v) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        const double t = __shfl_up(v, o, 64);
+        const long long t = __shfl_up(v, o, 64);
         if (lane >= o) v += t;
     }
     return v;
 }
 
-__device__ __forceinline__ double pay_net(double net, double bf) { return net >= 0.0 ? net : bf * net; }
+// r * 2^48 truncated to an integer (0 <= r < 2^15), branch-free: the
+// integer and fraction parts of r * 2^16 (both exact in FP32) give the high
+// and low words.
+__device__ __forceinline__ long long fx48_of(float r) {
+    const float t = r * 65536.0f;
+    const uint32_t hi = (uint32_t)t;
+    const uint32_t lo = (uint32_t)((t - (float)hi) * 4294967296.0f);
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float fx48_to_float(long long v) { return ldexpf((float)v, -48); }
+
+__device__ __forceinline__ float pay_net(float net, float bf) { return net >= 0.0f ? net : bf * net; }
 
 // Showdown payoff of player p (still in) when its active opponents, in seat
 // order, are weaker (digit 0), tied (1) or stronger (2) than p: outcome
@@ -134,22 +148,22 @@ __device__ __forceinline__ double pay_net(double net, double bf) { return net >=
 // p takes each layer it is eligible for unless an eligible opponent is
 // stronger, shared with the eligible tied ones.  Net chips, scaled by the
 // bubble factor when negative.
-__device__ double k4_showdown_pay(const mtt_flat_node4& nd, int p, int s, double bpp) {
+__device__ float k4_showdown_pay(const mtt_flat_node4& nd, int p, int s, float bpp) {
     int opp[3] = {0, 0, 0}, na = 0;
     for (int o = 0; o < P4; ++o)
         if (o != p && (nd.active_mask & (1 << o))) opp[na++] = o;
     int dig[3];
     for (int t = 0; t < 3; ++t) { dig[t] = s % 3; s /= 3; }
     // distinct commitments of the players still in, ascending
-    double lv[P4];
+    float lv[P4];
     int k = 0;
     for (int q = 0; q < P4; ++q) {
         if (!(nd.active_mask & (1 << q))) continue;
-        const double c = nd.committed[q];
+        const float c = nd.committed[q];
         int at = k;
         bool dup = false;
         for (int u = 0; u < k; ++u) {
-            if (fabs(lv[u] - c) <= 1e-6) dup = true;
+            if (fabsf(lv[u] - c) <= 1e-6f) dup = true;
             else if (lv[u] > c && at == k) at = u;
         }
         if (dup) continue;
@@ -157,23 +171,23 @@ __device__ double k4_showdown_pay(const mtt_flat_node4& nd, int p, int s, double
         lv[at] = c;
         k++;
     }
-    double won = 0.0, prev = 0.0;
+    float won = 0.0f, prev = 0.0f;
     for (int l = 0; l < k; ++l) {
-        const double top = lv[l];
-        double chips = 0.0;
-        for (int q = 0; q < P4; ++q) chips += fmin(fmax((double)nd.committed[q] - prev, 0.0), top - prev);
+        const float top = lv[l];
+        float chips = 0.0f;
+        for (int q = 0; q < P4; ++q) chips += fminf(fmaxf(nd.committed[q] - prev, 0.0f), top - prev);
         prev = top;
-        if ((double)nd.committed[p] < top - 1e-6) continue;      // p is not in this layer
+        if (nd.committed[p] < top - 1e-6f) continue;      // p is not in this layer
         bool lost = false;
         int tied = 0;
         for (int t = 0; t < na; ++t) {
-            if ((double)nd.committed[opp[t]] < top - 1e-6) continue;
+            if (nd.committed[opp[t]] < top - 1e-6f) continue;
             lost |= dig[t] == 2;
             tied += dig[t] == 1;
         }
-        if (!lost) won += chips / (double)(tied + 1);
+        if (!lost) won += chips / (float)(tied + 1);
     }
-    return pay_net(won - (double)nd.committed[p], bpp);
+    return pay_net(won - nd.committed[p], bpp);
 }
 
 __global__ __launch_bounds__(TT) void k4_terminals(const int* __restrict__ terms, int nterm,
@@ -181,14 +195,14 @@ __global__ __launch_bounds__(TT) void k4_terminals(const int* __restrict__ terms
                                                    const float* __restrict__ reach, const mtt_board4* __restrict__ boards,
                                                    const float* __restrict__ bubble, float* __restrict__ cfv,
                                                    size_t node_stride) {
-    __shared__ float r[P4][NP];
-    __shared__ double P[NA + 1];
-    __shared__ double Q[52][49];
-    __shared__ double wsum[TT / 64];
+    __shared__ long long rx[P4][NP];   // reach in 2^48 fixed point
+    __shared__ long long P[NA + 1];
+    __shared__ long long Q[52][49];
+    __shared__ long long wsum[TT / 64];
     __shared__ uint16_t pc[NP];
     __shared__ uint16_t cl[52][48];
     __shared__ uint8_t ncard[52];
-    __shared__ double pay[P4][27];
+    __shared__ float pay[P4][27];
     const int t = blockIdx.x;
     if (t >= nterm) return;
     const int f = blockIdx.y;
@@ -199,15 +213,14 @@ __global__ __launch_bounds__(TT) void k4_terminals(const int* __restrict__ terms
     const int tid = threadIdx.x;
     for (int e = tid; e < P4 * NP; e += TT) {
         const int p = e / NP, i = e % NP;
-        r[p][i] = i < NA ? reach[fn + ((size_t)id * P4 + p) * NP + i] : 0.0f;
+        rx[p][i] = i < NA ? fx48_of(reach[fn + ((size_t)id * P4 + p) * NP + i]) : 0;
     }
     for (int e = tid; e < NP; e += TT) pc[e] = B.pos_combo[e];
     for (int e = tid; e < 52 * 48; e += TT) cl[e / 48][e % 48] = B.card_list[e / 48][e % 48];
     if (tid < 52) ncard[tid] = B.n_card[tid];
     if (nd.type == MTT_NODE_SHOWDOWN && tid < P4 * 27) {
         const int p = tid / 27;
-        pay[p][tid % 27] = (nd.active_mask & (1 << p)) ? k4_showdown_pay(nd, p, tid % 27, (double)bubble[p * P4 + p])
-                                                      : 0.0;
+        pay[p][tid % 27] = (nd.active_mask & (1 << p)) ? k4_showdown_pay(nd, p, tid % 27, bubble[p * P4 + p]) : 0.0f;
     }
     // hero data, packed: cards | lt counts, tie group, le counts
     uint32_t hcl[HPT], hgr[HPT], hle[HPT];
@@ -221,32 +234,35 @@ __global__ __launch_bounds__(TT) void k4_terminals(const int* __restrict__ terms
         hgr[h] = ok ? (uint32_t)B.grp_lo[i] | ((uint32_t)B.grp_hi[i] << 16) : 0u;
         hle[h] = ok ? (uint32_t)B.card_le[i][0] | ((uint32_t)B.card_le[i][1] << 8) : 0u;
     }
-    float F[P4][HPT], G[P4][HPT], W[P4][HPT];
+    // per opponent and hero, FP32: weaker (F), tied (G - F) and stronger (W - G) reach for an opponent
+    // still in a showdown; otherwise the total W in the first slot (all a fold or an inactive seat needs)
+    float MF[P4][HPT], MT[P4][HPT], MS[P4][HPT];
     __syncthreads();
     const bool fold = nd.type == MTT_NODE_FOLD;
-    for (int o = 0; o < P4; ++o) {
+#pragma unroll
+    for (int o = 0; o < P4; ++o) {   // unrolled: the per-opponent arrays stay in registers
         const bool need_fg = !fold && (nd.active_mask & (1 << o));
         // exclusive prefix over positions: thread owns positions 5*tid..5*tid+4
-        double loc[HPT], s = 0.0;
+        long long loc[HPT], s = 0;
 #pragma unroll
         for (int h = 0; h < HPT; ++h) {
             const int q = tid * HPT + h;
-            loc[h] = q < NA ? (double)r[o][pc[q]] : 0.0;
+            loc[h] = q < NA ? rx[o][pc[q]] : 0;
             s += loc[h];
         }
-        const double inc = wave_incl_scan(s);
+        const long long inc = wave_incl_scan(s);
         if ((tid & 63) == 63) wsum[tid >> 6] = inc;
         // per-card lists
         if (tid < 52) {
-            double acc = 0.0;
-            Q[tid][0] = 0.0;
+            long long acc = 0;
+            Q[tid][0] = 0;
             for (int k = 0; k < ncard[tid]; ++k) {
-                acc += (double)r[o][cl[tid][k]];
+                acc += rx[o][cl[tid][k]];
                 Q[tid][k + 1] = acc;
             }
         }
         __syncthreads();
-        double base = inc - s;
+        long long base = inc - s;
         for (int w = 0; w < (tid >> 6); ++w) base += wsum[w];
 #pragma unroll
         for (int h = 0; h < HPT; ++h) {
@@ -255,20 +271,23 @@ __global__ __launch_bounds__(TT) void k4_terminals(const int* __restrict__ terms
             base += loc[h];
         }
         __syncthreads();
-        const double tot = P[NA];
+        const long long tot = P[NA];
 #pragma unroll
         for (int h = 0; h < HPT; ++h) {
             const int i = tid + TT * h;
-            if (i >= NA) { F[o][h] = G[o][h] = W[o][h] = 0.0f; continue; }
-            const double ri = (double)r[o][i];
+            if (i >= NA) { MF[o][h] = MT[o][h] = MS[o][h] = 0.0f; continue; }
+            const long long ri = rx[o][i];
             const int a = hcl[h] & 0xFF, b = (hcl[h] >> 8) & 0xFF;
-            W[o][h] = (float)(tot - Q[a][ncard[a]] - Q[b][ncard[b]] + ri);
+            const long long Wv = tot - Q[a][ncard[a]] - Q[b][ncard[b]] + ri;
             if (need_fg) {
-                F[o][h] = (float)(P[hgr[h] & 0xFFFF] - Q[a][(hcl[h] >> 16) & 0xFF] - Q[b][hcl[h] >> 24]);
-                G[o][h] = (float)(P[hgr[h] >> 16] - Q[a][hle[h] & 0xFF] - Q[b][(hle[h] >> 8) & 0xFF] + ri);
+                const long long Fv = P[hgr[h] & 0xFFFF] - Q[a][(hcl[h] >> 16) & 0xFF] - Q[b][hcl[h] >> 24];
+                const long long Gv = P[hgr[h] >> 16] - Q[a][hle[h] & 0xFF] - Q[b][(hle[h] >> 8) & 0xFF] + ri;
+                MF[o][h] = fx48_to_float(Fv);
+                MT[o][h] = fx48_to_float(Gv - Fv);
+                MS[o][h] = fx48_to_float(Wv - Gv);
             } else {
-                F[o][h] = 0.0f;
-                G[o][h] = 0.0f;
+                MF[o][h] = fx48_to_float(Wv);
+                MT[o][h] = MS[o][h] = 0.0f;
             }
         }
         __syncthreads();
@@ -279,46 +298,51 @@ __global__ __launch_bounds__(TT) void k4_terminals(const int* __restrict__ terms
             if (nd.active_mask & (1 << p)) winner = p;
 #pragma unroll
     for (int p = 0; p < P4; ++p) {
-        const double cp = nd.committed[p];
-        const double bpp = bubble[p * P4 + p];
+        const float cp = nd.committed[p];
+        const float bpp = bubble[p * P4 + p];
 #pragma unroll
         for (int h = 0; h < HPT; ++h) {
             const int i = tid + TT * h;
             if (i >= NP) continue;
-            double v = 0.0;
+            float v = 0.0f;
             if (i < NA) {
                 if (fold || !(nd.active_mask & (1 << p))) {
-                    double opp = 1.0;
+                    float opp = 1.0f;
 #pragma unroll
-                    for (int o = 0; o < P4; ++o) if (o != p) opp *= (double)W[o][h];
-                    const double pay = fold ? (p == winner ? nd.pot_size - cp : -(double)bubble[p * P4 + winner] * cp)
-                                            : -bpp * cp;
-                    v = pay * opp;
+                    for (int o = 0; o < P4; ++o) {
+                        if (o == p) continue;
+                        // a fold, or a seat not in the showdown: W of every other seat (an opponent still in
+                        // the showdown holds its three outcome masses, which sum to W)
+                        const bool parts = !fold && (nd.active_mask & (1 << o));
+                        opp *= parts ? (MF[o][h] + MT[o][h]) + MS[o][h] : MF[o][h];
+                    }
+                    const float pw = fold ? (p == winner ? nd.pot_size - cp : -bubble[p * P4 + winner] * cp) : -bpp * cp;
+                    v = pw * opp;
                 } else {
                     // outcome masses per active opponent (weaker, tied, stronger),
                     // contracted with p's payoff table; absent digits are "weaker" with mass 1
-                    double inact = 1.0, m[3][3];
+                    float inact = 1.0f, m[3][3];
                     int na = 0;
 #pragma unroll
                     for (int o = 0; o < P4; ++o) {
                         if (o == p) continue;
                         if (nd.active_mask & (1 << o)) {
-                            m[na][0] = F[o][h];
-                            m[na][1] = (double)G[o][h] - (double)F[o][h];
-                            m[na][2] = (double)W[o][h] - (double)G[o][h];
+                            m[na][0] = MF[o][h];
+                            m[na][1] = MT[o][h];
+                            m[na][2] = MS[o][h];
                             na++;
                         } else {
-                            inact *= W[o][h];
+                            inact *= MF[o][h];                 // an inactive seat's slot holds W
                         }
                     }
-                    for (int u = na; u < 3; ++u) { m[u][0] = 1.0; m[u][1] = 0.0; m[u][2] = 0.0; }
-                    double acc = 0.0;
+                    for (int u = na; u < 3; ++u) { m[u][0] = 1.0f; m[u][1] = 0.0f; m[u][2] = 0.0f; }
+                    float acc = 0.0f;
 #pragma unroll
                     for (int d2 = 0; d2 < 3; ++d2) {
-                        double a1 = 0.0;
+                        float a1 = 0.0f;
 #pragma unroll
                         for (int d1 = 0; d1 < 3; ++d1) {
-                            const double* row = &pay[p][3 * d1 + 9 * d2];
+                            const float* row = &pay[p][3 * d1 + 9 * d2];
                             a1 += m[1][d1] * (m[0][0] * row[0] + m[0][1] * row[1] + m[0][2] * row[2]);
                         }
                         acc += m[2][d2] * a1;
@@ -326,7 +350,7 @@ __global__ __launch_bounds__(TT) void k4_terminals(const int* __restrict__ terms
                     v = acc * inact;
                 }
             }
-            cfv[fn + ((size_t)id * P4 + p) * NP + i] = (float)v;
+            cfv[fn + ((size_t)id * P4 + p) * NP + i] = v;
         }
     }
 }
@@ -589,7 +613,7 @@ int mtt_4way_evaluate_terminals(const int* d_terminal_node_indices, int num_term
         rc = launch_4way_terminals(d_terminal_node_indices, num_terminals, d_nodes, d_reach_probs, d_board,
                                    d_bubble_factors, d_cfvs, 1, 0, stream);
     hipError_t e2 = hipStreamSynchronize(stream);   // the board buffer is freed below
-    hipFree(d_board);
+    (void)hipFree(d_board);
     if (e != hipSuccess) return hip_fail(e, "mtt_4way_evaluate_terminals: board upload", __FILE__, __LINE__);
     if (e2 != hipSuccess) return hip_fail(e2, "mtt_4way_evaluate_terminals", __FILE__, __LINE__);
     return rc;
