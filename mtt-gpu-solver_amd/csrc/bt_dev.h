@@ -107,6 +107,30 @@ typedef const __attribute__((address_space(4))) int* cint_p;
 // Wave-uniform reads of small per-level tables through the scalar data cache.
 __device__ __forceinline__ int ldu(const int* p) { return *(cint_p)p; }
 
+// r * 2^48 truncated to an integer (0 <= r < 2^15), without FP64: the
+// integer and fraction parts of r * 2^16 (both exact in FP32) give the high
+// and low words.
+__device__ __forceinline__ unsigned long long fx48_of(float r) {
+    const float t = r * 65536.0f;
+    const uint32_t hi = (uint32_t)t;
+    const uint32_t lo = (uint32_t)((t - (float)hi) * 4294967296.0f);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// v * 2^-48 rounded to nearest even, for a signed 64-bit v.  Written out
+// (one u32 -> f32 conversion of the top 32 significant bits, the bits below
+// folded into a sticky bit) so that the library and the sweeps generated at
+// run time, which may be built by different compiler releases, round
+// identically.
+__device__ __forceinline__ float fx48_to_float(long long v) {
+    const unsigned long long a = v < 0 ? 0ull - (unsigned long long)v : (unsigned long long)v;
+    const int z = a ? __clzll((long long)a) : 0;
+    const unsigned long long n = a << z;
+    const uint32_t top = (uint32_t)(n >> 32) | ((uint32_t)n != 0u ? 1u : 0u);
+    const float f = ldexpf((float)top, -16 - z);
+    return v < 0 ? -f : f;
+}
+
 // 1/n for an action count n in 1..4 (wave-uniform; exact constants).
 __device__ __forceinline__ float uniform_share(int n) {
     return n == 2 ? 0.5f : (n == 3 ? 1.0f / 3.0f : (n == 4 ? 0.25f : 1.0f));
@@ -118,8 +142,8 @@ __device__ __forceinline__ float uniform_share(int n) {
 // first difference can be negative (by at most own), so it is taken as a
 // two's-complement signed value.
 __device__ __forceinline__ float mass_u(const unsigned long long* s, int a, int b, float self) {
-    const long long m = (long long)(s[52] - s[a] - s[b]);
-    return (float)((double)m * (1.0 / FX) + (double)self);
+    const long long m = (long long)(s[52] - s[a] - s[b]) + (long long)fx48_of(self);
+    return fx48_to_float(m);
 }
 
 __device__ __forceinline__ float root_reach(const BatchedDev& D, const float* __restrict__ ranges, int level, int b,

@@ -118,8 +118,8 @@ void emit_bins(Src& o, const Tmpl& T, int n) {
     const DNode& nd = T.node(n);
     o("if (live) {");
     o("unsigned long long* bn = binl + %d;", (nd.sidx - T.t0) * 104);
-    o("const unsigned long long u0 = (unsigned long long)__double2ull_rn((double)x0_%d * FX);", n);
-    o("const unsigned long long u1 = (unsigned long long)__double2ull_rn((double)x1_%d * FX);", n);
+    o("const unsigned long long u0 = fx48_of(x0_%d);", n);
+    o("const unsigned long long u1 = fx48_of(x1_%d);", n);
     o("if (u0) { atomicAdd(&bn[c1], u0); atomicAdd(&bn[c2], u0); }");
     o("if (u1) { atomicAdd(&bn[52 + c1], u1); atomicAdd(&bn[52 + c2], u1); }");
     o("}");

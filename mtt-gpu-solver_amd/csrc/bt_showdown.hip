@@ -43,9 +43,9 @@ constexpr int NRB = NPR / RB;       // 34 ring blocks
 static_assert(NPR % RB == 0 && RB % OB == 0, "row blocks");
 static_assert(VISIT * REC / 4 == 64, "one visit is one uint4 per lane");
 
-__device__ __forceinline__ unsigned long long fx_of(float r) { return __double2ull_rn((double)r * FX); }
+__device__ __forceinline__ unsigned long long fx_of(float r) { return fx48_of(r); }
 // C = W - L, summed in 2^48 units (two's complement)
-__device__ __forceinline__ float val_of(unsigned long long v) { return (float)((double)(long long)v * 0x1p-48); }
+__device__ __forceinline__ float val_of(unsigned long long v) { return fx48_to_float((long long)v); }
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 // Word i of a record held (uniform) in four vector registers, as a scalar.
 __device__ __forceinline__ uint32_t qword(const uint4& q0, const uint4& q1, const uint4& q2, const uint4& q3, int i) {

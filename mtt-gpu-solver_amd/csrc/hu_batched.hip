@@ -206,8 +206,8 @@ __global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __
         }
         if (live) {  // fold or showdown: card sums of both players' reaches
             unsigned long long* bn = bins + (size_t)(nd.sidx - t0) * 104;
-            const unsigned long long v0 = (unsigned long long)__double2ull_rn((double)x0 * FX);
-            const unsigned long long v1 = (unsigned long long)__double2ull_rn((double)x1 * FX);
+            const unsigned long long v0 = fx48_of(x0);
+            const unsigned long long v1 = fx48_of(x1);
             if (v0) {
                 atomicAdd(&bn[c1], v0);
                 atomicAdd(&bn[c2], v0);
