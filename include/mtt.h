@@ -429,9 +429,22 @@ int mtt_batch_get_strategy(mtt_batch* h, int instance, float* out, size_t n_floa
 int mtt_batch_exploitability_fp64(mtt_batch* h, int instance, double* out);
 /* 0: launch kernels one by one (also while timing is on), 1: graph replay. */
 int mtt_batch_set_graph(mtt_batch* h, int on);
-/* Autotuner's choice for the showdown GEMM (hero tiles per workgroup) and
- * the two candidates' measured times (0 when read from the tuning file). */
+/* Autotuner's choice for the showdown GEMM as a form code (2 or 4: hero
+ * tiles per workgroup of the LDS-staged form; 10 + HT: the register-streamed
+ * form with HT = 2, 3 or 4 hero tiles per wave) and, in gemm_ms[5], the
+ * measured times of the candidates 2, 4, 12, 13, 14 (0 when the choice was
+ * read from the tuning file or forced by MTT_FLOP_GEMM). */
 int mtt_batch_tuning(mtt_batch* h, int* hero_tiles, double* gemm_ms);
+/* The iteration form in use: 0 = forward, showdown GEMM and backward as
+ * three launches per iteration (graph-replayed); 1 = one persistent launch
+ * per iteration whose workgroups take forward, GEMM and backward items of
+ * all instances from a queue (same bodies, results identical to form 0);
+ * -1 = not chosen yet.  Form 1 is switched off in this build (its GPU runs
+ * did not finish; MTT_FLOP_PERSISTENT=1 makes mtt_batch_run fail with
+ * MTT_ERR_ARG), so the mode is 0 after the first run; ms[0], ms[1] hold the
+ * per-iteration times of a tuning run of both forms (0 when none ran). ms
+ * may be null. */
+int mtt_batch_iter_mode(mtt_batch* h, int* mode, double* ms);
 int mtt_batch_timing(mtt_batch* h, int enable);
 int mtt_batch_kernel_times(mtt_batch* h, int idx, char* name, int name_cap, double* total_ms, int64_t* launches);
 int mtt_batch_destroy(mtt_batch* h);

@@ -117,18 +117,17 @@ __device__ __forceinline__ unsigned long long fx48_of(float r) {
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// v * 2^-48 rounded to nearest even, for a signed 64-bit v.  Written out
-// (one u32 -> f32 conversion of the top 32 significant bits, the bits below
-// folded into a sticky bit) so that the library and the sweeps generated at
-// run time, which may be built by different compiler releases, round
-// identically.
+// v * 2^-48 for a signed 64-bit v, from its two 32-bit halves: the high
+// half (signed) and the low half (unsigned) are converted to FP32, the low
+// one scaled by 2^-48 (exact), and one explicit FMA adds them.  Every step is
+// an IEEE operation with a fixed rounding, so the library and the sweeps
+// generated at run time, which may be built by different compiler releases,
+// produce identical bits; the result is within 2^-40 + half an ulp of
+// v * 2^-48 (four VALU operations instead of a 64-bit normalisation).
 __device__ __forceinline__ float fx48_to_float(long long v) {
-    const unsigned long long a = v < 0 ? 0ull - (unsigned long long)v : (unsigned long long)v;
-    const int z = a ? __clzll((long long)a) : 0;
-    const unsigned long long n = a << z;
-    const uint32_t top = (uint32_t)(n >> 32) | ((uint32_t)n != 0u ? 1u : 0u);
-    const float f = ldexpf((float)top, -16 - z);
-    return v < 0 ? -f : f;
+    const float h = (float)(int)(v >> 32);
+    const float l = (float)(uint32_t)v * 3.5527136788005009e-15f;   // 2^-48
+    return fmaf(h, 1.52587890625e-05f, l);                           // 2^-16
 }
 
 // 1/n for an action count n in 1..4 (wave-uniform; exact constants).

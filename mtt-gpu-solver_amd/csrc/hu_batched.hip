@@ -96,23 +96,24 @@ constexpr int SWEEP_NT = 576;   // threads per sweep workgroup; two workgroups c
 // At each terminal the reach of both players is added into per-card bins in
 // LDS with 64-bit fixed-point atomics (exact, order independent); each
 // workgroup writes its half's bins, and the backward adds the two halves.
-template <int LEVEL, int DM, int NT, int NRV = 2>
-__global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __restrict__ ranges, float ws,
-                                                   const float* __restrict__ wdev) {
+//
+// The body takes the (board, template) instance and the half explicitly, and
+// walks its HALF combos (NT, or 640 in the persistent flop kernel) in PASSES
+// slices of NT into the same bins (the persistent flop kernel runs a 640-combo half as three
+// 256-thread passes, the last one partly masked; the bins are exact integer
+// sums, so the result does not depend on it).
+template <int LEVEL, int DM, int NT, int NRV, int PASSES>
+__device__ __forceinline__ void bt_forward_body(const BatchedDev& D, const float* __restrict__ ranges, float ws,
+                                                const float* __restrict__ wdev, int inst, int half, EvRec* sev,
+                                                unsigned long long* bins /* [nt][2][52] */) {
     typedef float vS __attribute__((ext_vector_type(DM * 4)));
     typedef float vD __attribute__((ext_vector_type(DM)));
     const LevelDev& L = D.L[LEVEL];
-    extern __shared__ unsigned long long bins[];  // [nt][2][52]
-    __shared__ EvRec sev[MAXE];
-    const int inst = blockIdx.x >> 1, half = blockIdx.x & 1;
     const int b = LEVEL == 2 ? inst : inst / L.ntmpl;
     const int tm = LEVEL == 2 ? ldu(L.btmpl + b) : inst % L.ntmpl;
     if (wdev) ws = wdev[2];
     const int tid = threadIdx.x;
     const int np = L.np;
-    const int k = half * NT + tid;
-    const bool inb = k < np;
-    const bool live = k < ldu(L.nlive + b);
     const int e0 = ldu(L.tfev0 + tm), e1 = ldu(L.tfev0 + tm + 1);
     const int s0 = ldu(L.tslot0 + tm), nsl = ldu(L.tslot0 + tm + 1) - s0;
     const int t0 = ldu(L.tterm0 + tm), nt = ldu(L.tterm0 + tm + 1) - t0;
@@ -120,6 +121,13 @@ __global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __
     const int sb = LEVEL == 2 ? 0 : s0, tb = LEVEL == 2 ? 0 : t0, cb0 = LEVEL == 2 ? ldu(L.tcol0 + tm) : 0;
     fwd_bins_clear<NT>(bins, nt);
     ev_stage(sev, L.fev + e0, e1 - e0);
+    constexpr int HALF = PASSES == 1 ? NT : 640;   // combos per half
+    static_assert(PASSES * NT >= HALF, "passes cover the half");
+    for (int pass = 0; pass < PASSES; ++pass) {
+    const bool mine = pass * NT + tid < HALF;
+    const int k = half * HALF + (mine ? pass * NT + tid : 0);
+    const bool inb = mine && k < np;
+    const bool live = mine && k < ldu(L.nlive + b);
     int c1 = 0, c2 = 0;
     float root0 = 0.0f, root1 = 0.0f;
     f32x32_t Ra, Rb, Rc;
@@ -136,7 +144,7 @@ __global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __
         root0 = root_reach(D, ranges, LEVEL, b, tm, np, k, 0);
         root1 = root_reach(D, ranges, LEVEL, b, tm, np, k, 1);
     }
-    __syncthreads();
+    if (pass == 0) __syncthreads();   // bins cleared, events staged
     vD rs0, rs1;
     vS sg;
     const int ne = e1 - e0;
@@ -218,7 +226,16 @@ __global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __
             }
         }
     }
+    }  // pass
     fwd_bins_flush<NT>(L, bins, b, tb, half, nt);
+}
+
+template <int LEVEL, int DM, int NT, int NRV = 2>
+__global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __restrict__ ranges, float ws,
+                                                   const float* __restrict__ wdev) {
+    extern __shared__ unsigned long long bins[];  // [nt][2][52]
+    __shared__ EvRec sev[MAXE];
+    bt_forward_body<LEVEL, DM, NT, NRV, 1>(D, ranges, ws, wdev, (int)(blockIdx.x >> 1), (int)(blockIdx.x & 1), sev, bins);
 }
 
 // ----------------------------------------------------------- backward ----
@@ -232,16 +249,14 @@ __global__ __launch_bounds__(NT) void k_bt_forward(BatchedDev D, const float* __
 // showdown values and the card sums, and applies the DCFR-generalised
 // regret update R <- (R + r) * (R + r > 0 ? wp : wn) at each decision node
 // on the way up.
-template <int LEVEL, int DM, int NTB, int NRV = 2>
-__global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* __restrict__ ranges, float wp, float wn,
-                                                     const float* __restrict__ wdev) {
-    constexpr int NH = ((LEVEL == 2 ? NPF : NPT) + NTB - 1) / NTB;  // workgroups per (board, template)
+template <int LEVEL, int DM, int NTB, int NRV>
+__device__ __forceinline__ void bt_backward_body(const BatchedDev& D, const float* __restrict__ ranges, float wp,
+                                                 float wn, const float* __restrict__ wdev, int inst, int half,
+                                                 EvRec* sev,
+                                                 unsigned long long* ssum /* [nt][2][53], then float [ncolt][NTB] */) {
     typedef float vS __attribute__((ext_vector_type(DM * 4)));
     typedef float vD __attribute__((ext_vector_type(DM)));
     const LevelDev& L = D.L[LEVEL];
-    extern __shared__ unsigned long long ssum[];  // [nt][2][53] (slot 52 = total), then float [ncolt][NTB]
-    __shared__ EvRec sev[MAXE];
-    const int inst = blockIdx.x / NH, half = blockIdx.x % NH;
     const int b = LEVEL == 2 ? inst : inst / L.ntmpl;
     const int tm = LEVEL == 2 ? ldu(L.btmpl + b) : inst % L.ntmpl;
     if (wdev) {
@@ -387,6 +402,16 @@ __global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* 
     bwd_root_out<LEVEL>(D, L, b, tm, k, np, live, out0, out1);
 }
 
+template <int LEVEL, int DM, int NTB, int NRV = 2>
+__global__ __launch_bounds__(NTB) void k_bt_backward(BatchedDev D, const float* __restrict__ ranges, float wp, float wn,
+                                                     const float* __restrict__ wdev) {
+    constexpr int NH = ((LEVEL == 2 ? NPF : NPT) + NTB - 1) / NTB;  // workgroups per (board, template)
+    extern __shared__ unsigned long long ssum[];  // [nt][2][53] (slot 52 = total), then float [ncolt][NTB]
+    __shared__ EvRec sev[MAXE];
+    bt_backward_body<LEVEL, DM, NTB, NRV>(D, ranges, wp, wn, wdev, (int)(blockIdx.x / NH), (int)(blockIdx.x % NH), sev,
+                                          ssum);
+}
+
 // Turn-leaf values: mean over the 44 rivers that leave the combo alive of
 // the river-line root values (rivers are summed in deck order).  Rivers go
 // 16 at a time: their map entries, then (from an in-bounds index on every
@@ -463,9 +488,17 @@ __global__ void k_bt_s8(BatchedDev D, uint8_t* __restrict__ S8) {
 constexpr int GNCH = NPR / 32;           // K chunks of 32 villains
 
 // The same signs at two bits per pair: word (b, chunk c, hero i, half h)
-// holds villains 32c + 16h + e (e = 0..15) at bits 2e, 2e+1; code 1 = +1/2
-// (hero ranks higher), 2 = -1/2, 0 = tie, blocked or dead.  One thread per
-// word.
+// holds the codes of villains 32c + 16h + 8k + e (K step k = 0, 1; e =
+// 0..7) with code 1 = +1/2 (hero ranks higher), 2 = -1/2, 0 = tie, blocked
+// or dead.  The bits are placed so that a fragment decodes with one shift
+// and one masked OR per pair of elements: with m = e >> 1, step 0 keeps
+// even e at bit 8 + 2m and odd e at bit 24 + 2m, step 1 even e at bit 2m and
+// odd e at bit 16 + 2m.  One thread per word.
+__device__ __forceinline__ int s2_pos(int e16) {
+    const int k = e16 >> 3, e = e16 & 7;
+    return (k ? 0 : 8) + ((e & 1) ? 16 : 0) + 2 * (e >> 1);
+}
+
 __global__ void k_bt_s2(BatchedDev D, uint32_t* __restrict__ S2) {
     const int b = blockIdx.y, i = blockIdx.x, q = threadIdx.x;  // q = c * 2 + h
     if (q >= GNCH * 2) return;
@@ -478,23 +511,24 @@ __global__ void k_bt_s2(BatchedDev D, uint32_t* __restrict__ S2) {
         const int j = c * 32 + 16 * h + e;
         if (i < nl && j < nl && !conflict(xi, D.cy[(size_t)b * NPR + j])) {
             const uint16_t gj = D.grank[(size_t)b * NPR + j];
-            w |= (gi > gj ? 1u : (gi < gj ? 2u : 0u)) << (2 * e);
+            w |= (gi > gj ? 1u : (gi < gj ? 2u : 0u)) << s2_pos(e);
         }
     }
     S2[(((size_t)b * GNCH + c) * NPR + i) * 2 + h] = w;
 }
 
-// Eight 2-bit sign codes (bits 0..15 of x, element e at bits 2e, 2e+1) to
-// the bf16 B-fragment of S/2: one byte lookup per pair of elements
-// (table bytes 0x00, 0x3F, 0xBF: the high byte of bf16 0, +1/2, -1/2).
-__device__ __forceinline__ bf16x8_t s2_frag(uint32_t x) {
+// The eight codes of K step k of an S word to the bf16 B-fragment of S/2:
+// per output dword (elements 2m, 2m+1) the two codes are moved to bits 8 and
+// 24, which makes them the byte selectors of one v_perm over the table bytes
+// 0x00, 0x3F, 0xBF (the high byte of bf16 0, +1/2, -1/2); selector 0x0C
+// gives the zero low bytes.
+__device__ __forceinline__ bf16x8_t s2_frag(uint32_t w, int k) {
     constexpr uint32_t T = 0x00BF3F00u;
     uint32_t o[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        const uint32_t cp = (x >> (4 * m)) & 15u;
-        const uint32_t sel = 0x000C000Cu | ((cp & 3u) << 8) | ((cp >> 2) << 24);
-        o[m] = __builtin_amdgcn_perm(0u, T, sel);
+        const uint32_t x = k ? (w << (8 - 2 * m)) : (w >> (2 * m));
+        o[m] = __builtin_amdgcn_perm(0u, T, (x & 0x03000300u) | 0x000C000Cu);
     }
     bf16x8_t r;
     __builtin_memcpy(&r, o, 16);
@@ -865,8 +899,8 @@ __global__ __launch_bounds__(512) void k_bt_gemm6s(BatchedDev D, const uint32_t*
         }
         const int buf = c & 1;
         if (on0) {
-            const bf16x8_t b00 = s2_frag(sa0), b01 = s2_frag(sa0 >> 16);
-            const bf16x8_t b10 = s2_frag(sa1), b11 = s2_frag(sa1 >> 16);
+            const bf16x8_t b00 = s2_frag(sa0, 0), b01 = s2_frag(sa0, 1);
+            const bf16x8_t b10 = s2_frag(sa1, 0), b11 = s2_frag(sa1, 1);
 #pragma unroll
             for (int i = 0; i < NTL; ++i) {
                 if (i < nct) {
@@ -996,8 +1030,8 @@ __global__ __launch_bounds__(512) void k_bt_gemm6d(BatchedDev D, const uint32_t*
 #define G6D_MMA(BUF_, W0_, W1_)                                                                   \
     do {                                                                                          \
         if (on0) {                                                                                \
-            const bf16x8_t b00 = s2_frag(W0_), b01 = s2_frag((W0_) >> 16);                        \
-            const bf16x8_t b10 = s2_frag(W1_), b11 = s2_frag((W1_) >> 16);                        \
+            const bf16x8_t b00 = s2_frag(W0_, 0), b01 = s2_frag(W0_, 1);                          \
+            const bf16x8_t b10 = s2_frag(W1_, 0), b11 = s2_frag(W1_, 1);                          \
             _Pragma("unroll") for (int i = 0; i < NTL; ++i) {                                     \
                 if (i < NTL - 1 || i < nct) {  /* only the last tile can be absent */             \
                     const int ct = ct0 + i;                                                       \
@@ -1057,6 +1091,103 @@ __global__ __launch_bounds__(512) void k_bt_gemm6d(BatchedDev D, const uint32_t*
                         Cb[(size_t)cc * NPR + (ht0 + t) * 32 + l32] = 2.0f * acc[i][t][r];
                     }
                 }
+            }
+        }
+    }
+}
+
+// Register-streamed river GEMM: one single-wave workgroup per (board,
+// 32-column tile, group of GH hero tiles), no LDS and no barrier.  The K loop
+// runs over chunks of 64 villains; per chunk a lane loads, straight into
+// registers, 64 contiguous bytes of its column's reach in each plane
+// (villains 64c + 32h .. +31 for lane half h) and, per hero tile, the two S
+// words of its hero row for the same villains (8 contiguous bytes), one
+// chunk ahead in a second register set.  K step ks of a chunk pairs lane
+// half h's elements 8ks .. 8ks+7 of those 32 villains in both operands (a
+// permutation of K shared by A and B), so each reach fragment feeds GH hero
+// tiles and each decoded S fragment both planes.  Per accumulator the order
+// is chunk, K step, high plane, low plane.  Workgroups are numbered (board,
+// column tile, hero group) with the hero group fastest and all of a board's
+// workgroups on one XCD, so the waves that share a column's reach run
+// together and find it in L2.
+constexpr int G7_NCH = NPR / 64;                 // K chunks of 64 villains
+static_assert(NPR % 64 == 0, "64-villain chunks");
+
+template <int GH>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_bt_gemm7(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical, int ntiles) {
+    constexpr int NTH = NPR / 32;                // hero tiles per board
+    constexpr int NHG = (NTH + GH - 1) / GH;     // hero groups per board
+    const int per = (nlogical + 7) >> 3;
+    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (logical >= nlogical) return;
+    const int b = logical / (ntiles * NHG), rem = logical % (ntiles * NHG);
+    const int ct = rem / NHG, hg = rem % NHG;
+    const int lane = threadIdx.x, half = lane >> 5, l32 = lane & 31;
+    const int ht0 = hg * GH;
+    const int nht = __builtin_amdgcn_readfirstlane(min(GH, NTH - ht0));   // GH, or fewer in the last group (every tile guarded)
+    const int ncolp = D.L[1].ncolp;
+    const size_t row = ((size_t)b * ncolp + ct * 32 + l32) * NPR;
+    const uint4* rh = reinterpret_cast<const uint4*>(D.Bh + row) + 4 * half;
+    const uint4* rl = reinterpret_cast<const uint4*>(D.Bl + row) + 4 * half;
+    // S words of hero tile t: uint2 [b][32-villain chunk][hero], chunk 2c + half
+    const uint2* sw = reinterpret_cast<const uint2*>(S2) + (size_t)b * GNCH * NPR + (size_t)half * NPR;
+    int hero[GH];
+#pragma unroll
+    for (int t = 0; t < GH; ++t) hero[t] = min(ht0 + t, NTH - 1) * 32 + l32;   // a missing tile reads tile NTH-1
+    f32x16_t acc[GH];
+#pragma unroll
+    for (int t = 0; t < GH; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+    uint4 ah[4], al[4], bh[4], bl[4];
+    uint2 as[GH], bs[GH];
+#define G7_LOAD(C_, H_, L_, S_)                                                         \
+    do {                                                                                \
+        const int c_ = (C_);                                                            \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) {                                 \
+            H_[q] = rh[8 * c_ + q];                                                     \
+            L_[q] = rl[8 * c_ + q];                                                     \
+        }                                                                               \
+        _Pragma("unroll") for (int t = 0; t < GH; ++t) S_[t] = sw[(size_t)(2 * c_) * NPR + hero[t]]; \
+    } while (0)
+#define G7_MMA(H_, L_, S_)                                                              \
+    do {                                                                                \
+        _Pragma("unroll") for (int ks = 0; ks < 4; ++ks) {                              \
+            bf16x8_t fh, fl, sf[GH];                                                    \
+            __builtin_memcpy(&fh, &H_[ks], 16);                                         \
+            __builtin_memcpy(&fl, &L_[ks], 16);                                         \
+            _Pragma("unroll") for (int t = 0; t < GH; ++t) {                            \
+                const uint32_t w = ks < 2 ? S_[t].x : S_[t].y;                          \
+                sf[t] = s2_frag(w, ks & 1);                                             \
+                if (t < nht)                                              \
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, sf[t], acc[t], 0, 0, 0); \
+            }                                                                           \
+            _Pragma("unroll") for (int t = 0; t < GH; ++t) {                            \
+                if (t < nht)                                              \
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, sf[t], acc[t], 0, 0, 0); \
+            }                                                                           \
+        }                                                                               \
+    } while (0)
+    G7_LOAD(0, ah, al, as);
+    int c = 0;
+    for (; c + 1 < G7_NCH; c += 2) {
+        G7_LOAD(c + 1, bh, bl, bs);
+        G7_MMA(ah, al, as);
+        G7_LOAD(min(c + 2, G7_NCH - 1), ah, al, as);
+        G7_MMA(bh, bl, bs);
+    }
+    if (c < G7_NCH) G7_MMA(ah, al, as);
+#undef G7_LOAD
+#undef G7_MMA
+    float* Cb = D.C + ((size_t)b * ncolp + ct * 32) * NPR;
+#pragma unroll
+    for (int t = 0; t < GH; ++t) {
+        if (t < nht) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int cc = (r >> 2) * 8 + half * 4 + (r & 3);
+                Cb[(size_t)cc * NPR + (ht0 + t) * 32 + l32] = 2.0f * acc[t][r];
             }
         }
     }
@@ -1765,10 +1896,25 @@ int batched_create(SolverImpl& s) {
 // River showdown GEMM in one of its forms, by shape code: nw * 10 + pd for
 // the column-major form (k_bt_gemm5: NW column tiles per workgroup, PD K
 // chunks prefetched), 100 for the hero-major form (k_bt_gemm6), 101 for
-// the hero-major form reading S at two bits per pair (k_bt_gemm6s) and 102
-// for the latter with the reach fetched two chunks ahead (k_bt_gemm6d).
+// the hero-major form reading S at two bits per pair (k_bt_gemm6s), 102
+// for the latter with the reach fetched two chunks ahead (k_bt_gemm6d), and
+// 700 + GH for the register-streamed single-wave form with GH hero tiles
+// per wave (k_bt_gemm7, GH = 5, 6, 7).
 int river_gemm(const BatchedState& B, int shape, hipStream_t st) {
     const int ntiles = B.D.L[1].ncolp / 32;
+    if (shape >= 700 && shape < 800) {
+        const int gh = shape - 700, nhg = (NPR / 32 + gh - 1) / gh;
+        const int nlog = B.nb1 * ntiles * nhg;
+        const dim3 g(((nlog + 7) / 8) * 8);
+        switch (gh) {
+        case 5: hipLaunchKernelGGL(k_bt_gemm7<5>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, ntiles); break;
+        case 6: hipLaunchKernelGGL(k_bt_gemm7<6>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, ntiles); break;
+        case 7: hipLaunchKernelGGL(k_bt_gemm7<7>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, ntiles); break;
+        default: set_error("river GEMM shape %d not available", shape); return MTT_ERR_ARG;
+        }
+        MTT_HIP(hipGetLastError());
+        return MTT_OK;
+    }
     if (shape == 102) {
         const int nlog = B.nb1 * G6_HB;
         const dim3 g(((nlog + 7) / 8) * 8);
@@ -1824,15 +1970,17 @@ int river_gemm(const BatchedState& B, int shape, hipStream_t st) {
     return MTT_OK;
 }
 
-// Autotuner of the river GEMM: the hero-major form and the column-major
-// form with all of the board's column tiles, or a half or a third of them,
-// per workgroup, prefetching 2 or 3 K chunks; each timed on the real
-// operands, the winner kept in the tuning file per column-tile count.
+// Autotuner of the river GEMM: the register-streamed form (5, 6 or 7 hero
+// tiles per wave), the hero-major forms, and the column-major form with
+// all of the board's column tiles, or a half or a third of them, per
+// workgroup, prefetching 2 or 3 K chunks; each timed on the real operands,
+// the winner kept in the tuning file per column-tile count.
 // MTT_RIVER_GEMM=<shape code> forces one.
 int river_gemm_autotune(BatchedState& B, hipStream_t st) {
     const int ntiles = B.D.L[1].ncolp / 32;
     auto valid = [&](int v) {
-        return (v >= 100 && v <= 102) || (v / 10 >= 1 && v / 10 <= std::min(ntiles, 9) && v % 10 >= 2 && v % 10 <= 3);
+        return (v >= 100 && v <= 102) || v == 705 || v == 706 || v == 707 ||
+               (v / 10 >= 1 && v / 10 <= std::min(ntiles, 9) && v % 10 >= 2 && v % 10 <= 3);
     };
     if (const char* e = getenv("MTT_RIVER_GEMM")) {
         const int v = atoi(e);
@@ -1841,7 +1989,7 @@ int river_gemm_autotune(BatchedState& B, hipStream_t st) {
     const std::string key = "river_gemm_shape/tiles" + std::to_string(ntiles);
     const int v = tune_lookup(key);
     if (valid(v)) { B.gemm_shape = v; return MTT_OK; }
-    std::vector<int> cand{102, 101, 100};
+    std::vector<int> cand{707, 706, 705, 102, 101, 100};
     for (int d = 1; d <= 3; ++d) {
         const int nw = (ntiles + d - 1) / d;
         for (int pd = 2; pd <= 3; ++pd)
@@ -2180,19 +2328,237 @@ __global__ __launch_bounds__((NW < 4 ? 4 : NW) * 64) void k_bt_gemm_flop(Batched
     }
 }
 
+// Register-streamed flop showdown GEMM: one single-wave workgroup per
+// (instance, group of HT hero tiles), no LDS and no barrier.  The W - L rows
+// are the stream that matters (3 MB per instance, read once): per 32-villain
+// chunk a lane loads its hero row's 16 villains (two 16-byte loads per hero
+// tile) and its column's reach in both planes, three chunks deep in register
+// sets used in rotation; each W - L piece is split into BF16 parts in
+// registers.  The instance's column tiles are taken one after the other
+// (most instances have one).  Per accumulator the order of the four
+// products per K step is that of k_bt_gemm_flop, so the results are
+// identical to it.
+template <int HT>
+__device__ __forceinline__ void gemm_flop7_wave(const BatchedDev& D, const int16_t* __restrict__ S16,
+                                                const int* __restrict__ bncol, int b, int hg) {
+    constexpr int NTH = NPF / 32;                 // 38 hero tiles
+    const int lane = threadIdx.x & 63, half = lane >> 5, l32 = lane & 31;
+    const int ht0 = hg * HT;
+    const int nht = __builtin_amdgcn_readfirstlane(min(HT, NTH - ht0));    // HT, or fewer in the last group (every tile guarded)
+    const int nct = __builtin_amdgcn_readfirstlane((bncol[b] + 31) >> 5);
+    const int ncolp = D.L[2].ncolp;
+    // W - L of hero tile t: uint4 [b][chunk][hero][4], this lane's half = pieces 2h, 2h + 1
+    const uint4* sp = reinterpret_cast<const uint4*>(S16) + (size_t)b * FNCH * NPF * 4 + 2 * half;
+    int hrow[HT];
+#pragma unroll
+    for (int t = 0; t < HT; ++t) hrow[t] = (min(ht0 + t, NTH - 1) * 32 + l32) * 4;
+    for (int ct = 0; ct < nct; ++ct) {
+        const size_t row = ((size_t)b * ncolp + ct * 32 + l32) * NPF;
+        const uint4* rh = reinterpret_cast<const uint4*>(D.Bh + row) + 2 * half;
+        const uint4* rl = reinterpret_cast<const uint4*>(D.Bl + row) + 2 * half;
+        f32x16_t acc[HT];
+#pragma unroll
+        for (int t = 0; t < HT; ++t)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+        uint4 ah[2], al[2], as[HT][2], bh[2], bl[2], bs[HT][2], ch[2], cl[2], cs[HT][2];
+#define GF7_LOAD(C_, H_, L_, S_)                                                                  \
+        do {                                                                                      \
+            const int c_ = (C_);                                                                  \
+            _Pragma("unroll") for (int k = 0; k < 2; ++k) {                                       \
+                H_[k] = rh[4 * c_ + k];                                                           \
+                L_[k] = rl[4 * c_ + k];                                                           \
+            }                                                                                     \
+            _Pragma("unroll") for (int t = 0; t < HT; ++t)                                        \
+                _Pragma("unroll") for (int k = 0; k < 2; ++k)                                     \
+                    S_[t][k] = sp[(size_t)c_ * NPF * 4 + hrow[t] + k];                            \
+        } while (0)
+#define GF7_MMA(H_, L_, S_)                                                                       \
+        do {                                                                                      \
+            _Pragma("unroll") for (int ks = 0; ks < 2; ++ks) {                                    \
+                bf16x8_t fh, fl, sh[HT], sl[HT];                                                  \
+                __builtin_memcpy(&fh, &H_[ks], 16);                                               \
+                __builtin_memcpy(&fl, &L_[ks], 16);                                               \
+                _Pragma("unroll") for (int t = 0; t < HT; ++t) {                                  \
+                    uint4 hi, lo;                                                                 \
+                    s16_split(S_[t][ks], hi, lo);                                                 \
+                    __builtin_memcpy(&sh[t], &hi, 16);                                            \
+                    __builtin_memcpy(&sl[t], &lo, 16);                                            \
+                }                                                                                 \
+                _Pragma("unroll") for (int t = 0; t < HT; ++t)                                    \
+                    if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, sh[t], acc[t], 0, 0, 0); \
+                _Pragma("unroll") for (int t = 0; t < HT; ++t)                                    \
+                    if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, sh[t], acc[t], 0, 0, 0); \
+                _Pragma("unroll") for (int t = 0; t < HT; ++t)                                    \
+                    if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, sl[t], acc[t], 0, 0, 0); \
+                _Pragma("unroll") for (int t = 0; t < HT; ++t)                                    \
+                    if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, sl[t], acc[t], 0, 0, 0); \
+            }                                                                                     \
+        } while (0)
+        constexpr int LAST = FNCH - 1;
+        GF7_LOAD(0, ah, al, as);
+        GF7_LOAD(1, bh, bl, bs);
+        int c = 0;
+        for (; c + 2 < FNCH; c += 3) {
+            GF7_LOAD(c + 2, ch, cl, cs);
+            GF7_MMA(ah, al, as);
+            GF7_LOAD(min(c + 3, LAST), ah, al, as);
+            GF7_MMA(bh, bl, bs);
+            GF7_LOAD(min(c + 4, LAST), bh, bl, bs);
+            GF7_MMA(ch, cl, cs);
+        }
+        if (c < FNCH) GF7_MMA(ah, al, as);
+        if (c + 1 < FNCH) GF7_MMA(bh, bl, bs);
+#undef GF7_LOAD
+#undef GF7_MMA
+        float* Cb = D.C + ((size_t)b * ncolp + ct * 32) * NPF;
+#pragma unroll
+        for (int t = 0; t < HT; ++t) {
+            if (t < nht) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int cc = (r >> 2) * 8 + half * 4 + (r & 3);
+                    Cb[(size_t)cc * NPF + (ht0 + t) * 32 + l32] = acc[t][r];
+                }
+            }
+        }
+    }
+}
+
+template <int HT>
+__global__ __launch_bounds__(64) void k_bt_gemm_flop7(BatchedDev D, const int16_t* __restrict__ S16,
+                                                      const int* __restrict__ bncol, int nlogical) {
+    constexpr int NHG = (NPF / 32 + HT - 1) / HT;
+    const int per = (nlogical + 7) >> 3;
+    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (logical >= nlogical) return;
+    gemm_flop7_wave<HT>(D, S16, bncol, logical / NHG, logical % NHG);
+}
+
+constexpr int FLOP_NT = 640;   // sweep workgroup size for flop instances: two cover the 1216 padded combos
+
 // Next iteration's weights (w_pos, w_neg, w_strat) from the precomputed
 // table: part of the captured iteration, so replays advance by themselves.
-__global__ void k_next_weights(int* __restrict__ t, const float* __restrict__ wtab, float* __restrict__ wcur) {
+// With the persistent iteration (pq non-null) it also rewinds the work queue
+// and advances the epoch that the completion counters are compared with.
+__global__ void k_next_weights(int* __restrict__ t, const float* __restrict__ wtab, float* __restrict__ wcur,
+                               int* __restrict__ pq) {
     if (threadIdx.x == 0) {
         const int n = *t + 1;
         *t = n;
         wcur[0] = wtab[3 * (size_t)(n - 1)];
         wcur[1] = wtab[3 * (size_t)(n - 1) + 1];
         wcur[2] = wtab[3 * (size_t)(n - 1) + 2];
+        if (pq) {
+            pq[0] = 0;
+            pq[1] += 1;
+        }
     }
 }
 
-constexpr int FLOP_NT = 640;   // sweep workgroup size for flop instances: two cover the 1216 padded combos
+// ------------------------------------------- persistent flop iteration --
+// One launch per iteration: workgroups of PF_NT threads take work items
+// from a queue until it is empty.  Items (phase << 28 | instance << 8 | sub):
+//   phase 0, forward half `sub` of an instance (640 combos in three passes);
+//   phase 1, showdown GEMM of hero groups sub * 4 .. sub * 4 + 3 (one per
+//            wave, PF_HT hero tiles each), after both forward halves;
+//   phase 2, backward slice `sub` (256 combos), after every GEMM item.
+// Four waves per workgroup, one per SIMD, so each may hold up to 512
+// registers (the deepest backward needs about 250 architectural ones).
+// The host orders the queue by blocks of instances, largest first, as
+// forward(k), GEMM(k - 1), backward(k - 2), so the phases of different
+// instances overlap.  An item depends only on items earlier in the queue,
+// which running workgroups hold (a workgroup takes its next item only after
+// finishing the current one), so every wait ends.  Completion counters grow
+// by one per finished item over the whole solve and are compared with
+// epoch x items-per-instance.  The bodies are those of the phase kernels,
+// so the results are identical to theirs.
+constexpr int PF_NT = 256;
+constexpr int PF_HT = 4;
+constexpr int PF_NWV = PF_NT / 64;
+constexpr int PF_NHG = (NPF / 32 + PF_HT - 1) / PF_HT;
+constexpr int PF_NFWD = 2;
+constexpr int PF_NGEMM = (PF_NHG + PF_NWV - 1) / PF_NWV;
+constexpr int PF_NBWD = (NPF + PF_NT - 1) / PF_NT;
+
+struct FlopQueue {
+    int* q;                    // [0] next item, [1] epoch, [2] a wait ran out of its bound
+    int* fdone;                // [instance] forward items finished
+    int* gdone;                // [instance] GEMM items finished
+    const uint32_t* items;
+    int nitems;
+};
+
+// Completion of an item: the barrier makes every wave's stores complete (to
+// L2), then one lane writes the L2 back for the other XCDs (release fence at
+// device scope) and increments the instance's counter.  One fence per item:
+// a device-scope fence on this GPU writes back or invalidates a whole XCD's
+// L2, so it must not run per wave or per poll.
+__device__ __forceinline__ void flop_signal(int* cnt) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Wait until an instance's counter reaches target: one lane polls with
+// relaxed device-scope loads (coherent across XCDs, no cache maintenance)
+// and a bounded number of sleeps; past the bound (several seconds) it raises
+// the error flag and the item runs anyway, so the kernel always ends.  Then
+// one acquire fence (the CU's L1 and the XCD's L2 are shared by the
+// workgroup's waves) and the barrier.
+__device__ __forceinline__ void flop_wait(const int* cnt, int target, int* err) {
+    if (threadIdx.x == 0) {
+        int spins = 0;
+        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > (1 << 22)) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(8);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+template <int DM, int NRV>
+__global__ __launch_bounds__(PF_NT) void k_flop_persistent(BatchedDev D, const float* __restrict__ ranges,
+                                                           const float* __restrict__ wcur,
+                                                           const int16_t* __restrict__ S16,
+                                                           const int* __restrict__ bncol, FlopQueue Q) {
+    extern __shared__ unsigned long long dyn[];   // forward bins or backward card sums + showdown values
+    __shared__ EvRec sev[MAXE];
+    __shared__ int s_item;
+    const int ep = Q.q[1];
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(Q.q, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= Q.nitems) break;
+        const uint32_t w = Q.items[it];
+        const int phase = (int)(w >> 28), inst = (int)((w >> 8) & 0xFFFFFu), sub = (int)(w & 0xFFu);
+        // the argument block read through an opaque zero offset per item: otherwise its uniform loads are
+        // hoisted out of the item loop and stay live across all three bodies (scalar, then vector spills)
+        int z;
+        asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+        const BatchedDev& Dz = *reinterpret_cast<const BatchedDev*>(reinterpret_cast<const char*>(&D) + z);
+        if (phase == 0) {
+            bt_forward_body<2, DM, PF_NT, NRV, (FLOP_NT + PF_NT - 1) / PF_NT>(Dz, ranges, 0.0f, wcur, inst, sub, sev, dyn);
+            flop_signal(Q.fdone + inst);
+        } else if (phase == 1) {
+            flop_wait(Q.fdone + inst, ep * PF_NFWD, Q.q + 2);
+            const int hg = sub * PF_NWV + (int)(threadIdx.x >> 6);
+            if (hg < PF_NHG) gemm_flop7_wave<PF_HT>(Dz, S16, bncol, inst, hg);
+            flop_signal(Q.gdone + inst);
+        } else {
+            flop_wait(Q.gdone + inst, ep * PF_NGEMM, Q.q + 2);
+            bt_backward_body<2, DM, PF_NT, NRV>(Dz, ranges, 0.0f, 0.0f, wcur, inst, sub, sev, dyn);
+        }
+    }
+}
 
 }  // namespace
 
@@ -2217,13 +2583,39 @@ struct BatchImpl {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool use_graph = true;
-    double gemm_ms[2] = {0, 0};
+    double gemm_ms[5] = {0, 0, 0, 0, 0};   // autotuner times of the flop GEMM forms 2, 4, 12, 13, 14
+    // iteration form: 0 = forward, GEMM and backward launches; 1 = one persistent launch; -1 = to be tuned
+    int iter_mode = -1;
+    bool pf_ok = false;              // the persistent form fits (LDS)
+    size_t pf_lds = 0;
+    int pf_grid = 0;
+    int* d_pq = nullptr;             // queue head, epoch, timeout flag
+    int* d_fdone = nullptr;
+    int* d_gdone = nullptr;
+    uint32_t* d_items = nullptr;
+    int nitems = 0;
+    double mode_ms[2] = {0, 0};      // tuned per-iteration times of the two forms
+    int graph_mode = -1;             // the form the captured graph holds
 };
 
 namespace {
 
+// Flop GEMM by tuning code: 2 or 4 = k_bt_gemm_flop with that many hero
+// tiles per workgroup, 10 + HT = the register-streamed k_bt_gemm_flop7<HT>.
 int batch_launch_gemm(BatchImpl& B, int ht) {
     const LevelDev& L = B.D.L[2];
+    if (ht > 10) {
+        const int h = ht - 10, nlog = B.n * ((NPF / 32 + h - 1) / h);
+        const dim3 g(((nlog + 7) / 8) * 8);
+        switch (h) {
+        case 2: hipLaunchKernelGGL(k_bt_gemm_flop7<2>, g, dim3(64), 0, B.stream, B.D, B.d_S16, B.d_bncol, nlog); break;
+        case 3: hipLaunchKernelGGL(k_bt_gemm_flop7<3>, g, dim3(64), 0, B.stream, B.D, B.d_S16, B.d_bncol, nlog); break;
+        case 4: hipLaunchKernelGGL(k_bt_gemm_flop7<4>, g, dim3(64), 0, B.stream, B.D, B.d_S16, B.d_bncol, nlog); break;
+        default: set_error("flop GEMM tuning code %d not available", ht); return MTT_ERR_ARG;
+        }
+        MTT_HIP(hipGetLastError());
+        return MTT_OK;
+    }
     const int hb = ht == 4 ? (NPF + 127) / 128 : (NPF + 63) / 64;
     const int nlog = B.n * hb;
     const dim3 g(((nlog + 7) / 8) * 8);
@@ -2253,7 +2645,20 @@ int batch_iteration(BatchImpl& B) {
     const size_t shf = (size_t)L.maxterm * 104 * 8;
     const size_t shb = bwd_lds(L, ntb);
     const dim3 gf(B.n * 2), gb(B.n * ((NPF + ntb - 1) / ntb));  // backward: ceil(NPF / ntb) workgroups per instance
-    hipLaunchKernelGGL(k_next_weights, dim3(1), dim3(64), 0, B.stream, B.d_t, B.d_wtab, B.d_wcur);
+    if (B.iter_mode == 1) {
+        hipLaunchKernelGGL(k_next_weights, dim3(1), dim3(64), 0, B.stream, B.d_t, B.d_wtab, B.d_wcur, B.d_pq);
+        const FlopQueue Q{B.d_pq, B.d_fdone, B.d_gdone, B.d_items, B.nitems};
+#define MTT_PF(DM_, NRV_)                                                                                           \
+    hipLaunchKernelGGL((k_flop_persistent<DM_, NRV_>), dim3(B.pf_grid), dim3(PF_NT), B.pf_lds, B.stream, B.D,       \
+                       B.d_ranges, B.d_wcur, B.d_S16, B.d_bncol, Q)
+        MTT_TIMED(B, "flop_persistent",
+                  if (B.dm == 4) { if (wide) MTT_PF(4, 3); else MTT_PF(4, 2); }
+                  else { if (wide) MTT_PF(8, 3); else MTT_PF(8, 2); });
+#undef MTT_PF
+        MTT_HIP(hipGetLastError());
+        return MTT_OK;
+    }
+    hipLaunchKernelGGL(k_next_weights, dim3(1), dim3(64), 0, B.stream, B.d_t, B.d_wtab, B.d_wcur, (int*)nullptr);
 #define MTT_FWD(DM_, NRV_) \
     hipLaunchKernelGGL((k_bt_forward<2, DM_, FLOP_NT, NRV_>), gf, dim3(FLOP_NT), shf, B.stream, B.D, B.d_ranges, 0.0f, B.d_wcur)
 #define MTT_BWD(DM_, NT_, NRV_) \
@@ -2288,40 +2693,49 @@ int batch_weights(BatchImpl& B, int tmax) {
     return MTT_OK;
 }
 
-// Autotuner: the flop GEMM's hero-tile count (2 or 4 tiles of 32 per
-// workgroup) is timed on the real operands once per column-tile count and
-// batch-size bucket, and the winner is kept in the tuning file (see
-// tune_path).
+// Autotuner: the flop GEMM's form and hero-tile count (k_bt_gemm_flop with 2
+// or 4 tiles of 32 per workgroup, k_bt_gemm_flop7 with 2, 3 or 4 per wave)
+// is timed on the real operands once per column-tile count and batch-size
+// bucket, and the winner is kept in the tuning file (see tune_path).
 int batch_autotune(BatchImpl& B) {
-    // the best tile count depends on how many workgroups fill the GPU: tuned per power-of-two batch size
+    static const int kCand[5] = {2, 4, 12, 13, 14};
+    auto valid = [](int v) { return v == 2 || v == 4 || v == 12 || v == 13 || v == 14; };
+    if (const char* e = getenv("MTT_FLOP_GEMM")) {
+        const int v = atoi(e);
+        if (valid(v)) { B.ht = v; return MTT_OK; }
+    }
+    // the best form depends on how many workgroups fill the GPU: tuned per power-of-two batch size
     int bucket = 1;
     while (bucket < B.n) bucket <<= 1;
-    const std::string key = "flop_gemm_ht/nw" + std::to_string(B.D.L[2].ncolp / 32) + "/n" + std::to_string(bucket);
+    const std::string key = "flop_gemm_form/nw" + std::to_string(B.D.L[2].ncolp / 32) + "/n" + std::to_string(bucket);
     int v = tune_lookup(key);
-    if (v == 2 || v == 4) { B.ht = v; return MTT_OK; }
+    if (valid(v)) { B.ht = v; return MTT_OK; }
     hipEvent_t a, b;
     MTT_HIP(hipEventCreate(&a));
     MTT_HIP(hipEventCreate(&b));
-    const int cand[2] = {2, 4};
-    for (int k = 0; k < 2; ++k) {
-        int rc = batch_launch_gemm(B, cand[k]);  // warm
+    int best = 0;
+    for (int k = 0; k < 5; ++k) {
+        int rc = batch_launch_gemm(B, kCand[k]);  // warm
         if (rc) return rc;
         MTT_HIP(hipEventRecord(a, B.stream));
-        for (int r = 0; r < 3; ++r) batch_launch_gemm(B, cand[k]);
+        for (int r = 0; r < 3; ++r) batch_launch_gemm(B, kCand[k]);
         MTT_HIP(hipEventRecord(b, B.stream));
         MTT_HIP(hipEventSynchronize(b));
         float ms = 0;
         MTT_HIP(hipEventElapsedTime(&ms, a, b));
         B.gemm_ms[k] = ms / 3.0;
+        if (B.gemm_ms[k] < B.gemm_ms[best]) best = k;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    B.ht = B.gemm_ms[0] < B.gemm_ms[1] ? 2 : 4;
+    B.ht = kCand[best];
     tune_store(key, B.ht);
     return MTT_OK;
 }
 
 }  // namespace
+
+int batch_persistent_setup(BatchImpl& B);
 
 int batch_create(const mtt_hu_game* games, int n, const float* ranges, const mtt_solver_cfg* cfg, BatchImpl** out) {
     *out = nullptr;
@@ -2441,24 +2855,30 @@ int batch_create(const mtt_hu_game* games, int n, const float* ranges, const mtt
     if (rc) return rc;
     rc = batch_autotune(B);
     if (rc) return rc;
+    rc = batch_persistent_setup(B);
+    if (rc) return rc;
     MTT_HIP(hipStreamSynchronize(B.stream));
     *out = own.release();
     return MTT_OK;
 }
 
-int batch_run(BatchImpl& B, int iters) {
-    if (B.iters_done + iters > B.tmax) {
-        int rc = batch_weights(B, std::max(2 * B.tmax, B.iters_done + iters));
-        if (rc) return rc;
-        if (B.exec) { (void)hipGraphExecDestroy(B.exec); B.exec = nullptr; }
-        if (B.graph) { (void)hipGraphDestroy(B.graph); B.graph = nullptr; }
-    }
+namespace {
+
+void batch_drop_graph(BatchImpl& B) {
+    if (B.exec) { (void)hipGraphExecDestroy(B.exec); B.exec = nullptr; }
+    if (B.graph) { (void)hipGraphDestroy(B.graph); B.graph = nullptr; }
+}
+
+// Iterations in the current form: replayed from the captured graph, or
+// launched one by one (graph replay off, or per-kernel timing on).
+int batch_iters(BatchImpl& B, int iters) {
     if (!B.use_graph || B.timer.enabled) {
         for (int it = 0; it < iters; ++it) {
             int rc = batch_iteration(B);
             if (rc) return rc;
         }
     } else {
+        if (B.exec && B.graph_mode != B.iter_mode) batch_drop_graph(B);
         if (!B.exec) {
             MTT_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
             int rc = batch_iteration(B);
@@ -2468,11 +2888,149 @@ int batch_run(BatchImpl& B, int iters) {
             if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
             B.graph = g;
             MTT_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+            B.graph_mode = B.iter_mode;
         }
         for (int it = 0; it < iters; ++it) MTT_HIP(hipGraphLaunch(B.exec, B.stream));
     }
     B.iters_done += iters;
+    return MTT_OK;
+}
+
+// The iteration form (three phase launches, or the persistent launch): from
+// MTT_FLOP_PERSISTENT, the tuning file, or timed on the solve itself -- two
+// replayed iterations of each form after one that captures its graph (both
+// forms give identical states, so the tuning iterations are ordinary
+// iterations of the solve).  Returns the iterations it ran.
+int batch_pick_mode(BatchImpl& B, int iters, int* ran) {
+    *ran = 0;
+    if (const char* e = getenv("MTT_FLOP_PERSISTENT")) {
+        if (atoi(e) && !B.pf_ok) {
+            set_error("mtt_batch_run: the persistent iteration is switched off in this build");
+            return MTT_ERR_ARG;
+        }
+        B.iter_mode = atoi(e) ? 1 : 0;
+        return MTT_OK;
+    }
+    if (!B.pf_ok) { B.iter_mode = 0; return MTT_OK; }
+    int bucket = 1;
+    while (bucket < B.n) bucket <<= 1;
+    const std::string key = "flop_iter_mode/nw" + std::to_string(B.D.L[2].ncolp / 32) + "/n" + std::to_string(bucket);
+    const int v = tune_lookup(key);
+    if (v == 0 || v == 1) { B.iter_mode = v; return MTT_OK; }
+    if (iters < 6 || !B.use_graph || B.timer.enabled) return MTT_OK;   // not yet: this call runs the phase launches
+    hipEvent_t a, b;
+    MTT_HIP(hipEventCreate(&a));
+    MTT_HIP(hipEventCreate(&b));
+    int rc = MTT_OK;
+    for (int m = 0; m < 2 && !rc; ++m) {
+        B.iter_mode = m;
+        batch_drop_graph(B);
+        rc = batch_iters(B, 1);
+        if (rc) break;
+        (void)hipEventRecord(a, B.stream);
+        rc = batch_iters(B, 2);
+        if (rc) break;
+        (void)hipEventRecord(b, B.stream);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        B.mode_ms[m] = ms / 2.0;
+        *ran += 3;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    const int best = B.mode_ms[1] < B.mode_ms[0] ? 1 : 0;
+    if (best != B.iter_mode) batch_drop_graph(B);
+    B.iter_mode = best;
+    tune_store(key, best);
+    return MTT_OK;
+}
+
+}  // namespace
+
+// Work items, completion counters and launch shape of the persistent form.
+// The form is switched off: its first GPU runs did not finish within three
+// minutes, twice (the second after the polling lane stopped invalidating the
+// L2 on every poll), while a CPU simulation of the same queue order and
+// counters completes for every workgroup count; the cause is not known, so
+// the three phase launches are used and MTT_FLOP_PERSISTENT=1 is refused.
+constexpr bool kPersistentFlopIteration = false;
+
+int batch_persistent_setup(BatchImpl& B) {
+    B.pf_ok = false;
+    if (!kPersistentFlopIteration) return MTT_OK;
+    const LevelDev& L = B.D.L[2];
+    B.pf_lds = std::max((size_t)L.maxterm * 104 * 8, bwd_lds(L, PF_NT));
+    B.pf_ok = B.pf_lds + sizeof(EvRec) * MAXE + 64 <= 160 * 1024;
+    if (!B.pf_ok) return MTT_OK;
+    // instances by tree size, largest first, in blocks: forward(k), GEMM(k - 1), backward(k - 2)
+    std::vector<int> ord(B.n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return B.trees[x].flat.size() > B.trees[y].flat.size(); });
+    constexpr int BK = 64;
+    const int nblk = (B.n + BK - 1) / BK;
+    std::vector<uint32_t> items;
+    auto emit = [&](int phase, int blk, int nsub) {
+        if (blk < 0 || blk >= nblk) return;
+        for (int i = blk * BK; i < std::min(B.n, (blk + 1) * BK); ++i)
+            for (int sub = 0; sub < nsub; ++sub)
+                items.push_back((uint32_t)phase << 28 | (uint32_t)ord[i] << 8 | (uint32_t)sub);
+    };
+    for (int st = 0; st < nblk + 2; ++st) {
+        emit(0, st, PF_NFWD);
+        emit(1, st - 1, PF_NGEMM);
+        emit(2, st - 2, PF_NBWD);
+    }
+    B.nitems = (int)items.size();
+    int rc = B.arena.upload(items, &B.d_items);
+    rc = rc ? rc : B.arena.alloc(4, &B.d_pq);
+    rc = rc ? rc : B.arena.alloc((size_t)B.n, &B.d_fdone);
+    rc = rc ? rc : B.arena.alloc((size_t)B.n, &B.d_gdone);
+    if (rc) return rc;
+    MTT_HIP(hipMemsetAsync(B.d_pq, 0, 4 * sizeof(int), B.stream));
+    MTT_HIP(hipMemsetAsync(B.d_fdone, 0, (size_t)B.n * sizeof(int), B.stream));
+    MTT_HIP(hipMemsetAsync(B.d_gdone, 0, (size_t)B.n * sizeof(int), B.stream));
+    // as many workgroups as stay resident together (any count would be correct)
+    const bool wide = L.maxslot > 64;
+    int per_cu = 0, dev = 0;
+    const void* fn = B.dm == 4 ? (wide ? reinterpret_cast<const void*>(&k_flop_persistent<4, 3>)
+                                       : reinterpret_cast<const void*>(&k_flop_persistent<4, 2>))
+                               : (wide ? reinterpret_cast<const void*>(&k_flop_persistent<8, 3>)
+                                       : reinterpret_cast<const void*>(&k_flop_persistent<8, 2>));
+    MTT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, PF_NT, B.pf_lds));
+    MTT_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    MTT_HIP(hipGetDeviceProperties(&prop, dev));
+    B.pf_grid = std::max(1, std::min(B.nitems, std::max(per_cu, 1) * prop.multiProcessorCount));
+    return MTT_OK;
+}
+
+int batch_run(BatchImpl& B, int iters) {
+    if (B.iters_done + iters > B.tmax) {
+        int rc = batch_weights(B, std::max(2 * B.tmax, B.iters_done + iters));
+        if (rc) return rc;
+        batch_drop_graph(B);
+    }
+    int ran = 0;
+    if (B.iter_mode < 0) {
+        int rc = batch_pick_mode(B, iters, &ran);
+        if (rc) return rc;
+    }
+    const bool tuning_pending = B.iter_mode < 0;
+    if (tuning_pending) B.iter_mode = 0;
+    int rc = batch_iters(B, iters - ran);
+    if (tuning_pending) B.iter_mode = -1;   // the graph of the phase launches stays until a call can tune
+    if (rc) return rc;
     MTT_HIP(hipStreamSynchronize(B.stream));
+    if (B.pf_ok) {
+        int flag = 0;
+        MTT_HIP(hipMemcpy(&flag, B.d_pq + 2, sizeof(int), hipMemcpyDeviceToHost));
+        if (flag) {
+            set_error("mtt_batch_run: a persistent iteration's dependency wait exceeded its bound");
+            return MTT_ERR_HIP;
+        }
+    }
     return MTT_OK;
 }
 
@@ -2666,7 +3224,19 @@ int mtt_batch_tuning(mtt_batch* h, int* hero_tiles, double* gemm_ms) {
     using namespace mtt;
     MTT_ARG(h && hero_tiles, "mtt_batch_tuning: bad arguments");
     *hero_tiles = h->impl->ht;
-    if (gemm_ms) { gemm_ms[0] = h->impl->gemm_ms[0]; gemm_ms[1] = h->impl->gemm_ms[1]; }
+    if (gemm_ms)
+        for (int k = 0; k < 5; ++k) gemm_ms[k] = h->impl->gemm_ms[k];
+    return MTT_OK;
+}
+
+int mtt_batch_iter_mode(mtt_batch* h, int* mode, double* ms) {
+    using namespace mtt;
+    MTT_ARG(h && mode, "mtt_batch_iter_mode: bad arguments");
+    *mode = h->impl->iter_mode;
+    if (ms) {
+        ms[0] = h->impl->mode_ms[0];
+        ms[1] = h->impl->mode_ms[1];
+    }
     return MTT_OK;
 }
 

@@ -268,6 +268,7 @@ for name, args in {
     "mtt_batch_exploitability_fp64": [_P, _I, _P],
     "mtt_batch_set_graph": [_P, _I],
     "mtt_batch_tuning": [_P, _P, _P],
+    "mtt_batch_iter_mode": [_P, _P, _P],
     "mtt_batch_timing": [_P, _I],
     "mtt_batch_kernel_times": [_P, _I, ctypes.c_char_p, _I, _P, _P],
     "mtt_batch_destroy": [_P],
@@ -351,9 +352,17 @@ class Batch:
 
     def tuning(self):
         ht = ctypes.c_int()
-        ms = (ctypes.c_double * 2)()
+        ms = (ctypes.c_double * 5)()
         mtt.call("mtt_batch_tuning", self.h, ctypes.byref(ht), ms)
-        return ht.value, (ms[0], ms[1])
+        return ht.value, tuple(ms)
+
+    def iter_mode(self):
+        """(form, (ms per iteration of form 0, form 1)): 0 = three launches
+        per iteration, 1 = the persistent launch, -1 = not chosen yet."""
+        m = ctypes.c_int()
+        ms = (ctypes.c_double * 2)()
+        mtt.call("mtt_batch_iter_mode", self.h, ctypes.byref(m), ms)
+        return m.value, (ms[0], ms[1])
 
     def timing(self, on):
         mtt.call("mtt_batch_timing", self.h, 1 if on else 0)

@@ -45,7 +45,7 @@ def test_batch_matches_oracle(solver_api, oracle, torch_dev):
         assert np.abs(Rg - Rc).max() <= STATE_REL * np.abs(Rc).max(), k
         assert np.abs(Sg - Sc).max() <= STATE_REL * np.abs(Sc).max(), k
     ht, ms = b.tuning()
-    assert ht in (2, 4)
+    assert ht in (2, 4, 12, 13, 14)
     b.close()
 
 
@@ -92,6 +92,75 @@ def test_batch_graph_replay_matches_eager(solver_api, torch_dev):
     e.close()
 
 
+def test_flop_gemm_forms_agree(solver_api, torch_dev, monkeypatch):
+    """Every flop showdown GEMM form the autotuner chooses from (W - L staged
+    in LDS with 2 or 4 hero tiles per workgroup; register-streamed with 2, 3
+    or 4 hero tiles per wave) adds the four BF16 products of a K step in the
+    same order per accumulator, so the states after 4 iterations are
+    identical for every instance (one or two column tiles; the last hero
+    group holds 2 of 2, 3 or 4 tiles, and nothing may land outside an
+    instance's own values)."""
+    S = solver_api
+    games, ranges = S.config4(1, 24)
+    ref = None
+    for form in ("2", "4", "12", "13", "14"):
+        monkeypatch.setenv("MTT_FLOP_GEMM", form)
+        b = S.Batch(games, ranges)
+        assert b.tuning()[0] == int(form)
+        b.run(4)
+        st = [b.state(k) for k in range(len(games))]
+        b.close()
+        print(f"flop GEMM form {form} done", flush=True)
+        if ref is None:
+            ref = st
+        else:
+            for (Ra, Sa), (Rb, Sb) in zip(st, ref):
+                assert np.array_equal(Ra, Rb) and np.array_equal(Sa, Sb), form
+    monkeypatch.delenv("MTT_FLOP_GEMM")
+
+
+@pytest.mark.skip(reason="the persistent flop iteration is switched off: its first GPU runs did not finish within "
+                         "three minutes (twice), while a CPU simulation of its queue order and counters completes; "
+                         "cause unknown, not run again")
+def test_persistent_iteration_matches_phase_launches(solver_api, torch_dev, monkeypatch):
+    """The persistent iteration (one launch per iteration: workgroups take the
+    forward halves, GEMM hero groups and backward slices of every instance
+    from a queue, waiting on per-instance completion counters) runs the same
+    bodies as the three phase launches, so the states are identical: on the
+    512-instance configuration's 8 widest and its deepest trees mixed with 56
+    others (three register vectors, 8-level stacks), graph-replayed and
+    launched one by one, over several runs."""
+    S = solver_api
+    games, ranges = S.config4(0, 512)
+    info = []
+    for k, g in enumerate(games):
+        nodes, _, _ = S.tree(g)
+        info.append((int(nodes["num_actions"][nodes["type"] == 0].sum()), len(nodes), k))
+    picks = sorted({k for _, _, k in sorted(info)[-8:]} | {max(info, key=lambda x: x[1])[2]} | set(range(56)))
+    sub, rsub = [games[k] for k in picks], ranges[picks]
+    a = S.Batch(sub, rsub)
+    p = S.Batch(sub, rsub)
+    e = S.Batch(sub, rsub)
+    e.set_graph(False)
+    for step in (1, 2, 5):
+        monkeypatch.setenv("MTT_FLOP_PERSISTENT", "0")
+        a.run(step)
+        monkeypatch.setenv("MTT_FLOP_PERSISTENT", "1")
+        p.run(step)
+        e.run(step)
+        assert a.iter_mode()[0] == 0 and p.iter_mode()[0] == 1 and e.iter_mode()[0] == 1
+        for j in range(len(picks)):
+            Ra, Sa = a.state(j)
+            for o in (p, e):
+                Ro, So = o.state(j)
+                assert np.array_equal(Ra, Ro) and np.array_equal(Sa, So), (step, picks[j])
+        print(f"persistent vs phase launches: {len(picks)} instances equal after {a.iterations} iterations",
+              flush=True)
+    monkeypatch.delenv("MTT_FLOP_PERSISTENT")
+    for b in (a, p, e):
+        b.close()
+
+
 def test_config4_full_size(solver_api, oracle, torch_dev):
     """512 seeded instances, DCFR(1.5, 0, 2), 300 iterations in graph
     replay.  Instances 0, 311 and the last (511) match the oracle after 3
@@ -116,7 +185,9 @@ def test_config4_full_size(solver_api, oracle, torch_dev):
     t = time.time()
     b.run(292)
     dt = time.time() - t
-    print(f"config4: {292 / dt:.1f} iterations/s over 512 instances (graph replay); tuning {b.tuning()}", flush=True)
+    print(f"config4: {292 / dt:.1f} iterations/s over 512 instances (graph replay); GEMM tuning {b.tuning()}; "
+          f"iteration form {b.iter_mode()}", flush=True)
+    assert b.iter_mode()[0] == 0
     b.timing(True)
     b.run(5)
     kt = b.kernel_times()
