@@ -1125,7 +1125,9 @@ void k_bt_gemm7(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical, int
     const int ct = rem / NHG, hg = rem % NHG;
     const int lane = threadIdx.x, half = lane >> 5, l32 = lane & 31;
     const int ht0 = hg * GH;
-    const int nht = __builtin_amdgcn_readfirstlane(min(GH, NTH - ht0));   // GH, or fewer in the last group (every tile guarded)
+    const int nht = __builtin_amdgcn_readfirstlane(min(GH, NTH - ht0));   // GH, or fewer in the last group
+    // when the last group lacks at most one tile only that tile needs its guard (the others then form one block)
+    constexpr bool kOneShort = NTH % GH == 0 || NTH % GH == GH - 1;
     const int ncolp = D.L[1].ncolp;
     const size_t row = ((size_t)b * ncolp + ct * 32 + l32) * NPR;
     const uint4* rh = reinterpret_cast<const uint4*>(D.Bh + row) + 4 * half;
@@ -1160,11 +1162,11 @@ void k_bt_gemm7(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical, int
             _Pragma("unroll") for (int t = 0; t < GH; ++t) {                            \
                 const uint32_t w = ks < 2 ? S_[t].x : S_[t].y;                          \
                 sf[t] = s2_frag(w, ks & 1);                                             \
-                if (t < nht)                                              \
+                if ((kOneShort && t < GH - 1) || t < nht)                                              \
                     acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, sf[t], acc[t], 0, 0, 0); \
             }                                                                           \
             _Pragma("unroll") for (int t = 0; t < GH; ++t) {                            \
-                if (t < nht)                                              \
+                if ((kOneShort && t < GH - 1) || t < nht)                                              \
                     acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, sf[t], acc[t], 0, 0, 0); \
             }                                                                           \
         }                                                                               \
@@ -1183,7 +1185,7 @@ void k_bt_gemm7(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical, int
     float* Cb = D.C + ((size_t)b * ncolp + ct * 32) * NPR;
 #pragma unroll
     for (int t = 0; t < GH; ++t) {
-        if (t < nht) {
+        if ((kOneShort && t < GH - 1) || t < nht) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int cc = (r >> 2) * 8 + half * 4 + (r & 3);
@@ -2000,11 +2002,13 @@ int river_gemm_autotune(BatchedState& B, hipStream_t st) {
     MTT_HIP(hipEventCreate(&b));
     float best = 1e30f;
     int rc = MTT_OK;
+    // the clocks ramp up under load: a few untimed launches first, so the first candidate is not penalised
+    for (int r = 0; r < 4 && !rc; ++r) rc = river_gemm(B, cand[0], st);
     for (int shape : cand) {
-        rc = river_gemm(B, shape, st);  // warm
+        rc = rc ? rc : river_gemm(B, shape, st);  // warm
         if (rc) break;
         (void)hipEventRecord(a, st);
-        for (int r = 0; r < 2 && !rc; ++r) rc = river_gemm(B, shape, st);
+        for (int r = 0; r < 3 && !rc; ++r) rc = river_gemm(B, shape, st);
         if (rc) break;
         (void)hipEventRecord(b, st);
         (void)hipEventSynchronize(b);
@@ -2344,7 +2348,8 @@ __device__ __forceinline__ void gemm_flop7_wave(const BatchedDev& D, const int16
     constexpr int NTH = NPF / 32;                 // 38 hero tiles
     const int lane = threadIdx.x & 63, half = lane >> 5, l32 = lane & 31;
     const int ht0 = hg * HT;
-    const int nht = __builtin_amdgcn_readfirstlane(min(HT, NTH - ht0));    // HT, or fewer in the last group (every tile guarded)
+    const int nht = __builtin_amdgcn_readfirstlane(min(HT, NTH - ht0));    // HT, or fewer in the last group
+    constexpr bool kOneShort = NTH % HT == 0 || NTH % HT == HT - 1;    // at most one tile absent: guard only the last
     const int nct = __builtin_amdgcn_readfirstlane((bncol[b] + 31) >> 5);
     const int ncolp = D.L[2].ncolp;
     // W - L of hero tile t: uint4 [b][chunk][hero][4], this lane's half = pieces 2h, 2h + 1
@@ -2386,13 +2391,13 @@ __device__ __forceinline__ void gemm_flop7_wave(const BatchedDev& D, const int16
                     __builtin_memcpy(&sl[t], &lo, 16);                                            \
                 }                                                                                 \
                 _Pragma("unroll") for (int t = 0; t < HT; ++t)                                    \
-                    if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, sh[t], acc[t], 0, 0, 0); \
+                    if ((kOneShort && t < HT - 1) || t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, sh[t], acc[t], 0, 0, 0); \
                 _Pragma("unroll") for (int t = 0; t < HT; ++t)                                    \
-                    if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, sh[t], acc[t], 0, 0, 0); \
+                    if ((kOneShort && t < HT - 1) || t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, sh[t], acc[t], 0, 0, 0); \
                 _Pragma("unroll") for (int t = 0; t < HT; ++t)                                    \
-                    if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, sl[t], acc[t], 0, 0, 0); \
+                    if ((kOneShort && t < HT - 1) || t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, sl[t], acc[t], 0, 0, 0); \
                 _Pragma("unroll") for (int t = 0; t < HT; ++t)                                    \
-                    if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, sl[t], acc[t], 0, 0, 0); \
+                    if ((kOneShort && t < HT - 1) || t < nht) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, sl[t], acc[t], 0, 0, 0); \
             }                                                                                     \
         } while (0)
         constexpr int LAST = FNCH - 1;
@@ -2414,7 +2419,7 @@ __device__ __forceinline__ void gemm_flop7_wave(const BatchedDev& D, const int16
         float* Cb = D.C + ((size_t)b * ncolp + ct * 32) * NPF;
 #pragma unroll
         for (int t = 0; t < HT; ++t) {
-            if (t < nht) {
+            if ((kOneShort && t < HT - 1) || t < nht) {
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const int cc = (r >> 2) * 8 + half * 4 + (r & 3);
@@ -2714,6 +2719,10 @@ int batch_autotune(BatchImpl& B) {
     MTT_HIP(hipEventCreate(&a));
     MTT_HIP(hipEventCreate(&b));
     int best = 0;
+    for (int r = 0; r < 4; ++r) {   // the clocks ramp up under load: untimed launches first
+        int rc = batch_launch_gemm(B, kCand[0]);
+        if (rc) return rc;
+    }
     for (int k = 0; k < 5; ++k) {
         int rc = batch_launch_gemm(B, kCand[k]);  // warm
         if (rc) return rc;
