@@ -148,6 +148,10 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int sl
     o("const int c1r = L.cc[((size_t)b * NP + kc) * 2], c2r = L.cc[((size_t)b * NP + kc) * 2 + 1];");
     o("const int ri = root_index(D, LEVEL, b, kc);");
     for (int j = 0; j < T.nsl; ++j) o("const float Rv%d = Rl[%d];", j, j * J.np);
+    // the strategy sums are read with the regrets and updated by plain stores: each (slot, combo) has one
+    // writer per iteration, and a float atomic per entry cost the river forward ~22% of its time
+    o("const float* __restrict__ Sl = L.S + ((size_t)b * L.nslot + %d) * NP + kc;", T.s0);
+    for (int j = 0; j < T.nsl; ++j) o("const float Sv%d = Sl[%d];", j, j * J.np);
     o("const float rr0 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 0);");
     o("const float rr1 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 1);");
     o("const float root0 = live ? rr0 : 0.0f, root1 = live ? rr1 : 0.0f;");
@@ -166,11 +170,13 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int sl
             stack[d] = n;
             emit_strategy(o, T, n);
             const int ls = nd.slot - T.s0;
-            // a zero reach adds +0 to every strategy sum (all sums are >= +0), so its atomics are skipped
+            // a zero reach adds +0 to every strategy sum (all sums are >= +0), so its stores are skipped; the
+            // product is rounded before the sum, as an atomic add of it would (no contraction into an FMA)
             o("if (live) {");
             o("const float wr = ws * x%d_%d;", nd.player, n);
             o("if (wr != 0.0f) {");
-            for (int a = 0; a < nd.nact; ++a) o("unsafeAtomicAdd(Sg + %d, wr * s%d_%d);", (ls + a) * J.np, n, a);
+            o("#pragma clang fp contract(off)");
+            for (int a = 0; a < nd.nact; ++a) o("Sg[%d] = Sv%d + wr * s%d_%d;", (ls + a) * J.np, ls + a, n, a);
             o("}");
             o("}");
             continue;

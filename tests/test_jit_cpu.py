@@ -25,9 +25,14 @@ def test_sweep_source_has_one_case_per_template(S):
     assert n_river >= 2
     for k in ("mtt_jit_fwd0", "mtt_jit_bwd0", "mtt_jit_fwd1", "mtt_jit_bwd1"):
         assert f"void {k}(" in src
-    # every regret slot of the river level is loaded exactly once per template in each sweep
+    # every action slot of a river template is read once for its regret and once for its strategy sum,
+    # and the strategy sum is written back by a plain store (one thread owns each combo): no atomics
     fwd1 = src.split("void mtt_jit_fwd1(")[1].split("void mtt_jit_bwd1(")[0]
-    assert fwd1.count("unsafeAtomicAdd(") > 0
+    n_sv = len(re.findall(r"const float Sv\d+ = Sl\[", fwd1))
+    assert n_sv > 0
+    assert n_sv == len(re.findall(r"const float Rv\d+ = Rl\[", fwd1))
+    assert n_sv == len(re.findall(r"\bSg\[\d+\] = Sv\d+ \+ wr \* ", fwd1))
+    assert "Atomic" not in fwd1
     assert "#pragma once" not in src
 
 
