@@ -1114,15 +1114,8 @@ constexpr int G7_NCH = NPR / 64;                 // K chunks of 64 villains
 static_assert(NPR % 64 == 0, "64-villain chunks");
 
 template <int GH>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-void k_bt_gemm7(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical, int ntiles) {
+__device__ __forceinline__ void gemm7_tile(const BatchedDev& D, const uint32_t* __restrict__ S2, int b, int ct, int hg) {
     constexpr int NTH = NPR / 32;                // hero tiles per board
-    constexpr int NHG = (NTH + GH - 1) / GH;     // hero groups per board
-    const int per = (nlogical + 7) >> 3;
-    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (logical >= nlogical) return;
-    const int b = logical / (ntiles * NHG), rem = logical % (ntiles * NHG);
-    const int ct = rem / NHG, hg = rem % NHG;
     const int lane = threadIdx.x, half = lane >> 5, l32 = lane & 31;
     const int ht0 = hg * GH;
     const int nht = __builtin_amdgcn_readfirstlane(min(GH, NTH - ht0));   // GH, or fewer in the last group
@@ -1193,6 +1186,108 @@ void k_bt_gemm7(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical, int
             }
         }
     }
+}
+
+// The last column tile of a board when it holds at most 16 live columns
+// (262 = 8 x 32 + 6 on the benchmark game), as 16-column products
+// v_mfma_f32_16x16x32_bf16 over 16-hero tiles: half the MFMA work of the
+// 32-column tile, whose other 16 columns stay zero.  One single-wave
+// workgroup per (board, the 2 GH 16-hero tiles of a hero group).  Lane l:
+// row or column l & 15 of a fragment, K group g = l >> 4 (villains
+// 8g .. 8g+7 of a 32-villain K step).  Per 64-villain chunk, K step ks
+// (0, 1) covers 32-villain chunk 2c + ks: the reach of villains 8g .. +7
+// (one uint4 per plane) and, per 16-hero tile, the S word of (chunk, hero,
+// half g >> 1) decoded at its step g & 1.  Per accumulator the order is
+// chunk, K step, high plane, low plane, whatever GH is.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <int GH>
+__device__ __forceinline__ void gemm7t_tile(const BatchedDev& D, const uint32_t* __restrict__ S2, int b, int ct, int hg) {
+    constexpr int NT16 = 2 * GH;                  // 16-hero tiles per wave
+    constexpr int NT16_ALL = NPR / 16;
+    const int lane = threadIdx.x, g = lane >> 4, l16 = lane & 15;
+    const int h0 = hg * NT16;
+    const int nht = __builtin_amdgcn_readfirstlane(min(NT16, NT16_ALL - h0));
+    const int ncolp = D.L[1].ncolp;
+    const size_t row = ((size_t)b * ncolp + ct * 32 + l16) * NPR;
+    const uint4* rh = reinterpret_cast<const uint4*>(D.Bh + row) + g;   // + 4 per 32-villain chunk
+    const uint4* rl = reinterpret_cast<const uint4*>(D.Bl + row) + g;
+    const uint32_t* sw = S2 + (size_t)b * GNCH * NPR * 2 + (g >> 1);
+    int so[NT16];
+#pragma unroll
+    for (int t = 0; t < NT16; ++t) so[t] = (min(h0 + t, NT16_ALL - 1) * 16 + l16) * 2;
+    f32x4_t acc[NT16];
+#pragma unroll
+    for (int t = 0; t < NT16; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[t][r] = 0.0f;
+    uint4 ah[2], al[2], bh[2], bl[2];
+    uint32_t as[2][NT16], bs[2][NT16];
+#define G7T_LOAD(C_, H_, L_, S_)                                                        \
+    do {                                                                                \
+        const int c_ = (C_);                                                            \
+        _Pragma("unroll") for (int ks = 0; ks < 2; ++ks) {                              \
+            H_[ks] = rh[4 * (2 * c_ + ks)];                                             \
+            L_[ks] = rl[4 * (2 * c_ + ks)];                                             \
+            _Pragma("unroll") for (int t = 0; t < NT16; ++t)                            \
+                S_[ks][t] = sw[(size_t)(2 * c_ + ks) * NPR * 2 + so[t]];                \
+        }                                                                               \
+    } while (0)
+#define G7T_MMA(H_, L_, S_)                                                             \
+    do {                                                                                \
+        _Pragma("unroll") for (int ks = 0; ks < 2; ++ks) {                              \
+            bf16x8_t fh, fl, sf[NT16];                                                  \
+            __builtin_memcpy(&fh, &H_[ks], 16);                                         \
+            __builtin_memcpy(&fl, &L_[ks], 16);                                         \
+            _Pragma("unroll") for (int t = 0; t < NT16; ++t) {                          \
+                sf[t] = s2_frag(S_[ks][t], g & 1);                                      \
+                if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh, sf[t], acc[t], 0, 0, 0); \
+            }                                                                           \
+            _Pragma("unroll") for (int t = 0; t < NT16; ++t)                            \
+                if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fl, sf[t], acc[t], 0, 0, 0); \
+        }                                                                               \
+    } while (0)
+    G7T_LOAD(0, ah, al, as);
+    int c = 0;
+    for (; c + 1 < G7_NCH; c += 2) {
+        G7T_LOAD(c + 1, bh, bl, bs);
+        G7T_MMA(ah, al, as);
+        G7T_LOAD(min(c + 2, G7_NCH - 1), ah, al, as);
+        G7T_MMA(bh, bl, bs);
+    }
+    if (c < G7_NCH) G7T_MMA(ah, al, as);
+#undef G7T_LOAD
+#undef G7T_MMA
+    float* Cb = D.C + ((size_t)b * ncolp + ct * 32) * NPR;
+#pragma unroll
+    for (int t = 0; t < NT16; ++t) {
+        if (t < nht) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Cb[(size_t)(4 * g + r) * NPR + (h0 + t) * 16 + l16] = 2.0f * acc[t][r];
+        }
+    }
+}
+
+// The register-streamed river GEMM launch: per board, nmain 32-column tiles
+// x NHG hero groups of GH 32-hero tiles, then, when the board's last column
+// tile holds at most 16 live columns (ntail = 1), that tile as NHGT groups
+// of 2 x G7T_GH 16-hero tiles.  Workgroups are numbered (board, item) with
+// all of a board's items on one XCD, the tail items last.
+constexpr int G7T_GH = 5;   // 10 16-hero tiles per tail wave: 206 VGPRs, no spill
+
+template <int GH>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_bt_gemm7(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical, int nmain, int ntail) {
+    constexpr int NTH = NPR / 32;
+    constexpr int NHG = (NTH + GH - 1) / GH;
+    constexpr int NHGT = (NTH + G7T_GH - 1) / G7T_GH;
+    const int per = (nlogical + 7) >> 3;
+    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (logical >= nlogical) return;
+    const int nm = nmain * NHG, pb = nm + ntail * NHGT;
+    const int b = logical / pb, rem = logical % pb;
+    if (rem < nm) gemm7_tile<GH>(D, S2, b, rem / NHG, rem % NHG);
+    else gemm7t_tile<G7T_GH>(D, S2, b, nmain, rem - nm);
 }
 
 // Turn all-in showdowns: C0[t][col][i] = sum_j E[t][j][i] * B0[t][col][j].
@@ -1906,12 +2001,15 @@ int river_gemm(const BatchedState& B, int shape, hipStream_t st) {
     const int ntiles = B.D.L[1].ncolp / 32;
     if (shape >= 700 && shape < 800) {
         const int gh = shape - 700, nhg = (NPR / 32 + gh - 1) / gh;
-        const int nlog = B.nb1 * ntiles * nhg;
+        // a last column tile with at most 16 live columns is computed as 16-column products
+        const int ntail = ntiles > 1 && B.D.L[1].ncol - 32 * (ntiles - 1) <= 16 ? 1 : 0;
+        const int nmain = ntiles - ntail;
+        const int nlog = B.nb1 * (nmain * nhg + ntail * ((NPR / 32 + G7T_GH - 1) / G7T_GH));
         const dim3 g(((nlog + 7) / 8) * 8);
         switch (gh) {
-        case 5: hipLaunchKernelGGL(k_bt_gemm7<5>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, ntiles); break;
-        case 6: hipLaunchKernelGGL(k_bt_gemm7<6>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, ntiles); break;
-        case 7: hipLaunchKernelGGL(k_bt_gemm7<7>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, ntiles); break;
+        case 5: hipLaunchKernelGGL(k_bt_gemm7<5>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, nmain, ntail); break;
+        case 6: hipLaunchKernelGGL(k_bt_gemm7<6>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, nmain, ntail); break;
+        case 7: hipLaunchKernelGGL(k_bt_gemm7<7>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, nmain, ntail); break;
         default: set_error("river GEMM shape %d not available", shape); return MTT_ERR_ARG;
         }
         MTT_HIP(hipGetLastError());
