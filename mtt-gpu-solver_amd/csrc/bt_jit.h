@@ -60,6 +60,10 @@ struct JitConfig {
     // turn level keeps the two half workgroups (49 boards give too few
     // workgroups to slice).
     int fwd_nt = 384;
+    // river forward strategy sums: 0 = read with the regrets in the prologue; k > 0 = read, k decisions
+    // at a time once their reaches are known, by bounded buffer loads that skip lanes whose acting
+    // player has zero reach
+    int fwd_smask = 0;
 };
 
 // Passes of the backward of a level (0: turn, 1: river) under cfg.

@@ -125,7 +125,7 @@ void emit_bins(Src& o, const Tmpl& T, int n) {
     o("}");
 }
 
-void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int slices, int t) {
+void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int slices, int smask, int t) {
     Tmpl T(J, t);
     o("case %d: {", t);
     o("fwd_bins_clear<NT>(bins, %d);", T.nt * nbins);
@@ -148,16 +148,53 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int sl
     o("const int c1r = L.cc[((size_t)b * NP + kc) * 2], c2r = L.cc[((size_t)b * NP + kc) * 2 + 1];");
     o("const int ri = root_index(D, LEVEL, b, kc);");
     for (int j = 0; j < T.nsl; ++j) o("const float Rv%d = Rl[%d];", j, j * J.np);
-    // the strategy sums are read with the regrets and updated by plain stores: each (slot, combo) has one
-    // writer per iteration, and a float atomic per entry cost the river forward ~22% of its time
-    o("const float* __restrict__ Sl = L.S + ((size_t)b * L.nslot + %d) * NP + kc;", T.s0);
-    for (int j = 0; j < T.nsl; ++j) o("const float Sv%d = Sl[%d];", j, j * J.np);
+    // the strategy sums are updated by plain loads and stores: each (slot, combo) has one writer per
+    // iteration, and a float atomic per entry cost the river forward ~22% of its time.  Either they are
+    // read with the regrets, or (smask) each decision reads its own through a bounded buffer descriptor
+    // once the acting player's reach is known, with lanes of zero reach sent out of range: those loads
+    // return 0 without touching memory, and those lanes store nothing
+    if (smask) {
+        o("const __amdgpu_buffer_rsrc_t rsS = __builtin_amdgcn_make_buffer_rsrc("
+          "(void*)(L.S + ((size_t)b * L.nslot + %d) * NP), (short)0, %d, 0x00020000);",
+          T.s0, T.nsl * J.np * 4);
+    } else {
+        o("const float* __restrict__ Sl = L.S + ((size_t)b * L.nslot + %d) * NP + kc;", T.s0);
+        for (int j = 0; j < T.nsl; ++j) o("const float Sv%d = Sl[%d];", j, j * J.np);
+    }
     o("const float rr0 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 0);");
     o("const float rr1 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 1);");
     o("const float root0 = live ? rr0 : 0.0f, root1 = live ? rr1 : 0.0f;");
     o("const int c1 = live ? c1r : 0, c2 = live ? c2r : 0;");
     for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rv%d : 0.0f;", j, j);
     if (slices == 1) o("__syncthreads();");
+    // smask: strategy-sum updates of decisions deferred in groups of smask, each group's loads issued
+    // together and then its stores (a load cannot pass an earlier store to the same descriptor), all
+    // branch-free: the loads and the stores of a zero-reach lane (not-live lanes carry zero reach) both
+    // fall outside the descriptor, so the loads return 0 and the stores are dropped
+    std::vector<int> pend;
+    auto flush_sums = [&]() {
+        if (pend.empty()) return;
+        o("{");
+        o("#pragma clang fp contract(off)");
+        for (int n : pend) {
+            o("const float wr%d = ws * x%d_%d;", n, T.node(n).player, n);
+            o("const uint32_t so%d = wr%d != 0.0f ? (uint32_t)kc * 4u : 0x80000000u;", n, n);
+        }
+        for (int n : pend) {
+            const int ls = T.node(n).slot - T.s0;
+            for (int a = 0; a < T.node(n).nact; ++a)
+                o("const float Sv%d = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsS, so%d, "
+                  "%d, 0));", ls + a, n, (ls + a) * J.np * 4);
+        }
+        for (int n : pend) {
+            const int ls = T.node(n).slot - T.s0;
+            for (int a = 0; a < T.node(n).nact; ++a)
+                o("__builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, Sv%d + wr%d * s%d_%d), rsS, "
+                  "so%d, %d, 0);", ls + a, n, n, a, n, (ls + a) * J.np * 4);
+        }
+        o("}");
+        pend.clear();
+    };
     int stack[16];
     for (int e = T.e0; e < T.e1; ++e) {
         const int ev = (*J.events)[e], n = ev >> 2, kind = ev & 3;
@@ -172,9 +209,15 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int sl
             const int ls = nd.slot - T.s0;
             // a zero reach adds +0 to every strategy sum (all sums are >= +0), so its stores are skipped; the
             // product is rounded before the sum, as an atomic add of it would (no contraction into an FMA)
-            o("if (live) {");
-            o("const float wr = ws * x%d_%d;", nd.player, n);
-            o("if (wr != 0.0f) {");
+            if (smask) {
+                pend.push_back(n);
+                if ((int)pend.size() >= smask) flush_sums();
+                continue;
+            } else {
+                o("if (live) {");
+                o("const float wr = ws * x%d_%d;", nd.player, n);
+                o("if (wr != 0.0f) {");
+            }
             o("#pragma clang fp contract(off)");
             for (int a = 0; a < nd.nact; ++a) o("Sg[%d] = Sv%d + wr * s%d_%d;", (ls + a) * J.np, ls + a, n, a);
             o("}");
@@ -210,6 +253,7 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int sl
         }
         emit_bins(o, T, n);
     }
+    flush_sums();
     if (slices > 1) o("}");  // pass loop; the sums of all slices go to the first half of the partials
     const char* half = slices > 1 ? "0" : "half";
     if (nbins == 2) o("fwd_bins_flush2<NT>(L, bins, b, %d, %s, %d);", T.t0, half, T.nt);
@@ -382,7 +426,7 @@ void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
         o("(void)inb;");
     }
     o("switch (tm) {");
-    for (int t = 0; t < J.ntmpl; ++t) gen_forward_case(o, J, cfg.sd_fp32 != 0, cfg.fwd_bins, fsl, t);
+    for (int t = 0; t < J.ntmpl; ++t) gen_forward_case(o, J, cfg.sd_fp32 != 0, cfg.fwd_bins, fsl, J.level == 1 ? cfg.fwd_smask : 0, t);
     o("default: break;");
     o("}");
     o("}");
@@ -466,6 +510,7 @@ JitConfig bt_jit_config() {
     c.bwd_passes = env("MTT_JIT_BWD_PASSES", 1, 2, c.bwd_passes);
     c.fwd_bins = env("MTT_JIT_FWD_BINS", 1, 2, c.fwd_bins);
     c.fwd_nt = env("MTT_JIT_FWD_NT", 192, 576, c.fwd_nt);
+    c.fwd_smask = env("MTT_JIT_FWD_SMASK", 0, 64, c.fwd_smask);
     if (c.fwd_nt % 64) c.fwd_nt = JitConfig().fwd_nt;
     return c;
 }

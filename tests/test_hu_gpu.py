@@ -248,6 +248,33 @@ def test_generated_sweeps_match_interpreted(solver_api, torch_dev, which, monkey
     b.close()
 
 
+@pytest.mark.parametrize("which", ["config3_small", "config3"])
+def test_forward_strategy_sum_forms_identical(solver_api, torch_dev, which, monkeypatch):
+    """The generated river forward reads the strategy sums either with the
+    regrets in its prologue (MTT_JIT_FWD_SMASK=0) or k decisions at a time
+    through a bounded buffer descriptor that sends the lanes of zero reach
+    out of range (k = 1, 4, 64): the same sums in the same rounding, so the
+    states are identical, also after 40 iterations when many strategies are
+    pure and whole subtrees carry zero reach."""
+    S = solver_api
+    g, r0, r1 = getattr(S, which)(0 if which != "config3_small" else 1)
+    ref = None
+    for k in ("0", "1", "4", "64"):
+        monkeypatch.setenv("MTT_JIT_FWD_SMASK", k)
+        s = S.Solver(g, r0, r1, S.MODE_BF16_SPLIT)
+        assert s.jit_info()[0]
+        s.run(40)
+        st = s.state()
+        s.close()
+        print(f"{which}: strategy-sum form {k} done", flush=True)
+        if ref is None:
+            ref = st
+            assert np.any(ref[1] > 0)
+        else:
+            assert np.array_equal(st[0], ref[0]) and np.array_equal(st[1], ref[1]), k
+    monkeypatch.delenv("MTT_JIT_FWD_SMASK")
+
+
 @pytest.mark.parametrize("river", ["gemm", "scan"])
 def test_batched_config3_matches_oracle(solver_api, oracle, torch_dev, river, monkeypatch):
     """The benchmark's turn+river game (33%/75%/2.5x/all-in per street, nine
@@ -359,7 +386,7 @@ def test_plain_config3_matches_oracle(solver_api, oracle, torch_dev):
     b.close()
 
 
-@pytest.mark.parametrize("mode", ["sweep", "scan"])
+@pytest.mark.parametrize("mode", ["sweep", "scan", "gemm"])
 @pytest.mark.parametrize("which", ["config1", "config3_small", "config3"])
 def test_river_sweep_matches_dense_showdown(solver_api, torch_dev, which, mode, monkeypatch):
     """River showdown values by the strength-order sweep (MTT_RIVER_SD=sweep)
@@ -372,9 +399,17 @@ def test_river_sweep_matches_dense_showdown(solver_api, torch_dev, which, mode, 
     rounding (|err| <= 2^-23 |exact| + 2e-12).  The scan sums exactly in
     integer units u = 2^(e - 27) of the column (2^e > the column total), so
     each value is off by at most half a unit per live combo plus the final
-    rounding, and on average by far less.  Padded combos stay 0."""
+    rounding, and on average by far less.  The register-streamed BF16 hi + lo
+    GEMM (MTT_RIVER_GEMM=707: every column tile, a last tile of at most 16
+    columns by the 16-column tail kernel) agrees to its FP32
+    accumulation: within 2e-5 of the column's reach total.  Padded combos
+    stay 0."""
     monkeypatch.delenv("MTT_RIVER_GEMM", raising=False)
-    monkeypatch.setenv("MTT_RIVER_SD", mode)
+    if mode == "gemm":
+        monkeypatch.delenv("MTT_RIVER_SD", raising=False)
+        monkeypatch.setenv("MTT_RIVER_GEMM", "707")
+    else:
+        monkeypatch.setenv("MTT_RIVER_SD", mode)
     S = solver_api
     g, r0, r1 = getattr(S, which)(1 if which == "config3_small" else 0)
     s = S.Solver(g, r0, r1, S.MODE_BF16_SPLIT)
@@ -395,7 +430,12 @@ def test_river_sweep_matches_dense_showdown(solver_api, torch_dev, which, mode, 
         print(f"{which} board {board} ({mode}): {nl} combos, {reach.shape[0]} columns, max |value| "
               f"{np.abs(exact).max():.4g}, max err {err.max():.3g}", flush=True)
         assert np.abs(exact).max() > 0
-        if mode == "sweep":
+        if mode == "gemm":
+            # the BF16 hi + lo reach the GEMM multiplies is what the export returns (hi + lo), so only the
+            # FP32 accumulation of the MFMAs (2 x 68 K steps of 16 products) separates it from the FP64 sum
+            tot = reach[:, :nl].astype(np.float64).sum(axis=1)[:, None]
+            assert np.all(err <= 2e-5 * tot + 1e-12), (board, float((err / (tot + 1e-30)).max()))
+        elif mode == "sweep":
             bound = 2.0 ** -23 * np.abs(exact) + 2e-12
             if not np.all(err <= bound):
                 ci, hi = np.unravel_index(np.argmax(err / bound), err.shape)
