@@ -278,6 +278,20 @@ void emit_up(Src& o, const Tmpl& T, int n, int p) {
 
 void gen_backward_case(Src& o, const JitLevel& J, const JitConfig& cfg, int passes, int t) {
     Tmpl T(J, t);
+    // rpre > 0: a decision's regrets are loaded at the entry of the decision rpre places before it in
+    // walk order (the first rpre in the prologue) instead of all in the prologue, so only the regrets of
+    // the open decisions and of the next few stay live
+    const int rpre = passes > 1 ? cfg.bwd_rpre : 0;
+    std::vector<int> dec;
+    for (int e = T.e0; e < T.e1; ++e)
+        if (((*J.events)[e] & 3) == bt::EV_ENTER) dec.push_back((*J.events)[e] >> 2);
+    auto emit_rload = [&](int n) {
+        const int ls = T.node(n).slot - T.s0;
+        // unconditional loads from the clamped combo index, dead lanes zeroed after (no branch per load)
+        for (int a = 0; a < T.node(n).nact; ++a) o("const float Rv%d = Rl[%d];", ls + a, (ls + a) * J.np);
+        for (int a = 0; a < T.node(n).nact; ++a) o("const float R%d = live ? Rv%d : 0.0f;", ls + a, ls + a);
+    };
+    int nenter = 0;
     o("case %d: {", t);
     if (passes > 1) {
         // card sums and totals once; then each slice of the combos in turn (the showdown values a thread stages
@@ -302,14 +316,18 @@ void gen_backward_case(Src& o, const JitLevel& J, const JitConfig& cfg, int pass
         o("const int c1r = L.cc[((size_t)b * NP + kc) * 2], c2r = L.cc[((size_t)b * NP + kc) * 2 + 1];");
         o("const int ri = root_index(D, LEVEL, b, kc);");
         o("const float* __restrict__ Rl = L.R + ((size_t)b * L.nslot + %d) * NP + kc;", T.s0);
-        for (int j = 0; j < T.nsl; ++j) o("const float Rv%d = Rl[%d];", j, j * J.np);
+        if (rpre == 0)
+            for (int j = 0; j < T.nsl; ++j) o("const float Rv%d = Rl[%d];", j, j * J.np);
+        else
+            for (int i = 0; i < std::min(rpre, (int)dec.size()); ++i) emit_rload(dec[i]);
         if (T.ncolt > 0) o("bwd_cs_n<LEVEL, NTB, %d>(D, L, Cs, b, kc, live, %d);", T.ncolt, T.c0);
         o("const float rr0 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 0);");
         o("const float rr1 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 1);");
         o("const int c1 = live ? c1r : 0, c2 = live ? c2r : 0;");
         o("const float root0 = live ? rr0 : 0.0f, root1 = live ? rr1 : 0.0f;");
         o("float* __restrict__ Rg = L.R + ((size_t)b * L.nslot + %d) * NP + k;", T.s0);
-        for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rv%d : 0.0f;", j, j);
+        if (rpre == 0)
+            for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rv%d : 0.0f;", j, j);
         o("float out0 = 0.0f, out1 = 0.0f;");
     } else {
         o("bwd_stage<LEVEL, NTB>(D, L, ssum, Cs, b, k, live, %d, %d, %d, %d);", T.nt, T.t0, T.c0, T.ncolt);
@@ -326,6 +344,8 @@ void gen_backward_case(Src& o, const JitLevel& J, const JitConfig& cfg, int pass
         const int p = d ? stack[d - 1] : -1;
         if (kind == bt::EV_ENTER) {
             stack[d] = n;
+            if (rpre > 0 && nenter + rpre < (int)dec.size()) emit_rload(dec[nenter + rpre]);
+            ++nenter;
             emit_reach(o, T, n, p);
             emit_strategy(o, T, n);
             std::string cv;
@@ -511,6 +531,7 @@ JitConfig bt_jit_config() {
     c.fwd_bins = env("MTT_JIT_FWD_BINS", 1, 2, c.fwd_bins);
     c.fwd_nt = env("MTT_JIT_FWD_NT", 192, 576, c.fwd_nt);
     c.fwd_smask = env("MTT_JIT_FWD_SMASK", 0, 64, c.fwd_smask);
+    c.bwd_rpre = env("MTT_JIT_BWD_RPRE", 0, 64, c.bwd_rpre);
     if (c.fwd_nt % 64) c.fwd_nt = JitConfig().fwd_nt;
     return c;
 }

@@ -14,7 +14,7 @@ def S(mtt):
     return _load("mtt_solver", os.path.join(PKG, "solver.py"))
 
 
-def test_sweep_source_has_one_case_per_template(S):
+def test_sweep_source_has_one_case_per_template(S, monkeypatch):
     g, _, _ = S.config3(0)
     src = S.sweep_source(g)
     nodes, ni, _ = S.tree(g)
@@ -26,13 +26,20 @@ def test_sweep_source_has_one_case_per_template(S):
     for k in ("mtt_jit_fwd0", "mtt_jit_bwd0", "mtt_jit_fwd1", "mtt_jit_bwd1"):
         assert f"void {k}(" in src
     # every action slot of a river template is read once for its regret and once for its strategy sum,
-    # and the strategy sum is written back by a plain store (one thread owns each combo): no atomics
+    # and the strategy sum is written back by a plain store (one thread owns each combo): no atomics.
+    # By default the sums go through the bounded buffer descriptor after the walk (zero-reach lanes out
+    # of range); MTT_JIT_FWD_SMASK=0 reads them with the regrets in the prologue.
     fwd1 = src.split("void mtt_jit_fwd1(")[1].split("void mtt_jit_bwd1(")[0]
-    n_sv = len(re.findall(r"const float Sv\d+ = Sl\[", fwd1))
-    assert n_sv > 0
-    assert n_sv == len(re.findall(r"const float Rv\d+ = Rl\[", fwd1))
-    assert n_sv == len(re.findall(r"\bSg\[\d+\] = Sv\d+ \+ wr \* ", fwd1))
+    n_rv = len(re.findall(r"const float Rv\d+ = Rl\[", fwd1))
+    assert n_rv > 0
+    assert n_rv == len(re.findall(r"const float Sv\d+ = __builtin_bit_cast\(float, __builtin_amdgcn_raw_buffer_load_b32\(rsS, so\d+, ", fwd1))
+    assert n_rv == len(re.findall(r"__builtin_amdgcn_raw_buffer_store_b32\(__builtin_bit_cast\(uint32_t, Sv\d+ \+ wr\d+ \* ", fwd1))
     assert "Atomic" not in fwd1
+    monkeypatch.setenv("MTT_JIT_FWD_SMASK", "0")
+    fwd1 = S.sweep_source(g).split("void mtt_jit_fwd1(")[1].split("void mtt_jit_bwd1(")[0]
+    assert n_rv == len(re.findall(r"const float Sv\d+ = Sl\[", fwd1))
+    assert n_rv == len(re.findall(r"\bSg\[\d+\] = Sv\d+ \+ wr \* ", fwd1))
+    assert "Atomic" not in fwd1 and "rsS" not in fwd1
     assert "#pragma once" not in src
 
 
