@@ -2084,7 +2084,9 @@ int river_gemm_autotune(BatchedState& B, hipStream_t st) {
     };
     if (const char* e = getenv("MTT_RIVER_GEMM")) {
         const int v = atoi(e);
-        if (valid(v)) { B.gemm_shape = v; return MTT_OK; }
+        MTT_ARG(valid(v), "MTT_RIVER_GEMM=%s is not a river GEMM form for %d column tiles", e, ntiles);
+        B.gemm_shape = v;
+        return MTT_OK;
     }
     const std::string key = "river_gemm_shape/tiles" + std::to_string(ntiles);
     const int v = tune_lookup(key);
