@@ -1113,30 +1113,24 @@ __global__ __launch_bounds__(512) void k_bt_gemm6d(BatchedDev D, const uint32_t*
 constexpr int G7_NCH = NPR / 64;                 // K chunks of 64 villains
 static_assert(NPR % 64 == 0, "64-villain chunks");
 
-template <int GH>
+template <int GH, int NT>
 __device__ __forceinline__ void gemm7_tile(const BatchedDev& D, const uint32_t* __restrict__ S2, int b, int ct, int hg) {
-    constexpr int NTH = NPR / 32;                // hero tiles per board
+    static_assert(NT >= 1 && NT <= GH, "tile count");
     const int lane = threadIdx.x, half = lane >> 5, l32 = lane & 31;
     const int ht0 = hg * GH;
-    const int nht = __builtin_amdgcn_readfirstlane(min(GH, NTH - ht0));   // GH, or fewer in the last group
-    // when the last group lacks at most one tile only that tile needs its guard (the others then form one block)
-    constexpr bool kOneShort = NTH % GH == 0 || NTH % GH == GH - 1;
     const int ncolp = D.L[1].ncolp;
     const size_t row = ((size_t)b * ncolp + ct * 32 + l32) * NPR;
     const uint4* rh = reinterpret_cast<const uint4*>(D.Bh + row) + 4 * half;
     const uint4* rl = reinterpret_cast<const uint4*>(D.Bl + row) + 4 * half;
     // S words of hero tile t: uint2 [b][32-villain chunk][hero], chunk 2c + half
-    const uint2* sw = reinterpret_cast<const uint2*>(S2) + (size_t)b * GNCH * NPR + (size_t)half * NPR;
-    int hero[GH];
+    const uint2* sw = reinterpret_cast<const uint2*>(S2) + (size_t)b * GNCH * NPR + (size_t)half * NPR + ht0 * 32 + l32;
+    f32x16_t acc[NT];
 #pragma unroll
-    for (int t = 0; t < GH; ++t) hero[t] = min(ht0 + t, NTH - 1) * 32 + l32;   // a missing tile reads tile NTH-1
-    f32x16_t acc[GH];
-#pragma unroll
-    for (int t = 0; t < GH; ++t)
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
     uint4 ah[4], al[4], bh[4], bl[4];
-    uint2 as[GH], bs[GH];
+    uint2 as[NT], bs[NT];
 #define G7_LOAD(C_, H_, L_, S_)                                                         \
     do {                                                                                \
         const int c_ = (C_);                                                            \
@@ -1144,32 +1138,40 @@ __device__ __forceinline__ void gemm7_tile(const BatchedDev& D, const uint32_t* 
             H_[q] = rh[8 * c_ + q];                                                     \
             L_[q] = rl[8 * c_ + q];                                                     \
         }                                                                               \
-        _Pragma("unroll") for (int t = 0; t < GH; ++t) S_[t] = sw[(size_t)(2 * c_) * NPR + hero[t]]; \
+        _Pragma("unroll") for (int t = 0; t < NT; ++t) S_[t] = sw[(size_t)(2 * c_) * NPR + 32 * t]; \
     } while (0)
+    // per K step: the high-plane products as each tile's fragment is decoded,
+    // then the low-plane ones; the scheduler is asked to spread the decode
+    // VALU between the MFMAs
 #define G7_MMA(H_, L_, S_)                                                              \
     do {                                                                                \
         _Pragma("unroll") for (int ks = 0; ks < 4; ++ks) {                              \
-            bf16x8_t fh, fl, sf[GH];                                                    \
+            bf16x8_t fh, fl, sf[NT];                                                    \
             __builtin_memcpy(&fh, &H_[ks], 16);                                         \
             __builtin_memcpy(&fl, &L_[ks], 16);                                         \
-            _Pragma("unroll") for (int t = 0; t < GH; ++t) {                            \
+            _Pragma("unroll") for (int t = 0; t < NT; ++t) {                            \
                 const uint32_t w = ks < 2 ? S_[t].x : S_[t].y;                          \
                 sf[t] = s2_frag(w, ks & 1);                                             \
-                if ((kOneShort && t < GH - 1) || t < nht)                                              \
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, sf[t], acc[t], 0, 0, 0); \
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fh, sf[t], acc[t], 0, 0, 0); \
             }                                                                           \
-            _Pragma("unroll") for (int t = 0; t < GH; ++t) {                            \
-                if ((kOneShort && t < GH - 1) || t < nht)                                              \
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, sf[t], acc[t], 0, 0, 0); \
-            }                                                                           \
+            _Pragma("unroll") for (int t = 0; t < NT; ++t)                              \
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fl, sf[t], acc[t], 0, 0, 0); \
+        }                                                                               \
+        _Pragma("unroll") for (int i_ = 0; i_ < 8 * NT; ++i_) {                         \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                          \
+            __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);                          \
         }                                                                               \
     } while (0)
+    // the next chunk's loads are issued before the current chunk's products
+    // (scheduling barriers keep the compiler from sinking them to their use)
     G7_LOAD(0, ah, al, as);
     int c = 0;
     for (; c + 1 < G7_NCH; c += 2) {
         G7_LOAD(c + 1, bh, bl, bs);
+        __builtin_amdgcn_sched_barrier(0);
         G7_MMA(ah, al, as);
         G7_LOAD(min(c + 2, G7_NCH - 1), ah, al, as);
+        __builtin_amdgcn_sched_barrier(0);
         G7_MMA(bh, bl, bs);
     }
     if (c < G7_NCH) G7_MMA(ah, al, as);
@@ -1177,16 +1179,15 @@ __device__ __forceinline__ void gemm7_tile(const BatchedDev& D, const uint32_t* 
 #undef G7_MMA
     float* Cb = D.C + ((size_t)b * ncolp + ct * 32) * NPR;
 #pragma unroll
-    for (int t = 0; t < GH; ++t) {
-        if ((kOneShort && t < GH - 1) || t < nht) {
+    for (int t = 0; t < NT; ++t) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int cc = (r >> 2) * 8 + half * 4 + (r & 3);
-                Cb[(size_t)cc * NPR + (ht0 + t) * 32 + l32] = 2.0f * acc[t][r];
-            }
+        for (int r = 0; r < 16; ++r) {
+            const int cc = (r >> 2) * 8 + half * 4 + (r & 3);
+            Cb[(size_t)cc * NPR + (ht0 + t) * 32 + l32] = 2.0f * acc[t][r];
         }
     }
 }
+
 
 // The last column tile of a board when it holds at most 16 live columns
 // (262 = 8 x 32 + 6 on the benchmark game), as 16-column products
@@ -1286,8 +1287,14 @@ void k_bt_gemm7(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical, int
     if (logical >= nlogical) return;
     const int nm = nmain * NHG, pb = nm + ntail * NHGT;
     const int b = logical / pb, rem = logical % pb;
-    if (rem < nm) gemm7_tile<GH>(D, S2, b, rem / NHG, rem % NHG);
-    else gemm7t_tile<G7T_GH>(D, S2, b, nmain, rem - nm);
+    constexpr int NTL = NTH - GH * (NHG - 1);    // tiles of the last hero group
+    if (rem < nm) {
+        const int hg = rem % NHG;
+        if (NTL == GH || hg < NHG - 1) gemm7_tile<GH, GH>(D, S2, b, rem / NHG, hg);
+        else gemm7_tile<GH, NTL>(D, S2, b, rem / NHG, hg);
+    } else {
+        gemm7t_tile<G7T_GH>(D, S2, b, nmain, rem - nm);
+    }
 }
 
 // Turn all-in showdowns: C0[t][col][i] = sum_j E[t][j][i] * B0[t][col][j].
