@@ -1202,58 +1202,59 @@ __device__ __forceinline__ void gemm7_tile(const BatchedDev& D, const uint32_t* 
 // chunk, K step, high plane, low plane, whatever GH is.
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-template <int GH>
+template <int GH, int NT>
 __device__ __forceinline__ void gemm7t_tile(const BatchedDev& D, const uint32_t* __restrict__ S2, int b, int ct, int hg) {
-    constexpr int NT16 = 2 * GH;                  // 16-hero tiles per wave
-    constexpr int NT16_ALL = NPR / 16;
+    static_assert(NT >= 1 && NT <= 2 * GH, "tile count");
     const int lane = threadIdx.x, g = lane >> 4, l16 = lane & 15;
-    const int h0 = hg * NT16;
-    const int nht = __builtin_amdgcn_readfirstlane(min(NT16, NT16_ALL - h0));
+    const int h0 = hg * 2 * GH;                   // first 16-hero tile of the group
     const int ncolp = D.L[1].ncolp;
     const size_t row = ((size_t)b * ncolp + ct * 32 + l16) * NPR;
     const uint4* rh = reinterpret_cast<const uint4*>(D.Bh + row) + g;   // + 4 per 32-villain chunk
     const uint4* rl = reinterpret_cast<const uint4*>(D.Bl + row) + g;
-    const uint32_t* sw = S2 + (size_t)b * GNCH * NPR * 2 + (g >> 1);
-    int so[NT16];
+    const uint32_t* sw = S2 + (size_t)b * GNCH * NPR * 2 + (g >> 1) + (h0 * 16 + l16) * 2;
+    f32x4_t acc[NT];
 #pragma unroll
-    for (int t = 0; t < NT16; ++t) so[t] = (min(h0 + t, NT16_ALL - 1) * 16 + l16) * 2;
-    f32x4_t acc[NT16];
-#pragma unroll
-    for (int t = 0; t < NT16; ++t)
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[t][r] = 0.0f;
     uint4 ah[2], al[2], bh[2], bl[2];
-    uint32_t as[2][NT16], bs[2][NT16];
+    uint32_t as[2][NT], bs[2][NT];
 #define G7T_LOAD(C_, H_, L_, S_)                                                        \
     do {                                                                                \
         const int c_ = (C_);                                                            \
         _Pragma("unroll") for (int ks = 0; ks < 2; ++ks) {                              \
             H_[ks] = rh[4 * (2 * c_ + ks)];                                             \
             L_[ks] = rl[4 * (2 * c_ + ks)];                                             \
-            _Pragma("unroll") for (int t = 0; t < NT16; ++t)                            \
-                S_[ks][t] = sw[(size_t)(2 * c_ + ks) * NPR * 2 + so[t]];                \
+            _Pragma("unroll") for (int t = 0; t < NT; ++t)                              \
+                S_[ks][t] = sw[(size_t)(2 * c_ + ks) * NPR * 2 + 32 * t];               \
         }                                                                               \
     } while (0)
 #define G7T_MMA(H_, L_, S_)                                                             \
     do {                                                                                \
         _Pragma("unroll") for (int ks = 0; ks < 2; ++ks) {                              \
-            bf16x8_t fh, fl, sf[NT16];                                                  \
+            bf16x8_t fh, fl, sf[NT];                                                    \
             __builtin_memcpy(&fh, &H_[ks], 16);                                         \
             __builtin_memcpy(&fl, &L_[ks], 16);                                         \
-            _Pragma("unroll") for (int t = 0; t < NT16; ++t) {                          \
+            _Pragma("unroll") for (int t = 0; t < NT; ++t) {                            \
                 sf[t] = s2_frag(S_[ks][t], g & 1);                                      \
-                if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh, sf[t], acc[t], 0, 0, 0); \
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh, sf[t], acc[t], 0, 0, 0); \
             }                                                                           \
-            _Pragma("unroll") for (int t = 0; t < NT16; ++t)                            \
-                if (t < nht) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fl, sf[t], acc[t], 0, 0, 0); \
+            _Pragma("unroll") for (int t = 0; t < NT; ++t)                              \
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fl, sf[t], acc[t], 0, 0, 0); \
+        }                                                                               \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4 * NT; ++i_) {                         \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                          \
+            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);                          \
         }                                                                               \
     } while (0)
     G7T_LOAD(0, ah, al, as);
     int c = 0;
     for (; c + 1 < G7_NCH; c += 2) {
         G7T_LOAD(c + 1, bh, bl, bs);
+        __builtin_amdgcn_sched_barrier(0);
         G7T_MMA(ah, al, as);
         G7T_LOAD(min(c + 2, G7_NCH - 1), ah, al, as);
+        __builtin_amdgcn_sched_barrier(0);
         G7T_MMA(bh, bl, bs);
     }
     if (c < G7_NCH) G7T_MMA(ah, al, as);
@@ -1261,13 +1262,12 @@ __device__ __forceinline__ void gemm7t_tile(const BatchedDev& D, const uint32_t*
 #undef G7T_MMA
     float* Cb = D.C + ((size_t)b * ncolp + ct * 32) * NPR;
 #pragma unroll
-    for (int t = 0; t < NT16; ++t) {
-        if (t < nht) {
+    for (int t = 0; t < NT; ++t) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) Cb[(size_t)(4 * g + r) * NPR + (h0 + t) * 16 + l16] = 2.0f * acc[t][r];
-        }
+        for (int r = 0; r < 4; ++r) Cb[(size_t)(4 * g + r) * NPR + (h0 + t) * 16 + l16] = 2.0f * acc[t][r];
     }
 }
+
 
 // The register-streamed river GEMM launch: per board, nmain 32-column tiles
 // x NHG hero groups of GH 32-hero tiles, then, when the board's last column
@@ -1293,7 +1293,10 @@ void k_bt_gemm7(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical, int
         if (NTL == GH || hg < NHG - 1) gemm7_tile<GH, GH>(D, S2, b, rem / NHG, hg);
         else gemm7_tile<GH, NTL>(D, S2, b, rem / NHG, hg);
     } else {
-        gemm7t_tile<G7T_GH>(D, S2, b, nmain, rem - nm);
+        constexpr int NT16 = NPR / 16, NTLT = NT16 - 2 * G7T_GH * (NHGT - 1);   // 16-hero tiles of the last group
+        const int hg = rem - nm;
+        if (NTLT == 2 * G7T_GH || hg < NHGT - 1) gemm7t_tile<G7T_GH, 2 * G7T_GH>(D, S2, b, nmain, hg);
+        else gemm7t_tile<G7T_GH, NTLT>(D, S2, b, nmain, hg);
     }
 }
 
