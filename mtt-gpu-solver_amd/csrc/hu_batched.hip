@@ -520,15 +520,16 @@ __global__ void k_bt_s2(BatchedDev D, uint32_t* __restrict__ S2) {
 // The eight codes of K step k of an S word to the bf16 B-fragment of S/2:
 // per output dword (elements 2m, 2m+1) the two codes are moved to bits 8 and
 // 24, which makes them the byte selectors of one v_perm over the table bytes
-// 0x00, 0x3F, 0xBF (the high byte of bf16 0, +1/2, -1/2); selector 0x0C
-// gives the zero low bytes.
+// 0x00, 0x3F, 0xBF (the high byte of bf16 0, +1/2, -1/2); the masked-off
+// selector bytes 0 and 2 are 0, which selects the table's zero byte 0 for
+// the low bytes.  One shift (none for m = 0 of step 0), one AND, one perm.
 __device__ __forceinline__ bf16x8_t s2_frag(uint32_t w, int k) {
     constexpr uint32_t T = 0x00BF3F00u;
     uint32_t o[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         const uint32_t x = k ? (w << (8 - 2 * m)) : (w >> (2 * m));
-        o[m] = __builtin_amdgcn_perm(0u, T, (x & 0x03000300u) | 0x000C000Cu);
+        o[m] = __builtin_amdgcn_perm(0u, T, x & 0x03000300u);
     }
     bf16x8_t r;
     __builtin_memcpy(&r, o, 16);
