@@ -68,6 +68,10 @@ struct JitConfig {
     // river backward regrets: 0 = all loaded in the prologue; k > 0 = each decision's loaded k decisions
     // ahead of its entry in walk order
     int bwd_rpre = 0;
+    // river backward regrets of the next slice: 1 (the default) = loaded into a second register set while
+    // the current slice walks (with all regrets in the prologue, bwd_rpre = 0; 229 VGPRs, still two
+    // workgroups per CU): river backward 2.32 -> 2.26 ms; 0 = each slice loads its own in its prologue
+    int bwd_xpre = 1;
 };
 
 // Passes of the backward of a level (0: turn, 1: river) under cfg.

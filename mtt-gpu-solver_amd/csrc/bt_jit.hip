@@ -282,6 +282,8 @@ void gen_backward_case(Src& o, const JitLevel& J, const JitConfig& cfg, int pass
     // walk order (the first rpre in the prologue) instead of all in the prologue, so only the regrets of
     // the open decisions and of the next few stay live
     const int rpre = passes > 1 ? cfg.bwd_rpre : 0;
+    // xpre: the next slice's regrets are loaded into Rn* while the current slice walks
+    const bool xpre = passes > 1 && rpre == 0 && cfg.bwd_xpre;
     std::vector<int> dec;
     for (int e = T.e0; e < T.e1; ++e)
         if (((*J.events)[e] & 3) == bt::EV_ENTER) dec.push_back((*J.events)[e] >> 2);
@@ -298,6 +300,16 @@ void gen_backward_case(Src& o, const JitLevel& J, const JitConfig& cfg, int pass
         // in LDS are its own, so the passes need no barrier)
         if (T.nt > 0) o("bwd_sums_n<NTB, %d, %d>(L, ssum, b, %d);", T.nt, fwd_sliced(J.level, cfg) ? 1 : 2, T.t0);
         o("bwd_totals<NTB>(ssum, %d);", T.nt);
+        if (xpre) {
+            // the first slice's regrets (clamped combo index: every lane loads from a valid address)
+            std::string decl;
+            for (int j = 0; j < T.nsl; ++j) decl += (j ? ", Rn" : "Rn") + std::to_string(j);
+            if (T.nsl > 0) o("float %s;", decl.c_str());
+            o("{");
+            o("const float* __restrict__ Rp = L.R + ((size_t)b * L.nslot + %d) * NP + min(tid, NP - 1);", T.s0);
+            for (int j = 0; j < T.nsl; ++j) o("Rn%d = Rp[%d];", j, j * J.np);
+            o("}");
+        }
         o("#pragma unroll 1");  // one walk's registers at a time
         o("for (int pass = 0; pass < %d; ++pass) {", passes);
         // an offset the compiler cannot see is zero: the card-sum reads are not hoisted out of the pass loop
@@ -316,13 +328,28 @@ void gen_backward_case(Src& o, const JitLevel& J, const JitConfig& cfg, int pass
         o("const int c1r = L.cc[((size_t)b * NP + kc) * 2], c2r = L.cc[((size_t)b * NP + kc) * 2 + 1];");
         o("const int ri = root_index(D, LEVEL, b, kc);");
         o("const float* __restrict__ Rl = L.R + ((size_t)b * L.nslot + %d) * NP + kc;", T.s0);
-        if (rpre == 0)
+        if (xpre)
+            for (int j = 0; j < T.nsl; ++j) o("const float Rv%d = Rn%d;", j, j);
+        else if (rpre == 0)
             for (int j = 0; j < T.nsl; ++j) o("const float Rv%d = Rl[%d];", j, j * J.np);
         else
             for (int i = 0; i < std::min(rpre, (int)dec.size()); ++i) emit_rload(dec[i]);
         if (T.ncolt > 0) o("bwd_cs_n<LEVEL, NTB, %d>(D, L, Cs, b, kc, live, %d);", T.ncolt, T.c0);
         o("const float rr0 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 0);");
         o("const float rr1 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 1);");
+        if (xpre && T.nsl > 0) {
+            // the next slice's regrets, in flight during this slice's walk; the scheduling barriers keep them
+            // behind this slice's loads (whose waits would otherwise cover them) and keep the compiler from
+            // sinking them towards their use in the next pass.  Unconditional (a branch around them would
+            // make the waits after the join count them): in the last pass every lane reads the clamped last
+            // combo, one cache line per slot, and the values go unused
+            o("__builtin_amdgcn_sched_barrier(0);");
+            o("{");
+            o("const float* __restrict__ Rp = L.R + ((size_t)b * L.nslot + %d) * NP + min(k + NTB, NP - 1);", T.s0);
+            for (int j = 0; j < T.nsl; ++j) o("Rn%d = Rp[%d];", j, j * J.np);
+            o("}");
+            o("__builtin_amdgcn_sched_barrier(0);");
+        }
         o("const int c1 = live ? c1r : 0, c2 = live ? c2r : 0;");
         o("const float root0 = live ? rr0 : 0.0f, root1 = live ? rr1 : 0.0f;");
         o("float* __restrict__ Rg = L.R + ((size_t)b * L.nslot + %d) * NP + k;", T.s0);
@@ -532,6 +559,7 @@ JitConfig bt_jit_config() {
     c.fwd_nt = env("MTT_JIT_FWD_NT", 192, 576, c.fwd_nt);
     c.fwd_smask = env("MTT_JIT_FWD_SMASK", 0, 64, c.fwd_smask);
     c.bwd_rpre = env("MTT_JIT_BWD_RPRE", 0, 64, c.bwd_rpre);
+    c.bwd_xpre = env("MTT_JIT_BWD_XPRE", 0, 1, c.bwd_xpre);
     if (c.fwd_nt % 64) c.fwd_nt = JitConfig().fwd_nt;
     return c;
 }
