@@ -72,6 +72,9 @@ struct JitConfig {
     // the current slice walks (with all regrets in the prologue, bwd_rpre = 0; 229 VGPRs, still two
     // workgroups per CU): river backward 2.32 -> 2.26 ms; 0 = each slice loads its own in its prologue
     int bwd_xpre = 1;
+    // river forward regrets of the next slice, the same way: 1 (the default) = in flight during the current
+    // slice's walk (158 VGPRs, still two 384-thread workgroups per CU): river forward 3.13 -> 3.07 ms
+    int fwd_xpre = 1;
 };
 
 // Passes of the backward of a level (0: turn, 1: river) under cfg.

@@ -125,8 +125,9 @@ void emit_bins(Src& o, const Tmpl& T, int n) {
     o("}");
 }
 
-void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int slices, int smask, int t) {
+void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int slices, int smask, bool xpre, int t) {
     Tmpl T(J, t);
+    xpre = xpre && slices > 1 && T.nsl > 0;
     o("case %d: {", t);
     o("fwd_bins_clear<NT>(bins, %d);", T.nt * nbins);
     if (nbins == 2) o("unsigned long long* const binl = bins + (threadIdx.x & 1) * %d;", T.nt * 104);
@@ -134,6 +135,17 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int sl
     if (slices > 1) {
         // one workgroup walks the combos in slices, adding every slice into the same bins
         o("__syncthreads();");
+        if (xpre) {
+            // xpre: the first slice's regrets here, each later slice's loaded during the walk before it
+            std::string decl;
+            for (int j = 0; j < T.nsl; ++j) decl += (j ? ", Rn" : "Rn") + std::to_string(j);
+            o("float %s;", decl.c_str());
+            o("{");
+            o("const float* __restrict__ Rp = L.R + ((size_t)b * L.nslot + %d) * NP + min((int)threadIdx.x, NP - 1);",
+              T.s0);
+            for (int j = 0; j < T.nsl; ++j) o("Rn%d = Rp[%d];", j, j * J.np);
+            o("}");
+        }
         o("#pragma unroll 1");
         o("for (int pass = 0; pass < %d; ++pass) {", slices);
         o("if (__builtin_amdgcn_readfirstlane(pass * NT + ((int)threadIdx.x & ~63)) >= NP) continue;");
@@ -147,7 +159,10 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int sl
     o("float* __restrict__ Sg = L.S + rb;");
     o("const int c1r = L.cc[((size_t)b * NP + kc) * 2], c2r = L.cc[((size_t)b * NP + kc) * 2 + 1];");
     o("const int ri = root_index(D, LEVEL, b, kc);");
-    for (int j = 0; j < T.nsl; ++j) o("const float Rv%d = Rl[%d];", j, j * J.np);
+    if (xpre)
+        for (int j = 0; j < T.nsl; ++j) o("const float Rv%d = Rn%d;", j, j);
+    else
+        for (int j = 0; j < T.nsl; ++j) o("const float Rv%d = Rl[%d];", j, j * J.np);
     // the strategy sums are updated by plain loads and stores: each (slot, combo) has one writer per
     // iteration, and a float atomic per entry cost the river forward ~22% of its time.  Either they are
     // read with the regrets, or (smask) each decision reads its own through a bounded buffer descriptor
@@ -163,6 +178,16 @@ void gen_forward_case(Src& o, const JitLevel& J, bool sd_fp32, int nbins, int sl
     }
     o("const float rr0 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 0);");
     o("const float rr1 = root_reach_at(D, ranges, LEVEL, b, tm, NP, ri, 1);");
+    if (xpre) {
+        // the next slice's regrets, behind this slice's loads and pinned there (as in the backward);
+        // unconditional, the last slice's reading the clamped last combo
+        o("__builtin_amdgcn_sched_barrier(0);");
+        o("{");
+        o("const float* __restrict__ Rp = L.R + ((size_t)b * L.nslot + %d) * NP + min(k + NT, NP - 1);", T.s0);
+        for (int j = 0; j < T.nsl; ++j) o("Rn%d = Rp[%d];", j, j * J.np);
+        o("}");
+        o("__builtin_amdgcn_sched_barrier(0);");
+    }
     o("const float root0 = live ? rr0 : 0.0f, root1 = live ? rr1 : 0.0f;");
     o("const int c1 = live ? c1r : 0, c2 = live ? c2r : 0;");
     for (int j = 0; j < T.nsl; ++j) o("const float R%d = live ? Rv%d : 0.0f;", j, j);
@@ -473,7 +498,8 @@ void gen_level(Src& o, const JitLevel& J, const JitConfig& cfg) {
         o("(void)inb;");
     }
     o("switch (tm) {");
-    for (int t = 0; t < J.ntmpl; ++t) gen_forward_case(o, J, cfg.sd_fp32 != 0, cfg.fwd_bins, fsl, J.level == 1 ? cfg.fwd_smask : 0, t);
+    for (int t = 0; t < J.ntmpl; ++t) gen_forward_case(o, J, cfg.sd_fp32 != 0, cfg.fwd_bins, fsl, J.level == 1 ? cfg.fwd_smask : 0,
+                                                    J.level == 1 && cfg.fwd_xpre != 0, t);
     o("default: break;");
     o("}");
     o("}");
@@ -560,6 +586,7 @@ JitConfig bt_jit_config() {
     c.fwd_smask = env("MTT_JIT_FWD_SMASK", 0, 64, c.fwd_smask);
     c.bwd_rpre = env("MTT_JIT_BWD_RPRE", 0, 64, c.bwd_rpre);
     c.bwd_xpre = env("MTT_JIT_BWD_XPRE", 0, 1, c.bwd_xpre);
+    c.fwd_xpre = env("MTT_JIT_FWD_XPRE", 0, 1, c.fwd_xpre);
     if (c.fwd_nt % 64) c.fwd_nt = JitConfig().fwd_nt;
     return c;
 }
