@@ -29,9 +29,18 @@ def test_sweep_source_has_one_case_per_template(S, monkeypatch):
     # and the strategy sum is written back by a plain store (one thread owns each combo): no atomics.
     # By default the sums go through the bounded buffer descriptor after the walk (zero-reach lanes out
     # of range); MTT_JIT_FWD_SMASK=0 reads them with the regrets in the prologue.
+    # By default each slice's regrets come from the register set the previous slice loaded (Rn*: the
+    # first slice's before the slice loop, each later slice's during the walk before it, so every slot
+    # has two such loads); MTT_JIT_FWD_XPRE=0 has each slice load its own (Rl).
     fwd1 = src.split("void mtt_jit_fwd1(")[1].split("void mtt_jit_bwd1(")[0]
-    n_rv = len(re.findall(r"const float Rv\d+ = Rl\[", fwd1))
+    n_rv = len(re.findall(r"const float Rv(\d+) = Rn\1;", fwd1))
     assert n_rv > 0
+    assert 2 * n_rv == len(re.findall(r"\bRn\d+ = Rp\[", fwd1))
+    assert not re.search(r"const float Rv\d+ = Rl\[", fwd1)
+    monkeypatch.setenv("MTT_JIT_FWD_XPRE", "0")
+    fwd1_own = S.sweep_source(g).split("void mtt_jit_fwd1(")[1].split("void mtt_jit_bwd1(")[0]
+    assert n_rv == len(re.findall(r"const float Rv\d+ = Rl\[", fwd1_own)) and "Rn0" not in fwd1_own
+    monkeypatch.delenv("MTT_JIT_FWD_XPRE")
     assert n_rv == len(re.findall(r"const float Sv\d+ = __builtin_bit_cast\(float, __builtin_amdgcn_raw_buffer_load_b32\(rsS, so\d+, ", fwd1))
     assert n_rv == len(re.findall(r"__builtin_amdgcn_raw_buffer_store_b32\(__builtin_bit_cast\(uint32_t, Sv\d+ \+ wr\d+ \* ", fwd1))
     assert "Atomic" not in fwd1
