@@ -1164,16 +1164,22 @@ __device__ __forceinline__ void gemm7_tile(const BatchedDev& D, const uint32_t* 
         }                                                                               \
     } while (0)
     // the next chunk's loads are issued before the current chunk's products
-    // (scheduling barriers keep the compiler from sinking them to their use)
+    // (scheduling barriers keep the compiler from sinking them to their use);
+    // a wave raises its issue priority for its products, so of a SIMD's two
+    // waves the one with MFMA work ready is served first
     G7_LOAD(0, ah, al, as);
     int c = 0;
     for (; c + 1 < G7_NCH; c += 2) {
         G7_LOAD(c + 1, bh, bl, bs);
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(1);
         G7_MMA(ah, al, as);
+        __builtin_amdgcn_s_setprio(0);
         G7_LOAD(min(c + 2, G7_NCH - 1), ah, al, as);
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(1);
         G7_MMA(bh, bl, bs);
+        __builtin_amdgcn_s_setprio(0);
     }
     if (c < G7_NCH) G7_MMA(ah, al, as);
 #undef G7_LOAD
@@ -1253,10 +1259,14 @@ __device__ __forceinline__ void gemm7t_tile(const BatchedDev& D, const uint32_t*
     for (; c + 1 < G7_NCH; c += 2) {
         G7T_LOAD(c + 1, bh, bl, bs);
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(1);
         G7T_MMA(ah, al, as);
+        __builtin_amdgcn_s_setprio(0);
         G7T_LOAD(min(c + 2, G7_NCH - 1), ah, al, as);
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(1);
         G7T_MMA(bh, bl, bs);
+        __builtin_amdgcn_s_setprio(0);
     }
     if (c < G7_NCH) G7T_MMA(ah, al, as);
 #undef G7T_LOAD
@@ -2517,11 +2527,17 @@ __device__ __forceinline__ void gemm_flop7_wave(const BatchedDev& D, const int16
         int c = 0;
         for (; c + 2 < FNCH; c += 3) {
             GF7_LOAD(c + 2, ch, cl, cs);
+            __builtin_amdgcn_s_setprio(1);
             GF7_MMA(ah, al, as);
+            __builtin_amdgcn_s_setprio(0);
             GF7_LOAD(min(c + 3, LAST), ah, al, as);
+            __builtin_amdgcn_s_setprio(1);
             GF7_MMA(bh, bl, bs);
+            __builtin_amdgcn_s_setprio(0);
             GF7_LOAD(min(c + 4, LAST), bh, bl, bs);
+            __builtin_amdgcn_s_setprio(1);
             GF7_MMA(ch, cl, cs);
+            __builtin_amdgcn_s_setprio(0);
         }
         if (c < FNCH) GF7_MMA(ah, al, as);
         if (c + 1 < FNCH) GF7_MMA(bh, bl, bs);
