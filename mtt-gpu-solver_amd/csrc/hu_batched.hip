@@ -1287,8 +1287,11 @@ __device__ __forceinline__ void gemm7t_tile(const BatchedDev& D, const uint32_t*
 // all of a board's items on one XCD, the tail items last.
 constexpr int G7T_GH = 5;   // 10 16-hero tiles per tail wave: 206 VGPRs, no spill
 
-template <int GH>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+// WPE: waves per SIMD the register budget is sized for (2: up to 256 registers;
+// 1: up to 512, the accumulators of 12 hero tiles in AGPRs, 3 hero groups per
+// column tile instead of 5, so each reach chunk feeds more products).
+template <int GH, int WPE = 2>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_bt_gemm7(BatchedDev D, const uint32_t* __restrict__ S2, int nlogical, int nmain, int ntail) {
     constexpr int NTH = NPR / 32;
     constexpr int NHG = (NTH + GH - 1) / GH;
@@ -2017,7 +2020,8 @@ int batched_create(SolverImpl& s) {
 // the hero-major form reading S at two bits per pair (k_bt_gemm6s), 102
 // for the latter with the reach fetched two chunks ahead (k_bt_gemm6d), and
 // 700 + GH for the register-streamed single-wave form with GH hero tiles
-// per wave (k_bt_gemm7, GH = 5, 6, 7).
+// per wave (k_bt_gemm7, GH = 5, 6, 7 at two waves per SIMD; 12 at one wave per
+// SIMD, its accumulators in AGPRs).
 int river_gemm(const BatchedState& B, int shape, hipStream_t st) {
     const int ntiles = B.D.L[1].ncolp / 32;
     if (shape >= 700 && shape < 800) {
@@ -2031,6 +2035,7 @@ int river_gemm(const BatchedState& B, int shape, hipStream_t st) {
         case 5: hipLaunchKernelGGL(k_bt_gemm7<5>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, nmain, ntail); break;
         case 6: hipLaunchKernelGGL(k_bt_gemm7<6>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, nmain, ntail); break;
         case 7: hipLaunchKernelGGL(k_bt_gemm7<7>, g, dim3(64), 0, st, B.D, B.d_S2, nlog, nmain, ntail); break;
+        case 12: hipLaunchKernelGGL((k_bt_gemm7<12, 1>), g, dim3(64), 0, st, B.D, B.d_S2, nlog, nmain, ntail); break;
         default: set_error("river GEMM shape %d not available", shape); return MTT_ERR_ARG;
         }
         MTT_HIP(hipGetLastError());
@@ -2100,7 +2105,7 @@ int river_gemm(const BatchedState& B, int shape, hipStream_t st) {
 int river_gemm_autotune(BatchedState& B, hipStream_t st) {
     const int ntiles = B.D.L[1].ncolp / 32;
     auto valid = [&](int v) {
-        return (v >= 100 && v <= 102) || v == 705 || v == 706 || v == 707 ||
+        return (v >= 100 && v <= 102) || v == 705 || v == 706 || v == 707 || v == 712 ||
                (v / 10 >= 1 && v / 10 <= std::min(ntiles, 9) && v % 10 >= 2 && v % 10 <= 3);
     };
     if (const char* e = getenv("MTT_RIVER_GEMM")) {
@@ -2112,7 +2117,7 @@ int river_gemm_autotune(BatchedState& B, hipStream_t st) {
     const std::string key = "river_gemm_shape/tiles" + std::to_string(ntiles);
     const int v = tune_lookup(key);
     if (valid(v)) { B.gemm_shape = v; return MTT_OK; }
-    std::vector<int> cand{707, 706, 705, 102, 101, 100};
+    std::vector<int> cand{707, 712, 706, 705, 102, 101, 100};
     for (int d = 1; d <= 3; ++d) {
         const int nw = (ntiles + d - 1) / d;
         for (int pd = 2; pd <= 3; ++pd)

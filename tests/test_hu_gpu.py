@@ -481,13 +481,13 @@ def test_river_gemm_forms_agree(solver_api, torch_dev, which, monkeypatch):
             ref = (R, Sm)
         else:
             assert np.array_equal(R, ref[0]) and np.array_equal(Sm, ref[1]), shape
-    # the register-streamed forms (5, 6 or 7 hero tiles per wave) share one
+    # the register-streamed forms (5, 6, 7 or 12 hero tiles per wave) share one
     # accumulation order of their own (64-villain chunks): identical to each
     # other after 3 iterations, and after one iteration (uniform strategies,
     # so the same reach goes into both products) within FP32 rounding of the
     # forms above
     ref7 = None
-    for shape in ("707", "706", "705"):
+    for shape in ("707", "706", "705", "712"):
         monkeypatch.setenv("MTT_RIVER_GEMM", shape)
         s = S.Solver(g, r0, r1, S.MODE_BF16_SPLIT)
         s.run(3)
